@@ -1,0 +1,236 @@
+// Shared device helpers for the hbtx kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HBTX_WAVE 64
+#define HBTX_DEV __device__ __forceinline__
+#define HBTX_HD __host__ __device__ __forceinline__
+
+// Number of octree levels encoded in a 63-bit key (3 bits per level).
+#define HBTX_KEY_LEVELS 21
+
+#define GRID_STRIDE(i, n) for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+namespace hbtx
+{
+
+// Subhalo status values
+enum : int32_t
+{
+  ST_ACTIVE = 0,
+  ST_CONVERGED = 1,
+  ST_DISRUPTED = 2
+};
+// Per-round mode of a subhalo
+enum : int32_t
+{
+  MODE_IDLE = 0,
+  MODE_FULL = 1,  // all current candidates are targets, sources = candidates (or subsample)
+  MODE_CORR = 2   // correction: sources = particles removed in the previous round
+};
+
+// Device copy of the unbinding parameters (derived values precomputed on the host
+// exactly as the reference derives them, src/config_parser.cpp:200-212,641-649).
+struct DevParams
+{
+  double G_over_a;     // PhysicalConst::G / ScaleFactor
+  double res;          // TreeNodeResolution (0.1 SofteningHalo, float -> double)
+  float eps;           // GetCurrentSoftening(a)
+  float h;             // 2.8 eps
+  float hinv;          // 1 / h
+  float h2;            // h^2
+  float theta2;        // TreeNodeOpenAngleSquare (float product)
+  float a;             // ScaleFactor
+  float hz;            // Hz
+  float box;           // BoxSize
+  float box_half;      // BoxHalf
+  float prec;          // BoundMassPrecision
+  float core_factor;   // SubCoreSizeFactor
+  float refine_frac;   // BoundFractionCenterRefinement
+  int32_t periodic;
+  int32_t thermal;
+  int32_t dm_only;
+  int32_t tracer_mask;
+  int32_t donot_mask;
+  int32_t per_type;
+  int32_t refine;
+  int64_t mss;         // MaxSampleSizeOfPotentialEstimate
+  int64_t min_part;
+  int64_t min_tracer;
+  int64_t core_min;
+  int64_t direct_cutoff;
+};
+
+// Loads through the constant address space: a wave-uniform index then becomes a
+// scalar (s_load) access.  Only for buffers that no kernel writes while it reads them.
+HBTX_DEV float4 const_load(const float4 *p, int i)
+{
+  typedef const __attribute__((address_space(4))) float cf;
+  const cf *q = (const cf *)(uintptr_t)(p + i);
+  return make_float4(q[0], q[1], q[2], q[3]);
+}
+HBTX_DEV int2 const_load(const int2 *p, int i)
+{
+  typedef const __attribute__((address_space(4))) int ci;
+  const ci *q = (const ci *)(uintptr_t)(p + i);
+  return make_int2(q[0], q[1]);
+}
+
+HBTX_DEV uint64_t ballot(bool p)
+{
+  return __ballot(p);
+}
+
+HBTX_DEV int lane_id()
+{
+  return __lane_id();
+}
+
+HBTX_DEV int uniform(int v)
+{
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+HBTX_DEV uint64_t lanemask_lt()
+{
+  const int l = lane_id();
+  return (l == 0) ? 0ull : (~0ull >> (64 - l));
+}
+
+// float <-> uint32 with the float ordering preserved
+HBTX_HD uint32_t float_to_ordered(float f)
+{
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+HBTX_HD float ordered_to_float(uint32_t u)
+{
+  uint32_t v = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+  return __uint_as_float(v);
+}
+
+// NEAREST, src/config_parser.h:218-219
+HBTX_DEV float nearest_f(float x, float box, float half)
+{
+  return (x > half) ? (x - box) : ((x < -half) ? (x + box) : x);
+}
+HBTX_DEV double nearest_d(double x, double box, double half)
+{
+  return (x > half) ? (x - box) : ((x < -half) ? (x + box) : x);
+}
+
+HBTX_DEV int is_tracer(const DevParams &P, int type)
+{
+  return P.dm_only ? 1 : ((1 << type) & P.tracer_mask);
+}
+HBTX_DEV int donot_subsample(const DevParams &P, int type)
+{
+  return P.dm_only ? 0 : ((1 << type) & P.donot_mask);
+}
+
+// 63-bit octree key of a point in the cube (centre, lenhalf): the sequence of
+// child indices (bit0 x, bit1 y, bit2 z) that the reference insertion computes
+// level by level in double precision (src/oct_tree.tpp:377-410).
+HBTX_HD uint64_t octree_key(float x, float y, float z, double cx, double cy, double cz, double lenhalf)
+{
+  uint64_t key = 0;
+  double lh = lenhalf;
+  const double px = x, py = y, pz = z;
+#pragma unroll 3
+  for (int l = 0; l < HBTX_KEY_LEVELS; l++)
+  {
+    lh *= 0.5;
+    uint64_t sub = 0;
+    if (px > cx) { cx += lh; sub |= 1; } else { cx -= lh; }
+    if (py > cy) { cy += lh; sub |= 2; } else { cy -= lh; }
+    if (pz > cz) { cz += lh; sub |= 4; } else { cz -= lh; }
+    key = (key << 3) | sub;
+  }
+  return key;
+}
+
+// Depth below which the reference stops placing particles geometrically: a new
+// cell of half-size lenhalf < TreeNodeResolutionHalf sends the displaced particle
+// to a random child (src/oct_tree.tpp:426-438).  Returns the last geometric level.
+HBTX_HD int resolution_depth(double lenhalf, double res_half)
+{
+  int d = 1;
+  double lh = lenhalf * 0.5;
+  while (d < HBTX_KEY_LEVELS && !(lh < res_half))
+  {
+    lh *= 0.5;
+    d++;
+  }
+  return d;
+}
+
+// Levels deeper than the resolution depth take digits from a hash of the
+// particle's index inside its subhalo: deterministic, and like the reference's
+// random placement they separate coincident particles within a few levels.
+HBTX_HD uint64_t hash_digits(uint64_t key, int dcut, uint32_t local_index)
+{
+  if (dcut >= HBTX_KEY_LEVELS)
+    return key;
+  uint64_t h = (uint64_t)local_index * 0x9E3779B97F4A7C15ull;
+  h ^= h >> 29;
+  h *= 0xBF58476D1CE4E5B9ull;
+  h ^= h >> 32;
+  const uint64_t mask = (1ull << (3 * (HBTX_KEY_LEVELS - dcut))) - 1ull;
+  return (key & ~mask) | (h & mask);
+}
+
+// Number of leading octree levels two keys share (0..21).
+HBTX_HD int common_levels(uint64_t a, uint64_t b)
+{
+  uint64_t x = a ^ b;
+  if (x == 0)
+    return HBTX_KEY_LEVELS;
+#ifdef __HIP_DEVICE_COMPILE__
+  int lz = __clzll((long long)x);
+#else
+  int lz = __builtin_clzll(x);
+#endif
+  return (lz - 1) / 3;
+}
+
+// Prefix of a key at depth l (l levels kept).
+HBTX_HD uint64_t key_prefix(uint64_t k, int l)
+{
+  return (l == 0) ? 0ull : (k >> (3 * (HBTX_KEY_LEVELS - l)));
+}
+
+// Softened pair potential term -m * W(r) / ... in units where the final sum is
+// multiplied by G/a: r >= h gives -m/r; otherwise the GADGET-2 cubic-spline
+// polynomial of src/gravity_tree.cpp:150-162.
+HBTX_DEV float spline_near(float r2, float m, float hinv)
+{
+  float r = __fsqrt_rn(r2);
+  float u = r * hinv;
+  float wp;
+  if (u < 0.5f)
+    wp = -2.8f + u * u * (5.333333333333f + u * u * (6.4f * u - 9.6f));
+  else
+    wp = -3.2f + 0.066666666667f / u + u * u * (10.666666666667f + u * (-16.0f + u * (9.6f - 2.133333333333f * u)));
+  return m * hinv * wp;
+}
+
+// Kahan-compensated accumulation of FP32 terms.
+struct Kahan
+{
+  float s, c;
+  HBTX_DEV void init()
+  {
+    s = 0.f;
+    c = 0.f;
+  }
+  HBTX_DEV void add(float x)
+  {
+    float y = x - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+} // namespace hbtx

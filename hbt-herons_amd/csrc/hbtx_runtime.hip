@@ -1,0 +1,1430 @@
+// hbtx runtime: device buffers, the round pipeline, and the C-ABI entry points
+// declared in include/hbtx.h.  All kernels live in the headers included below
+// and are launched on the context's stream.
+#include "hbtx.h"
+
+#include "common.h"
+#include "scan.h"
+#include "sort.h"
+#include "tree.h"
+#include "walk.h"
+#include "unbind.h"
+#include "points.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace hbtx;
+
+namespace
+{
+
+struct Error : public std::runtime_error
+{
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                                    \
+  do                                                                                                                 \
+  {                                                                                                                  \
+    hipError_t e_ = (x);                                                                                             \
+    if (e_ != hipSuccess)                                                                                            \
+      throw Error(HBTX_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + ":" +            \
+                                  std::to_string(__LINE__) + ")");                                                   \
+  } while (0)
+
+#define REQUIRE(cond, code, msg)                                                                                     \
+  do                                                                                                                 \
+  {                                                                                                                  \
+    if (!(cond))                                                                                                     \
+      throw Error(code, msg);                                                                                        \
+  } while (0)
+
+template <class T>
+struct DArr
+{
+  T *p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count)
+  {
+    if (p && count <= n)
+      return;
+    release();
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void **)&p, bytes);
+    if (e != hipSuccess)
+    {
+      p = nullptr;
+      n = 0;
+      throw Error(HBTX_E_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    n = count;
+  }
+  void release()
+  {
+    if (p)
+      (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DArr()
+  {
+    release();
+  }
+  DArr() = default;
+  DArr(const DArr &) = delete;
+  DArr &operator=(const DArr &) = delete;
+};
+
+inline int blocks_for(int64_t n, int threads = 256, int cap = 16384)
+{
+  int64_t b = (n + threads - 1) / threads;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, cap));
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Batch state
+// ---------------------------------------------------------------------------
+struct Batch
+{
+  int64_t n = 0;
+  int32_t nsub = 0;
+  int32_t max_count = 0;
+  bool has_type = false;
+  bool has_u = false;
+  bool has_proposal = false;
+  std::vector<int64_t> h_off;
+  std::vector<int64_t> h_id;
+  std::vector<int32_t> h_count;
+  // particles
+  DArr<float4> posm, velu;
+  DArr<int8_t> ptype;
+  DArr<int32_t> sub_of, sub_zero, off, off_pairs, order, order_zero;
+  DArr<int32_t> state, cur, cur2, src, fvals, ford;
+  DArr<float> energy, selfm, fenergy, fpot;
+  DArr<uint8_t> proposal;
+  DArr<int32_t> flag, fscan, crank, cnt, cscan;
+  DArr<int64_t> flag64, fscan64;
+  DArr<uint64_t> keys;
+  DArr<float4> ent_pm;
+  DArr<int2> ent_aux;
+  DArr<int8_t> ent_depth;
+  int32_t ent_cap = 0;
+  DArr<int2> items;
+  DArr<int32_t> icnt, iscan;
+  // subhaloes
+  DArr<int32_t> status, corr, iters, mode, treemode, nlast, nbound, nrem, nuns, ntgt, nsrc, src_off, ent_begin,
+    ent_end, nitems, bnd_cnt, bnd_tr, first_idx, seg_begin, seg_len, rbegin, rlen, type_cnt, first_tracer;
+  DArr<float> mlast, mbound, ref, oldref, mb, upscale, ref_in, mb_in;
+  DArr<double> cube, len0, red, red2;
+  DArr<uint32_t> bbox;
+  DArr<int32_t> misc; // [0] active, [1] work counter, [2] overflow, [3] entry total
+  DArr<hbtx_sub_result> dres;
+  bool ran = false;
+};
+
+struct View
+{
+  int64_t n;
+  int32_t nsub;
+  SubArrays S;
+  PartArrays Q;
+  const int32_t *order;
+  int64_t p0; // first particle of the view in the batch
+  int32_t s0; // first subhalo of the view
+  int32_t *ford;
+  float *fenergy, *fpot;
+  int32_t *fvals;
+  hbtx_sub_result *dres;
+  double *red, *red2;
+  int32_t *type_cnt, *first_tracer;
+};
+
+struct hbtx_tree
+{
+  hbtx_ctx *ctx = nullptr;
+  int64_t n = 0;
+  int32_t ne = 0;
+  int64_t ncells = 0;
+  DevParams P;
+  DArr<float4> ent_pm;
+  DArr<int2> ent_aux;
+  DArr<int8_t> ent_depth;
+  DArr<int32_t> src_of_entry;
+  DArr<double> cube;
+  int32_t treemode = 1;
+};
+
+struct hbtx_ctx
+{
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int num_cu = 256;
+  // scratch
+  DArr<int32_t> part32;
+  DArr<int64_t> part64;
+  DArr<uint64_t> kalt;
+  DArr<int32_t> valt, sort_ntile, sort_toff, ghist;
+  DArr<int4> sort_tiles;
+  DArr<int32_t> red_ntile, red_toff;
+  DArr<int4> red_tiles;
+  DArr<double> red_partial;
+  DArr<unsigned long long> stats;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, wev0 = nullptr, wev1 = nullptr;
+  double walk_ms = 0.0;
+  double stage_ms[8] = {0};
+  int rounds = 0;
+  Batch B;
+};
+
+namespace
+{
+
+// ---------------------------------------------------------------------------
+// primitives
+// ---------------------------------------------------------------------------
+template <class T>
+void scan_excl(hbtx_ctx *c, const T *in, T *out, int64_t n, DArr<T> &part)
+{
+  const int64_t nb = std::max<int64_t>(1, (n + SCAN_TILE - 1) / SCAN_TILE);
+  part.alloc(nb);
+  k_scan_reduce<T><<<(unsigned)nb, SCAN_THREADS, 0, c->stream>>>(in, n, part.p);
+  k_scan_partials<T><<<1, SCAN_THREADS, 0, c->stream>>>(part.p, nb);
+  k_scan_down<T><<<(unsigned)nb, SCAN_THREADS, 0, c->stream>>>(in, out, n, part.p);
+  HIPCHK(hipGetLastError());
+}
+
+void scan32(hbtx_ctx *c, const int32_t *in, int32_t *out, int64_t n)
+{
+  scan_excl<int32_t>(c, in, out, n, c->part32);
+}
+void scan64(hbtx_ctx *c, const int64_t *in, int64_t *out, int64_t n)
+{
+  scan_excl<int64_t>(c, in, out, n, c->part64);
+}
+
+void reserve_sort(hbtx_ctx *c, int64_t n, int32_t nseg)
+{
+  c->kalt.alloc(n);
+  c->valt.alloc(n);
+  c->sort_ntile.alloc(nseg + 1);
+  c->sort_toff.alloc(nseg + 1);
+  const int64_t maxt = n / (SORT_SMALL_MAX) + 2;
+  c->sort_tiles.alloc(maxt);
+  c->ghist.alloc(maxt * RADIX_BINS);
+}
+
+// Sort (keys, vals) within each segment [seg_begin, seg_begin+seg_len); keys use
+// the low key_bits bits.
+void seg_sort(hbtx_ctx *c, uint64_t *keys, int32_t *vals, int32_t nseg, const int32_t *seg_begin,
+              const int32_t *seg_len, int64_t n_cap, int key_bits)
+{
+  reserve_sort(c, n_cap, nseg);
+  k_sort_small<<<std::max(1, std::min(nseg, 16384)), SORT_THREADS, 0, c->stream>>>(keys, vals, seg_begin, seg_len,
+                                                                                    nseg);
+  k_sort_ntiles<<<blocks_for(nseg, 256, 1 << 20), 256, 0, c->stream>>>(seg_len, nseg, c->sort_ntile.p);
+  scan32(c, c->sort_ntile.p, c->sort_toff.p, nseg);
+  k_sort_tiles<<<blocks_for(nseg, 256, 1 << 20), 256, 0, c->stream>>>(seg_begin, seg_len, c->sort_toff.p, nseg,
+                                                                       c->sort_tiles.p);
+  const int32_t *ntiles_p = c->sort_toff.p + nseg;
+  const int64_t maxt = n_cap / SORT_SMALL_MAX + 2;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxt, 4 * c->num_cu));
+  int passes = (key_bits + RADIX_BITS - 1) / RADIX_BITS;
+  passes += passes & 1;
+  for (int pass = 0; pass < passes; pass++)
+  {
+    const int shift = pass * RADIX_BITS;
+    const uint64_t *kin = (pass & 1) ? c->kalt.p : keys;
+    const int32_t *vin = (pass & 1) ? c->valt.p : vals;
+    uint64_t *kout = (pass & 1) ? keys : c->kalt.p;
+    int32_t *vout = (pass & 1) ? vals : c->valt.p;
+    k_radix_hist<<<grid, SORT_THREADS, 0, c->stream>>>(kin, c->sort_tiles.p, ntiles_p, shift, c->ghist.p);
+    k_radix_offsets<<<std::max(1, std::min(nseg, 4096)), SORT_THREADS, 0, c->stream>>>(
+      seg_begin, seg_len, c->sort_toff.p, c->sort_ntile.p, nseg, c->ghist.p);
+    k_radix_scatter<<<grid, SORT_THREADS, 0, c->stream>>>(kin, vin, kout, vout, c->sort_tiles.p, ntiles_p, shift,
+                                                          c->ghist.p);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+// Tiles of RED_TILE slots over [begin[s], begin[s]+len[s]); returns device pointer to tile count.
+const int32_t *red_tiles(hbtx_ctx *c, const int32_t *begin, const int32_t *len, int32_t nsub, int64_t n_cap)
+{
+  c->red_ntile.alloc(nsub + 1);
+  c->red_toff.alloc(nsub + 1);
+  const int64_t maxt = n_cap / RED_TILE + nsub + 1;
+  c->red_tiles.alloc(maxt);
+  c->red_partial.alloc(maxt * RED_NV);
+  k_red_ntiles<<<blocks_for(nsub, 256, 1 << 20), 256, 0, c->stream>>>(len, nsub, c->red_ntile.p);
+  scan32(c, c->red_ntile.p, c->red_toff.p, nsub);
+  k_red_tiles<<<blocks_for(nsub, 256, 1 << 20), 256, 0, c->stream>>>(begin, len, c->red_toff.p, nsub,
+                                                                      c->red_tiles.p);
+  return c->red_toff.p + nsub;
+}
+
+int red_grid(hbtx_ctx *c, int64_t n_cap, int32_t nsub)
+{
+  return (int)std::max<int64_t>(1, std::min<int64_t>(n_cap / RED_TILE + nsub + 1, 8 * c->num_cu));
+}
+
+DevParams make_dev_params(const hbtx_params &p)
+{
+  REQUIRE(p.scale_factor > 0.f, HBTX_E_INVALID, "scale_factor must be positive");
+  REQUIRE(p.softening > 0.f, HBTX_E_INVALID, "softening must be positive");
+  REQUIRE(p.theta >= 0.f, HBTX_E_INVALID, "theta must be non-negative");
+  REQUIRE(!p.periodic || p.box_size > 0.f, HBTX_E_INVALID, "box_size must be positive when periodic");
+  DevParams P;
+  std::memset(&P, 0, sizeof(P));
+  const float eps = (p.max_phys_softening == -1.f) ? p.softening
+                                                   : std::min(p.softening, p.max_phys_softening / p.scale_factor);
+  P.G_over_a = p.G / (double)p.scale_factor;
+  const float res = p.softening * 0.1f;
+  P.res = res;
+  P.eps = eps;
+  P.h = (float)(2.8 * (double)eps);
+  P.hinv = (float)(1.0 / (2.8 * (double)eps));
+  P.h2 = P.h * P.h;
+  P.theta2 = p.theta * p.theta;
+  P.a = p.scale_factor;
+  P.hz = p.hz;
+  P.box = p.box_size;
+  P.box_half = p.box_size / 2.f;
+  P.prec = p.bound_mass_precision;
+  P.core_factor = p.sub_core_size_factor;
+  P.refine_frac = p.bound_fraction_center_refinement;
+  P.periodic = p.periodic;
+  P.thermal = p.dm_only ? 0 : p.thermal;
+  P.dm_only = p.dm_only;
+  P.tracer_mask = p.tracer_mask;
+  P.donot_mask = p.donot_subsample_mask;
+  P.per_type = p.upscale_per_type;
+  P.refine = (p.max_sample_size > 0 && p.refine_most_bound) ? 1 : 0;
+  P.mss = p.max_sample_size;
+  P.min_part = p.min_num_part;
+  P.min_tracer = p.min_num_tracer;
+  P.core_min = p.sub_core_size_min;
+  P.direct_cutoff = p.direct_cutoff;
+  return P;
+}
+
+// default direct/tree cutoff when the caller asks for the tuned value
+int64_t tuned_direct_cutoff()
+{
+  const char *e = getenv("HBTX_DIRECT_CUTOFF");
+  if (e)
+    return atoll(e);
+  return 1024;
+}
+
+void batch_reserve(hbtx_ctx *c, Batch &B)
+{
+  const int64_t n = B.n;
+  const int32_t ns = B.nsub;
+  B.posm.alloc(n);
+  B.velu.alloc(n);
+  B.ptype.alloc(n);
+  B.sub_of.alloc(n);
+  B.sub_zero.alloc(std::max<int64_t>(B.max_count, 1));
+  B.off.alloc(ns + 1);
+  B.off_pairs.alloc(2 * (int64_t)ns + 2);
+  B.order.alloc(ns + 1);
+  B.order_zero.alloc(1);
+  B.state.alloc(n);
+  B.cur.alloc(n);
+  B.cur2.alloc(n);
+  B.src.alloc(n);
+  B.fvals.alloc(n);
+  B.ford.alloc(n);
+  B.energy.alloc(n);
+  B.selfm.alloc(n);
+  B.fenergy.alloc(n);
+  B.fpot.alloc(n);
+  B.proposal.alloc(n);
+  B.flag.alloc(n + 1);
+  B.fscan.alloc(n + 1);
+  B.crank.alloc(n + 1);
+  B.cnt.alloc(n + 1);
+  B.cscan.alloc(n + 1);
+  B.flag64.alloc(n + 1);
+  B.fscan64.alloc(n + 1);
+  B.keys.alloc(n);
+  B.ent_cap = (int32_t)std::min<int64_t>(3 * n + 64, (int64_t)0x7ffffff0);
+  B.ent_pm.alloc(B.ent_cap);
+  B.ent_aux.alloc(B.ent_cap);
+  B.ent_depth.alloc(B.ent_cap);
+  B.items.alloc(n / HBTX_WAVE + ns + 1);
+  B.icnt.alloc(ns + 1);
+  B.iscan.alloc(ns + 1);
+  DArr<int32_t> *i32s[] = {&B.status, &B.corr, &B.iters, &B.mode, &B.treemode, &B.nlast, &B.nbound, &B.nrem,
+                           &B.nuns, &B.ntgt, &B.nsrc, &B.src_off, &B.ent_begin, &B.ent_end, &B.nitems, &B.bnd_cnt,
+                           &B.bnd_tr, &B.first_idx, &B.seg_begin, &B.seg_len, &B.rbegin, &B.rlen, &B.first_tracer};
+  for (auto *a : i32s)
+    a->alloc(ns + 1);
+  B.type_cnt.alloc(6 * (int64_t)ns + 6);
+  DArr<float> *f1[] = {&B.mlast, &B.mbound};
+  for (auto *a : f1)
+    a->alloc(ns + 1);
+  DArr<float> *f6[] = {&B.ref, &B.oldref, &B.mb, &B.upscale, &B.ref_in, &B.mb_in};
+  for (auto *a : f6)
+    a->alloc(6 * (int64_t)ns + 6);
+  B.cube.alloc(4 * (int64_t)ns + 4);
+  B.len0.alloc(ns + 1);
+  B.red.alloc((int64_t)RED_NV * ns + RED_NV);
+  B.red2.alloc((int64_t)RED_NV * ns + RED_NV);
+  B.bbox.alloc(6 * (int64_t)ns + 6);
+  B.misc.alloc(8);
+  B.dres.alloc(ns + 1);
+  c->stats.alloc(NSTATS);
+  reserve_sort(c, n, ns);
+}
+
+View make_view(Batch &B, int64_t p0, int64_t n, int32_t s0, int32_t nsub, bool whole)
+{
+  View V;
+  V.n = n;
+  V.nsub = nsub;
+  V.p0 = p0;
+  V.s0 = s0;
+  SubArrays &S = V.S;
+  S.nsub = nsub;
+  S.off = whole ? B.off.p : B.off_pairs.p + 2 * s0;
+  S.sub_of = whole ? B.sub_of.p : B.sub_zero.p;
+  S.status = B.status.p + s0;
+  S.corr = B.corr.p + s0;
+  S.iters = B.iters.p + s0;
+  S.mode = B.mode.p + s0;
+  S.treemode = B.treemode.p + s0;
+  S.nlast = B.nlast.p + s0;
+  S.nbound = B.nbound.p + s0;
+  S.nrem = B.nrem.p + s0;
+  S.nuns = B.nuns.p + s0;
+  S.ntgt = B.ntgt.p + s0;
+  S.nsrc = B.nsrc.p + s0;
+  S.mlast = B.mlast.p + s0;
+  S.mbound = B.mbound.p + s0;
+  S.ref = B.ref.p + 6 * s0;
+  S.oldref = B.oldref.p + 6 * s0;
+  S.mb = B.mb.p + 6 * s0;
+  S.upscale = B.upscale.p + 6 * s0;
+  S.src_off = B.src_off.p + s0;
+  S.ent_begin = B.ent_begin.p + s0;
+  S.ent_end = B.ent_end.p + s0;
+  S.nitems = B.nitems.p + s0;
+  S.cube = B.cube.p + 4 * s0;
+  S.len0 = B.len0.p + s0;
+  S.bbox = B.bbox.p + 6 * s0;
+  S.bnd_cnt = B.bnd_cnt.p + s0;
+  S.bnd_tr = B.bnd_tr.p + s0;
+  S.first_idx = B.first_idx.p + s0;
+  S.active_count = B.misc.p + 0;
+  PartArrays &Q = V.Q;
+  Q.posm = B.posm.p + p0;
+  Q.velu = B.velu.p + p0;
+  Q.ptype = B.has_type ? B.ptype.p + p0 : nullptr;
+  Q.state = B.state.p + p0;
+  Q.energy = B.energy.p + p0;
+  Q.selfm = B.selfm.p + p0;
+  Q.cur = B.cur.p + p0;
+  Q.cur2 = B.cur2.p + p0;
+  V.order = whole ? B.order.p : B.order_zero.p;
+  V.ford = B.ford.p + p0;
+  V.fenergy = B.fenergy.p + p0;
+  V.fpot = B.fpot.p + p0;
+  V.fvals = B.fvals.p + p0;
+  V.dres = B.dres.p + s0;
+  V.red = B.red.p + (int64_t)RED_NV * s0;
+  V.red2 = B.red2.p + (int64_t)RED_NV * s0;
+  V.type_cnt = B.type_cnt.p + 6 * s0;
+  V.first_tracer = B.first_tracer.p + s0;
+  return V;
+}
+
+TreeArgs tree_args(Batch &B, View &V, const DevParams &P)
+{
+  TreeArgs T;
+  T.src = B.src.p;
+  T.keys = B.keys.p;
+  T.sub_of = V.S.sub_of;
+  T.src_off = V.S.src_off;
+  T.nsrc = V.S.nsrc;
+  T.treemode = V.S.treemode;
+  T.mode = V.S.mode;
+  T.len0 = V.S.len0;
+  T.posm = V.Q.posm;
+  T.ptype = V.Q.ptype;
+  T.upscale = V.S.upscale;
+  T.cnt = B.cnt.p;
+  T.cscan = B.cscan.p;
+  T.s_total = B.fscan.p + V.n;
+  T.ent_pm = B.ent_pm.p;
+  T.ent_aux = B.ent_aux.p;
+  T.ent_depth = B.ent_depth.p;
+  T.cap = B.ent_cap;
+  T.overflow = B.misc.p + 2;
+  T.P = P;
+  return T;
+}
+
+// Sources of the round -> dense list, then (tree-mode subhaloes) the octree.
+void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int round)
+{
+  hipStream_t st = c->stream;
+  const int64_t n = V.n;
+  const int gb = blocks_for(n);
+  const int gs = blocks_for(V.nsub, 256, 1 << 20);
+  const int32_t *crank = nullptr;
+  if (P.mss > 0)
+  {
+    k_mark_candidates<<<gb, 256, 0, st>>>(V.S, V.Q, n, B.flag.p);
+    scan32(c, B.flag.p, B.crank.p, n);
+    crank = B.crank.p;
+  }
+  else
+  {
+    // rank is irrelevant without subsampling (every candidate is a source)
+    k_fill_i32<<<gb, 256, 0, st>>>(B.crank.p, n + 1, 0);
+    crank = B.crank.p;
+  }
+  k_mark_sources<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, round, crank, B.flag.p);
+  scan32(c, B.flag.p, B.fscan.p, n);
+  k_scatter_sources<<<gb, 256, 0, st>>>(V.S, n, B.flag.p, B.fscan.p, B.src.p);
+  k_sub_src<<<gs, 256, 0, st>>>(V.S, B.fscan.p);
+  const int32_t *s_total = B.fscan.p + n;
+  k_bbox<<<gb, 256, 0, st>>>(V.S, V.Q.posm, B.src.p, s_total);
+  k_cube<<<gs, 256, 0, st>>>(V.S);
+  k_keys<<<gb, 256, 0, st>>>(V.S, P, V.Q.posm, B.src.p, s_total, B.keys.p);
+  k_sort_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
+  seg_sort(c, B.keys.p, B.src.p, V.nsub, B.seg_begin.p, B.seg_len.p, n, 63);
+  TreeArgs T = tree_args(B, V, P);
+  k_tree_count<<<blocks_for(n + 1, 256, 1 << 24), 256, 0, st>>>(T, (int32_t)n, B.cnt.p);
+  scan32(c, B.cnt.p, B.cscan.p, n);
+  // exact entry count of this round's trees; grow the entry buffers if needed
+  k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, 0x7fffffff, B.misc.p + 2);
+  int32_t etot = 0;
+  HIPCHK(hipMemcpyAsync(&etot, B.misc.p + 3, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  REQUIRE(etot >= 0, HBTX_E_CAPACITY, "tree entry count overflows int32");
+  if (etot > B.ent_cap)
+  {
+    const int64_t cap = std::min<int64_t>((int64_t)etot + etot / 4 + 64, (int64_t)0x7ffffff0);
+    B.ent_pm.release();
+    B.ent_aux.release();
+    B.ent_depth.release();
+    B.ent_pm.alloc(cap);
+    B.ent_aux.alloc(cap);
+    B.ent_depth.alloc(cap);
+    B.ent_cap = (int32_t)cap;
+    T = tree_args(B, V, P);
+  }
+  k_tree_emit<<<blocks_for(n, 256, 1 << 24), 256, 0, st>>>(T);
+  k_sub_entries<<<gs, 256, 0, st>>>(V.S, B.cscan.p, B.ent_cap);
+  k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, B.ent_cap, B.misc.p + 2);
+  for (int d = HBTX_KEY_LEVELS; d >= 0; d--)
+    k_tree_moments<<<blocks_for(2 * n, 256, 8192), 256, 0, st>>>(B.ent_pm.p, B.ent_aux.p, B.ent_depth.p,
+                                                                 B.misc.p + 3, d);
+  HIPCHK(hipGetLastError());
+}
+
+WalkArgs walk_args(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
+{
+  WalkArgs W;
+  W.items = B.items.p;
+  W.nitems = B.iscan.p + V.nsub;
+  W.counter = B.misc.p + 1;
+  W.cur = V.Q.cur;
+  W.off = V.S.off;
+  W.ntgt = V.S.ntgt;
+  W.mode = V.S.mode;
+  W.treemode = V.S.treemode;
+  W.ent_begin = V.S.ent_begin;
+  W.ent_end = V.S.ent_end;
+  W.ent_pm = B.ent_pm.p;
+  W.ent_aux = B.ent_aux.p;
+  W.posm = V.Q.posm;
+  W.velu = V.Q.velu;
+  W.selfm = V.Q.selfm;
+  W.ref = V.S.ref;
+  W.oldref = V.S.oldref;
+  W.energy = V.Q.energy;
+  W.phi_out = nullptr;
+  W.stats = c->stats.p;
+  W.P = P;
+  return W;
+}
+
+void round_walk(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
+{
+  hipStream_t st = c->stream;
+  const int gs = blocks_for(V.nsub, 256, 1 << 20);
+  k_items_count<<<gs, 256, 0, st>>>(V.S, V.order, B.icnt.p);
+  scan32(c, B.icnt.p, B.iscan.p, V.nsub);
+  k_items_emit<<<gs, 256, 0, st>>>(V.S, V.order, B.iscan.p, B.items.p);
+  k_fill_i32<<<1, 64, 0, st>>>(B.misc.p + 1, 1, 0);
+  WalkArgs W = walk_args(c, B, V, P);
+  const int64_t maxitems = V.n / HBTX_WAVE + V.nsub + 1;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxitems, 32 * c->num_cu));
+  HIPCHK(hipEventRecord(c->wev0, st));
+  k_walk_energy<<<grid, 64, 0, st>>>(W);
+  HIPCHK(hipEventRecord(c->wev1, st));
+  HIPCHK(hipGetLastError());
+}
+
+void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int round)
+{
+  hipStream_t st = c->stream;
+  const int64_t n = V.n;
+  const int gb = blocks_for(n);
+  const int gs = blocks_for(V.nsub, 256, 1 << 20);
+  k_partition_flags<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, B.flag64.p);
+  scan64(c, B.flag64.p, B.fscan64.p, n);
+  k_partition_scatter<<<gb, 256, 0, st>>>(V.S, V.Q, n, round, B.fscan64.p);
+  k_copy_cur<<<gb, 256, 0, st>>>(V.S, V.Q, n, B.fscan64.p);
+  k_round_decide<<<gs, 256, 0, st>>>(V.S, P, B.fscan64.p);
+  k_com_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p);
+  if (P.periodic)
+  {
+    k_fill_i32<<<gs, 256, 0, st>>>(V.S.first_idx, V.nsub, 0x7fffffff);
+    k_first_index<<<gb, 256, 0, st>>>(V.S, V.Q, n);
+  }
+  const int32_t *ntiles = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, n);
+  k_red_tile<ComOp><<<red_grid(c, n, V.nsub), RED_THREADS, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, ntiles,
+                                                                     c->red_partial.p);
+  k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
+  k_round_finish<<<gs, 256, 0, st>>>(V.S, P, V.red);
+  if (P.periodic)
+    k_round_finish_periodic<<<gs, 256, 0, st>>>(V.S, V.Q, P, V.red);
+  HIPCHK(hipGetLastError());
+}
+
+__global__ void k_final_local(SubArrays S, int64_t n, const int32_t *vals, const float *energy, int32_t *ford,
+                              float *fenergy)
+{
+  GRID_STRIDE(q, n)
+  {
+    const int s = S.sub_of[q];
+    const int p = vals[q];
+    ford[q] = p - S.off[s];
+    fenergy[q] = energy[p];
+  }
+}
+
+__global__ void k_final_sub(SubArrays S, PartArrays Q, DevParams P, const int32_t *vals, const double *kin,
+                            const double *core, const int32_t *type_cnt, const int32_t *first_tracer,
+                            const float *ref_in, hbtx_sub_result *out)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub)
+    return;
+  hbtx_sub_result R = {};
+  const int n = S.off[s + 1] - S.off[s];
+  const int st = S.status[s];
+  R.iterations = S.iters[s];
+  R.mostbound_id = -1;
+  const float *mb = S.mb + 6 * s;
+  if (n == 0)
+  {
+    // orphan early exit (src/subhalo_unbind.cpp:472-489): frame unchanged, core = most-bound
+    for (int k = 0; k < 3; k++)
+    {
+      R.com_pos[k] = ref_in[6 * s + k];
+      R.com_vel[k] = ref_in[6 * s + 3 + k];
+      R.mostbound_pos[k] = mb[k];
+      R.mostbound_vel[k] = mb[3 + k];
+      R.core_pos[k] = mb[k];
+      R.core_vel[k] = mb[3 + k];
+    }
+    out[s] = R;
+    return;
+  }
+  const float *rf = S.ref + 6 * s;
+  for (int k = 0; k < 3; k++)
+  {
+    R.com_pos[k] = rf[k];
+    R.com_vel[k] = rf[3 + k];
+  }
+  if (st != ST_CONVERGED)
+  {
+    R.disrupted = 1;
+    for (int k = 0; k < 3; k++)
+    {
+      R.mostbound_pos[k] = mb[k];
+      R.mostbound_vel[k] = mb[3 + k];
+      R.core_pos[k] = mb[k];
+      R.core_vel[k] = mb[3 + k];
+    }
+    out[s] = R;
+    return;
+  }
+  const int nb = S.nbound[s];
+  R.nbound = nb;
+  R.mbound = S.mbound[s];
+  const int p0 = vals[S.off[s]];
+  const float4 x0 = Q.posm[p0];
+  const float4 v0 = Q.velu[p0];
+  R.mostbound_pos[0] = x0.x;
+  R.mostbound_pos[1] = x0.y;
+  R.mostbound_pos[2] = x0.z;
+  R.mostbound_vel[0] = v0.x;
+  R.mostbound_vel[1] = v0.y;
+  R.mostbound_vel[2] = v0.z;
+  const double *v = kin + (int64_t)RED_NV * s;
+  if (nb > 1)
+  {
+    const double M = v[6];
+    double E = v[0] - v[1];
+    E /= M;
+    double K = v[2] * (0.5 / M);
+    R.spec_pot = (float)(E - K);
+    R.spec_kin = (float)K;
+    R.spec_am[0] = (float)(v[3] / M);
+    R.spec_am[1] = (float)(v[4] / M);
+    R.spec_am[2] = (float)(v[5] / M);
+  }
+  if (!P.dm_only)
+  {
+    for (int k = 0; k < 6; k++)
+    {
+      R.nbound_type[k] = type_cnt[6 * s + k];
+      R.mbound_type[k] = (float)v[7 + k];
+    }
+    const int ft = first_tracer[s];
+    R.tracer_index = (ft == 0x7fffffff) ? 0 : ft;
+  }
+  else
+    R.tracer_index = 0;
+  const double *c = core + (int64_t)RED_NV * s;
+  const double msum = c[0];
+  double s2r = 0., s2v = 0.;
+  for (int d = 0; d < 3; d++)
+  {
+    const double pos = c[1 + d] / msum;
+    const double pos2 = c[4 + d] / msum - pos * pos;
+    const double vel = c[7 + d] / msum;
+    const double vel2 = c[10 + d] / msum - vel * vel;
+    float cp = (float)pos;
+    if (P.periodic)
+      cp = (float)(pos + (double)x0.x * (d == 0) + (double)x0.y * (d == 1) + (double)x0.z * (d == 2));
+    R.core_pos[d] = cp;
+    R.core_vel[d] = (float)vel;
+    s2r += pos2;
+    s2v += vel2;
+  }
+  R.core_sigma_r = (float)sqrt(s2r);
+  R.core_sigma_v = (float)sqrt(s2v);
+  out[s] = R;
+}
+
+void run_final(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot)
+{
+  hipStream_t st = c->stream;
+  const int64_t n = V.n;
+  const int gb = blocks_for(n);
+  const int gs = blocks_for(V.nsub, 256, 1 << 20);
+  k_final_keys<<<gb, 256, 0, st>>>(V.S, V.Q, n, B.keys.p, V.fvals);
+  k_final_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
+  seg_sort(c, B.keys.p, V.fvals, V.nsub, B.seg_begin.p, B.seg_len.p, n, 48);
+  k_final_local<<<gb, 256, 0, st>>>(V.S, n, V.fvals, V.Q.energy, V.ford, V.fenergy);
+  // kinematics / types over the bound part
+  k_final_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p, 0, P);
+  k_fill_i32<<<blocks_for(6 * V.nsub), 256, 0, st>>>(V.type_cnt, 6 * (int64_t)V.nsub, 0);
+  k_fill_i32<<<gs, 256, 0, st>>>(V.first_tracer, V.nsub, 0x7fffffff);
+  const int32_t *ntiles = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, n);
+  k_final_tile<<<red_grid(c, n, V.nsub), RED_THREADS, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, ntiles, V.fvals,
+                                                                c->red_partial.p, V.type_cnt, V.first_tracer);
+  k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
+  // core phase space
+  k_core_flags<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, V.fvals, B.flag.p);
+  scan32(c, B.flag.p, B.fscan.p, n);
+  k_core_tile<<<red_grid(c, n, V.nsub), RED_THREADS, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, ntiles, V.fvals,
+                                                               B.fscan.p, c->red_partial.p);
+  k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red2);
+  k_final_sub<<<gs, 256, 0, st>>>(V.S, V.Q, P, V.fvals, V.red, V.red2, V.type_cnt, V.first_tracer,
+                                  B.ref_in.p + 6 * V.s0, V.dres);
+  if (want_pot)
+    k_potential_energy<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, V.fvals, V.fpot);
+  HIPCHK(hipGetLastError());
+}
+
+// Iterate one view to convergence. Returns rounds used.
+int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot)
+{
+  hipStream_t st = c->stream;
+  const int gb = blocks_for(V.n);
+  const int gs = blocks_for(V.nsub, 256, 1 << 20);
+  k_init_particles<<<gb, 256, 0, st>>>(V.S, V.Q, V.n);
+  k_init_subs<<<gs, 256, 0, st>>>(V.S, B.ref_in.p + 6 * V.s0, B.mb_in.p + 6 * V.s0, nullptr);
+  k_fill_i32<<<1, 64, 0, st>>>(B.misc.p, 4, 0);
+  HIPCHK(hipGetLastError());
+  int round = 0;
+  const int max_rounds = 2000;
+  while (true)
+  {
+    REQUIRE(round < max_rounds, HBTX_E_INVALID, "unbinding did not converge within the round limit");
+    round++;
+    k_fill_i32<<<1, 64, 0, st>>>(V.S.active_count, 1, 0);
+    k_plan<<<gs, 256, 0, st>>>(V.S, P, round);
+    round_sources_and_tree(c, B, V, P, round);
+    round_walk(c, B, V, P);
+    round_partition(c, B, V, P, round);
+    int32_t h_misc[4];
+    HIPCHK(hipMemcpyAsync(h_misc, B.misc.p, sizeof(h_misc), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float wms = 0.f;
+    HIPCHK(hipEventElapsedTime(&wms, c->wev0, c->wev1));
+    c->walk_ms += wms;
+    REQUIRE(h_misc[2] == 0, HBTX_E_CAPACITY, "tree entry capacity exceeded");
+    if (h_misc[0] == 0)
+      break;
+  }
+  run_final(c, B, V, P, want_pot);
+  return round;
+}
+
+void upload_batch(hbtx_ctx *c, Batch &B, int64_t n, const float *pos, const float *vel, const float *mass,
+                  const float *u, const int32_t *type, const int64_t *id, int64_t nsub, const int64_t *offsets,
+                  const float *ref_pos, const float *ref_vel, const float *mb_pos, const float *mb_vel)
+{
+  REQUIRE(n >= 0 && nsub >= 1, HBTX_E_INVALID, "need n >= 0 and nsub >= 1");
+  REQUIRE(n < (int64_t)0x20000000, HBTX_E_INVALID, "at most 2^29 particles per batch");
+  REQUIRE(nsub < (int64_t)0x7fffffff, HBTX_E_INVALID, "too many subhaloes");
+  REQUIRE(pos && vel && mass && offsets && ref_pos && ref_vel, HBTX_E_INVALID, "null input array");
+  REQUIRE(offsets[0] == 0 && offsets[nsub] == n, HBTX_E_INVALID, "offsets must start at 0 and end at n");
+  B.n = n;
+  B.nsub = (int32_t)nsub;
+  B.has_type = type != nullptr;
+  B.has_u = u != nullptr;
+  B.has_proposal = false;
+  B.h_off.assign(offsets, offsets + nsub + 1);
+  B.h_count.resize(nsub);
+  B.max_count = 0;
+  for (int64_t s = 0; s < nsub; s++)
+  {
+    REQUIRE(offsets[s + 1] >= offsets[s], HBTX_E_INVALID, "offsets must be non-decreasing");
+    B.h_count[s] = (int32_t)(offsets[s + 1] - offsets[s]);
+    B.max_count = std::max(B.max_count, B.h_count[s]);
+  }
+  if (id)
+    B.h_id.assign(id, id + n);
+  else
+  {
+    B.h_id.resize(n);
+    for (int64_t s = 0; s < nsub; s++)
+      for (int64_t i = offsets[s]; i < offsets[s + 1]; i++)
+        B.h_id[i] = i - offsets[s];
+  }
+  batch_reserve(c, B);
+  std::vector<float4> hp(n), hv(n);
+  std::vector<int32_t> hsub(n);
+  for (int64_t i = 0; i < n; i++)
+  {
+    hp[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], mass[i]);
+    hv[i] = make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], u ? u[i] : 0.f);
+  }
+  for (int64_t s = 0; s < nsub; s++)
+    for (int64_t i = offsets[s]; i < offsets[s + 1]; i++)
+      hsub[i] = (int32_t)s;
+  std::vector<int32_t> hoff(nsub + 1), hpairs(2 * nsub + 2, 0), horder(nsub);
+  for (int64_t s = 0; s <= nsub; s++)
+    hoff[s] = (int32_t)offsets[s];
+  for (int64_t s = 0; s < nsub; s++)
+    hpairs[2 * s + 1] = B.h_count[s];
+  std::iota(horder.begin(), horder.end(), 0);
+  std::stable_sort(horder.begin(), horder.end(),
+                   [&](int32_t a, int32_t b) { return B.h_count[a] > B.h_count[b]; });
+  std::vector<float> hr(6 * nsub), hm(6 * nsub);
+  for (int64_t s = 0; s < nsub; s++)
+    for (int k = 0; k < 3; k++)
+    {
+      hr[6 * s + k] = ref_pos[3 * s + k];
+      hr[6 * s + 3 + k] = ref_vel[3 * s + k];
+      hm[6 * s + k] = mb_pos ? mb_pos[3 * s + k] : ref_pos[3 * s + k];
+      hm[6 * s + 3 + k] = mb_vel ? mb_vel[3 * s + k] : ref_vel[3 * s + k];
+    }
+  hipStream_t st = c->stream;
+  HIPCHK(hipMemcpyAsync(B.posm.p, hp.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(B.velu.p, hv.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
+  if (type)
+  {
+    std::vector<int8_t> ht(n);
+    for (int64_t i = 0; i < n; i++)
+    {
+      REQUIRE(type[i] >= 0 && type[i] < 6, HBTX_E_INVALID, "particle type out of range [0,6)");
+      ht[i] = (int8_t)type[i];
+    }
+    HIPCHK(hipMemcpyAsync(B.ptype.p, ht.data(), n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  HIPCHK(hipMemcpyAsync(B.sub_of.p, hsub.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(B.sub_zero.p, 0, std::max<int64_t>(B.max_count, 1) * sizeof(int32_t), st));
+  HIPCHK(hipMemsetAsync(B.order_zero.p, 0, sizeof(int32_t), st));
+  HIPCHK(hipMemcpyAsync(B.off.p, hoff.data(), (nsub + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(B.off_pairs.p, hpairs.data(), (2 * nsub + 2) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(B.order.p, horder.data(), nsub * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(B.ref_in.p, hr.data(), 6 * nsub * sizeof(float), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(B.mb_in.p, hm.data(), 6 * nsub * sizeof(float), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  B.ran = false;
+}
+
+void run_batch(hbtx_ctx *c, const hbtx_params &prm, uint32_t flags, double *elapsed_ms)
+{
+  Batch &B = c->B;
+  REQUIRE(B.n > 0 || B.nsub > 0, HBTX_E_INVALID, "no batch loaded");
+  hbtx_params p = prm;
+  if (p.direct_cutoff < 0)
+    p.direct_cutoff = tuned_direct_cutoff();
+  REQUIRE(p.max_sample_size == 0, HBTX_E_NOT_IMPLEMENTED,
+          "MaxSampleSizeOfPotentialEstimate > 0 (subsampling) is not supported by this build");
+  REQUIRE(!(flags & HBTX_WARMSTART), HBTX_E_NOT_IMPLEMENTED, "warm start is not supported by this build");
+  const DevParams P = make_dev_params(p);
+  hipStream_t st = c->stream;
+  HIPCHK(hipMemsetAsync(c->stats.p, 0, NSTATS * sizeof(unsigned long long), st));
+  c->walk_ms = 0.0;
+  c->rounds = 0;
+  HIPCHK(hipEventRecord(c->ev0, st));
+  const bool want_pot = (flags & HBTX_SAVE_POTENTIAL) != 0;
+  if (flags & HBTX_PLAIN)
+  {
+    for (int32_t s = 0; s < B.nsub; s++)
+    {
+      View V = make_view(B, B.h_off[s], B.h_count[s], s, 1, false);
+      c->rounds += run_view(c, B, V, P, want_pot);
+    }
+  }
+  else
+  {
+    View V = make_view(B, 0, B.n, 0, B.nsub, true);
+    c->rounds = run_view(c, B, V, P, want_pot);
+  }
+  HIPCHK(hipEventRecord(c->ev1, st));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (elapsed_ms)
+    *elapsed_ms = ms;
+  B.ran = true;
+}
+
+void fetch_batch(hbtx_ctx *c, int64_t *out_order, float *out_energy, float *out_potential,
+                 hbtx_sub_result *out_res)
+{
+  Batch &B = c->B;
+  REQUIRE(B.ran, HBTX_E_INVALID, "batch has not been run");
+  hipStream_t st = c->stream;
+  const int64_t n = B.n;
+  std::vector<int32_t> ford(n);
+  HIPCHK(hipMemcpyAsync(ford.data(), B.ford.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  if (out_energy)
+    HIPCHK(hipMemcpyAsync(out_energy, B.fenergy.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+  if (out_potential)
+    HIPCHK(hipMemcpyAsync(out_potential, B.fpot.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+  std::vector<hbtx_sub_result> res(B.nsub);
+  HIPCHK(hipMemcpyAsync(res.data(), B.dres.p, B.nsub * sizeof(hbtx_sub_result), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int64_t s = 0; s < B.nsub; s++)
+  {
+    const int64_t a = B.h_off[s];
+    for (int64_t k = 0; k < B.h_count[s]; k++)
+      if (out_order)
+        out_order[a + k] = ford[a + k];
+    hbtx_sub_result &R = res[s];
+    if (R.nbound > 0 && !R.disrupted)
+    {
+      const int64_t ti = R.tracer_index;
+      R.mostbound_id = B.h_id[a + ford[a + ti]];
+    }
+    else
+      R.mostbound_id = -1;
+    if (out_res)
+      out_res[s] = R;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// tree API
+// ---------------------------------------------------------------------------
+void build_tree(hbtx_ctx *c, const hbtx_params &prm, int64_t n, const float *pos, const float *mass, hbtx_tree *T,
+                int treemode)
+{
+  REQUIRE(n >= 1, HBTX_E_INVALID, "a tree needs at least one source");
+  Batch &B = c->B;
+  std::vector<float> vel(3 * n, 0.f);
+  const int64_t off[2] = {0, n};
+  float zero3[3] = {0.f, 0.f, 0.f};
+  upload_batch(c, B, n, pos, vel.data(), mass, nullptr, nullptr, nullptr, 1, off, zero3, zero3, nullptr, nullptr);
+  hbtx_params p = prm;
+  p.direct_cutoff = treemode ? 0 : ((int64_t)1 << 40);
+  p.max_sample_size = 0;
+  const DevParams P = make_dev_params(p);
+  View V = make_view(B, 0, n, 0, 1, true);
+  hipStream_t st = c->stream;
+  k_init_particles<<<blocks_for(n), 256, 0, st>>>(V.S, V.Q, n);
+  k_init_subs<<<1, 64, 0, st>>>(V.S, B.ref_in.p, B.mb_in.p, nullptr);
+  k_fill_i32<<<1, 64, 0, st>>>(B.misc.p, 4, 0);
+  k_plan<<<1, 64, 0, st>>>(V.S, P, 1);
+  round_sources_and_tree(c, B, V, P, 1);
+  int32_t h_misc[4];
+  HIPCHK(hipMemcpyAsync(h_misc, B.misc.p, sizeof(h_misc), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  REQUIRE(h_misc[2] == 0, HBTX_E_CAPACITY, "tree entry capacity exceeded");
+  const int32_t ne = h_misc[3];
+  T->ctx = c;
+  T->n = n;
+  T->ne = ne;
+  T->ncells = ne - n;
+  T->P = P;
+  T->treemode = treemode;
+  T->ent_pm.alloc(ne);
+  T->ent_aux.alloc(ne);
+  T->ent_depth.alloc(ne);
+  T->src_of_entry.alloc(n);
+  T->cube.alloc(4);
+  HIPCHK(hipMemcpyAsync(T->ent_pm.p, B.ent_pm.p, ne * sizeof(float4), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(T->ent_aux.p, B.ent_aux.p, ne * sizeof(int2), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(T->ent_depth.p, B.ent_depth.p, ne, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(T->src_of_entry.p, B.src.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(T->cube.p, B.cube.p, 4 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  B.ran = false;
+}
+
+void tree_points(hbtx_ctx *c, hbtx_tree *T, int64_t nt, const float *tpos, const float *tmass, double *out)
+{
+  REQUIRE(nt >= 0 && nt < (int64_t)0x20000000, HBTX_E_INVALID, "bad target count");
+  if (nt == 0)
+    return;
+  hipStream_t st = c->stream;
+  DArr<float4> dt;
+  DArr<float> phi;
+  DArr<double> phid;
+  DArr<uint64_t> keys;
+  DArr<int32_t> vals, segb, segl, counter;
+  dt.alloc(nt);
+  phi.alloc(nt);
+  phid.alloc(nt);
+  keys.alloc(nt);
+  vals.alloc(nt);
+  segb.alloc(1);
+  segl.alloc(1);
+  counter.alloc(1);
+  std::vector<float4> h(nt);
+  for (int64_t i = 0; i < nt; i++)
+    h[i] = make_float4(tpos[3 * i], tpos[3 * i + 1], tpos[3 * i + 2], tmass ? tmass[i] : 0.f);
+  HIPCHK(hipMemcpyAsync(dt.p, h.data(), nt * sizeof(float4), hipMemcpyHostToDevice, st));
+  // walk order: targets sorted by their key in the tree cube
+  k_point_keys<<<blocks_for(nt), 256, 0, st>>>(dt.p, (int32_t)nt, T->cube.p, keys.p, vals.p);
+  int32_t hb = 0, hl = (int32_t)nt;
+  HIPCHK(hipMemcpyAsync(segb.p, &hb, 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(segl.p, &hl, 4, hipMemcpyHostToDevice, st));
+  if (T->treemode)
+    seg_sort(c, keys.p, vals.p, 1, segb.p, segl.p, nt, 63);
+  HIPCHK(hipMemsetAsync(counter.p, 0, 4, st));
+  PointArgs A;
+  A.tgt = dt.p;
+  A.tgt_order = vals.p;
+  A.ntgt = (int32_t)nt;
+  A.eb = 0;
+  A.ee = T->ne;
+  A.treemode = T->treemode;
+  A.ent_pm = T->ent_pm.p;
+  A.ent_aux = T->ent_aux.p;
+  A.phi = phi.p;
+  A.phi_d = phid.p;
+  A.counter = counter.p;
+  A.stats = c->stats.p;
+  A.P = T->P;
+  HIPCHK(hipMemsetAsync(c->stats.p, 0, NSTATS * sizeof(unsigned long long), st));
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nt / HBTX_WAVE + 1, 32 * c->num_cu));
+  HIPCHK(hipEventRecord(c->wev0, st));
+  k_points_potential<<<grid, 64, 0, st>>>(A);
+  HIPCHK(hipEventRecord(c->wev1, st));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, phid.p, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float wms = 0.f;
+  HIPCHK(hipEventElapsedTime(&wms, c->wev0, c->wev1));
+  c->walk_ms = wms;
+}
+
+int fail(hbtx_ctx *c, int code, const std::string &msg)
+{
+  if (c)
+    c->err = msg;
+  return code;
+}
+
+} // namespace
+
+#define API_GUARD(ctx, ...)                                                                                          \
+  try                                                                                                                \
+  {                                                                                                                  \
+    __VA_ARGS__;                                                                                                     \
+    return HBTX_OK;                                                                                                  \
+  }                                                                                                                  \
+  catch (const Error &e)                                                                                             \
+  {                                                                                                                  \
+    return fail(ctx, e.code, e.what());                                                                              \
+  }                                                                                                                  \
+  catch (const std::exception &e)                                                                                    \
+  {                                                                                                                  \
+    return fail(ctx, HBTX_E_INVALID, e.what());                                                                      \
+  }
+
+extern "C" {
+
+void hbtx_default_params(hbtx_params *p)
+{
+  if (!p)
+    return;
+  std::memset(p, 0, sizeof(*p));
+  p->G = 43.0071;
+  p->scale_factor = 1.f;
+  p->hz = 0.f;
+  p->softening = 0.001f;
+  p->max_phys_softening = -1.f;
+  p->theta = 0.45f;
+  p->bound_mass_precision = 0.995f;
+  p->max_sample_size = 1000;
+  p->refine_most_bound = 1;
+  p->bound_fraction_center_refinement = 0.1f;
+  p->min_num_part = 20;
+  p->min_num_tracer = 10;
+  p->tracer_mask = (1 << 1) | (1 << 4);
+  p->donot_subsample_mask = (1 << 5);
+  p->periodic = 1;
+  p->box_size = 0.f;
+  p->thermal = 0;
+  p->dm_only = 1;
+  p->upscale_per_type = 1;
+  p->sub_core_size_factor = 0.f;
+  p->sub_core_size_min = 20;
+  p->direct_cutoff = -1;
+}
+
+int hbtx_create(hbtx_ctx **out, int device)
+{
+  if (!out)
+    return HBTX_E_INVALID;
+  *out = nullptr;
+  hbtx_ctx *c = new (std::nothrow) hbtx_ctx();
+  if (!c)
+    return HBTX_E_ALLOC;
+  try
+  {
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    REQUIRE(device >= 0 && device < ndev, HBTX_E_INVALID, "no such HIP device");
+    c->device = device;
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    c->num_cu = std::max(1, prop.multiProcessorCount);
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&c->ev0));
+    HIPCHK(hipEventCreate(&c->ev1));
+    HIPCHK(hipEventCreate(&c->wev0));
+    HIPCHK(hipEventCreate(&c->wev1));
+    c->stats.alloc(NSTATS);
+  }
+  catch (const Error &e)
+  {
+    int code = e.code;
+    delete c;
+    return code;
+  }
+  *out = c;
+  return HBTX_OK;
+}
+
+void hbtx_destroy(hbtx_ctx *c)
+{
+  if (!c)
+    return;
+  (void)hipSetDevice(c->device);
+  if (c->stream)
+    (void)hipStreamSynchronize(c->stream);
+  if (c->ev0)
+    (void)hipEventDestroy(c->ev0);
+  if (c->ev1)
+    (void)hipEventDestroy(c->ev1);
+  if (c->wev0)
+    (void)hipEventDestroy(c->wev0);
+  if (c->wev1)
+    (void)hipEventDestroy(c->wev1);
+  hipStream_t st = c->stream;
+  delete c;
+  if (st)
+    (void)hipStreamDestroy(st);
+}
+
+const char *hbtx_last_error(const hbtx_ctx *c)
+{
+  return c ? c->err.c_str() : "null context";
+}
+
+int hbtx_device_sync(hbtx_ctx *c)
+{
+  if (!c)
+    return HBTX_E_INVALID;
+  API_GUARD(c, HIPCHK(hipStreamSynchronize(c->stream)));
+}
+
+int hbtx_batch_load(hbtx_ctx *c, int64_t n, const float *pos, const float *vel, const float *mass, const float *u,
+                    const int32_t *type, const int64_t *id, int64_t nsub, const int64_t *offsets,
+                    const float *ref_pos, const float *ref_vel, const float *mb_pos, const float *mb_vel)
+{
+  if (!c)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    upload_batch(c, c->B, n, pos, vel, mass, u, type, id, nsub, offsets, ref_pos, ref_vel, mb_pos, mb_vel);
+  });
+}
+
+int hbtx_batch_set_proposal(hbtx_ctx *c, const uint8_t *proposal)
+{
+  if (!c)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    REQUIRE(proposal, HBTX_E_INVALID, "null proposal");
+    HIPCHK(hipMemcpyAsync(c->B.proposal.p, proposal, c->B.n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->B.has_proposal = true;
+  });
+}
+
+int hbtx_batch_run(hbtx_ctx *c, const hbtx_params *prm, uint32_t flags, double *elapsed_ms)
+{
+  if (!c || !prm)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    run_batch(c, *prm, flags, elapsed_ms);
+  });
+}
+
+int hbtx_batch_fetch(hbtx_ctx *c, int64_t *out_order, float *out_energy, float *out_potential,
+                     hbtx_sub_result *out_res)
+{
+  if (!c)
+    return HBTX_E_INVALID;
+  API_GUARD(c, fetch_batch(c, out_order, out_energy, out_potential, out_res));
+}
+
+static const char *kStatNames[NSTATS] = {"tree_wave_steps", "tree_lane_visits", "tree_lane_accepts",
+                                         "direct_lane_pairs", "targets", "work_items", "near_interactions",
+                                         "unused7", "unused8", "unused9", "unused10", "unused11", "walk_ms",
+                                         "rounds", "unused14", "unused15"};
+
+int hbtx_batch_stats(hbtx_ctx *c, double *stats, int nstats)
+{
+  if (!c || !stats)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    unsigned long long h[NSTATS];
+    HIPCHK(hipMemcpy(h, c->stats.p, sizeof(h), hipMemcpyDeviceToHost));
+    for (int i = 0; i < nstats && i < NSTATS; i++)
+      stats[i] = (double)h[i];
+    if (nstats > 12)
+      stats[12] = c->walk_ms;
+    if (nstats > 13)
+      stats[13] = c->rounds;
+  });
+}
+
+const char *hbtx_stat_name(int i)
+{
+  return (i >= 0 && i < NSTATS) ? kStatNames[i] : "";
+}
+
+int hbtx_unbind_batch(hbtx_ctx *c, const hbtx_params *prm, int64_t n, const float *pos, const float *vel,
+                      const float *mass, const float *u, const int32_t *type, const int64_t *id, int64_t nsub,
+                      const int64_t *offsets, const float *ref_pos, const float *ref_vel, const float *mb_pos,
+                      const float *mb_vel, uint32_t flags, int64_t *out_order, float *out_energy,
+                      float *out_potential, hbtx_sub_result *out_res)
+{
+  if (!c || !prm)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    upload_batch(c, c->B, n, pos, vel, mass, u, type, id, nsub, offsets, ref_pos, ref_vel, mb_pos, mb_vel);
+    run_batch(c, *prm, flags, nullptr);
+    fetch_batch(c, out_order, out_energy, out_potential, out_res);
+  });
+}
+
+int hbtx_tree_build(hbtx_ctx *c, const hbtx_params *prm, int64_t n, const float *pos, const float *mass,
+                    hbtx_tree **tree, int64_t *ncells)
+{
+  if (!c || !prm || !tree || !pos || !mass)
+    return HBTX_E_INVALID;
+  *tree = nullptr;
+  hbtx_tree *T = new (std::nothrow) hbtx_tree();
+  if (!T)
+    return HBTX_E_ALLOC;
+  try
+  {
+    HIPCHK(hipSetDevice(c->device));
+    build_tree(c, *prm, n, pos, mass, T, 1);
+  }
+  catch (const Error &e)
+  {
+    delete T;
+    return fail(c, e.code, e.what());
+  }
+  *tree = T;
+  if (ncells)
+    *ncells = T->ncells;
+  return HBTX_OK;
+}
+
+int hbtx_tree_potential(hbtx_ctx *c, hbtx_tree *T, int64_t n_tgt, const float *tgt_pos, const float *tgt_mass,
+                        double *out_pot)
+{
+  if (!c || !T || (n_tgt > 0 && (!tgt_pos || !out_pot)))
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    tree_points(c, T, n_tgt, tgt_pos, tgt_mass, out_pot);
+  });
+}
+
+int hbtx_tree_binding_energy(hbtx_ctx *c, hbtx_tree *T, int64_t n_tgt, const float *tgt_pos, const float *tgt_vel,
+                             const float *ref_pos, const float *ref_vel, const float *tgt_mass, double *out_energy)
+{
+  if (!c || !T || (n_tgt > 0 && (!tgt_pos || !tgt_vel || !ref_pos || !ref_vel || !out_energy)))
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    tree_points(c, T, n_tgt, tgt_pos, tgt_mass, out_energy);
+    const float ha = T->P.hz * T->P.a;
+    for (int64_t i = 0; i < n_tgt; i++)
+    {
+      // Snapshot_t::RelativeVelocity in float, then VecNorm * 0.5 + pot
+      float k2 = 0.f;
+      for (int j = 0; j < 3; j++)
+      {
+        float dx = tgt_pos[3 * i + j] - ref_pos[j];
+        if (T->P.periodic)
+          dx = (dx > T->P.box_half) ? dx - T->P.box : ((dx < -T->P.box_half) ? dx + T->P.box : dx);
+        float dv = tgt_vel[3 * i + j] - ref_vel[j];
+        dv += ha * dx;
+        k2 += dv * dv;
+      }
+      out_energy[i] = (double)k2 * 0.5 + out_energy[i];
+    }
+  });
+}
+
+int hbtx_tree_export(hbtx_ctx *c, hbtx_tree *T, int64_t cap, int32_t *kind, int64_t *pidx, float *mass, float *com,
+                     float *len, int64_t *skip, int32_t *depth, int64_t *nentries)
+{
+  if (!c || !T)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    const int32_t ne = T->ne;
+    if (nentries)
+      *nentries = ne;
+    REQUIRE(cap >= ne, HBTX_E_CAPACITY, "export capacity too small");
+    std::vector<float4> pm(ne);
+    std::vector<int2> aux(ne);
+    std::vector<int8_t> dep(ne);
+    std::vector<int32_t> src(T->n);
+    HIPCHK(hipMemcpy(pm.data(), T->ent_pm.p, ne * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(aux.data(), T->ent_aux.p, ne * sizeof(int2), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dep.data(), T->ent_depth.p, ne, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(src.data(), T->src_of_entry.p, T->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    int64_t pi = 0;
+    for (int32_t q = 0; q < ne; q++)
+    {
+      const bool part = dep[q] < 0;
+      if (kind)
+        kind[q] = part ? 0 : 1;
+      if (pidx)
+        pidx[q] = part ? (int64_t)src[pi] : -1;
+      if (part)
+        pi++;
+      if (mass)
+        mass[q] = pm[q].w;
+      if (com)
+      {
+        com[3 * q] = pm[q].x;
+        com[3 * q + 1] = pm[q].y;
+        com[3 * q + 2] = pm[q].z;
+      }
+      float l2;
+      std::memcpy(&l2, &aux[q].x, sizeof(float));
+      if (len)
+        len[q] = part ? 0.f : std::sqrt(l2);
+      if (skip)
+        skip[q] = aux[q].y;
+      if (depth)
+        depth[q] = dep[q];
+    }
+  });
+}
+
+void hbtx_tree_free(hbtx_tree *T)
+{
+  delete T;
+}
+
+int hbtx_direct_potential(hbtx_ctx *c, const hbtx_params *prm, int64_t n_src, const float *src_pos,
+                          const float *src_mass, int64_t n_tgt, const float *tgt_pos, const float *tgt_mass,
+                          double *out_pot)
+{
+  if (!c || !prm)
+    return HBTX_E_INVALID;
+  hbtx_tree T;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    build_tree(c, *prm, n_src, src_pos, src_mass, &T, 0);
+    tree_points(c, &T, n_tgt, tgt_pos, tgt_mass, out_pot);
+  });
+}
+
+int hbtx_segmented_sort(hbtx_ctx *c, int64_t n, uint64_t *keys, int32_t *vals, int64_t nseg,
+                        const int64_t *seg_offsets)
+{
+  if (!c || (n > 0 && (!keys || !vals)) || !seg_offsets)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    REQUIRE(n < (int64_t)0x40000000 && nseg < (int64_t)0x7fffffff, HBTX_E_INVALID, "too large");
+    hipStream_t st = c->stream;
+    DArr<uint64_t> dk;
+    DArr<int32_t> dv, sb, sl;
+    dk.alloc(n);
+    dv.alloc(n);
+    sb.alloc(nseg);
+    sl.alloc(nseg);
+    std::vector<int32_t> hb(nseg), hl(nseg);
+    for (int64_t s = 0; s < nseg; s++)
+    {
+      REQUIRE(seg_offsets[s + 1] >= seg_offsets[s], HBTX_E_INVALID, "bad segments");
+      hb[s] = (int32_t)seg_offsets[s];
+      hl[s] = (int32_t)(seg_offsets[s + 1] - seg_offsets[s]);
+    }
+    HIPCHK(hipMemcpyAsync(dk.p, keys, n * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dv.p, vals, n * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(sb.p, hb.data(), nseg * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(sl.p, hl.data(), nseg * 4, hipMemcpyHostToDevice, st));
+    seg_sort(c, dk.p, dv.p, (int32_t)nseg, sb.p, sl.p, n, 64);
+    HIPCHK(hipMemcpyAsync(keys, dk.p, n * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(vals, dv.p, n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  });
+}
+
+} // extern "C"

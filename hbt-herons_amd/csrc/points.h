@@ -1,0 +1,86 @@
+// Potential at arbitrary points (GravityTree_t::EvaluatePotential on a batch of
+// targets, src/gravity_tree.cpp:82-166) and the exact direct sum.
+#pragma once
+#include "walk.h"
+
+namespace hbtx
+{
+
+struct PointArgs
+{
+  const float4 *tgt;       // target position + self mass (targetMass)
+  const int32_t *tgt_order;// walk order of targets (spatially sorted) or null
+  int32_t ntgt;
+  int32_t eb, ee;          // entry range
+  int32_t treemode;
+  const float4 *__restrict__ ent_pm;
+  const int2 *__restrict__ ent_aux;
+  float *phi;              // specific potential, scattered to the target's input index
+  double *phi_d;           // optional double output
+  int32_t *counter;
+  unsigned long long *stats;
+  DevParams P;
+};
+
+__global__ __launch_bounds__(64) void k_points_potential(PointArgs A)
+{
+  const int lane = lane_id();
+  unsigned long long st[NSTATS];
+#pragma unroll
+  for (int k = 0; k < NSTATS; k++)
+    st[k] = 0;
+  const int nitems = (A.ntgt + HBTX_WAVE - 1) / HBTX_WAVE;
+  WalkArgs W;
+  W.ent_pm = A.ent_pm;
+  W.ent_aux = A.ent_aux;
+  W.P = A.P;
+  for (;;)
+  {
+    int it = 0;
+    if (lane == 0)
+      it = atomicAdd(A.counter, 1);
+    it = uniform(__shfl(it, 0, HBTX_WAVE));
+    if (it >= nitems)
+      break;
+    const int t = it * HBTX_WAVE + lane;
+    const bool valid = t < A.ntgt;
+    const int ti = valid ? (A.tgt_order ? A.tgt_order[t] : t) : 0;
+    float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid)
+      x = A.tgt[ti];
+    Kahan acc;
+    acc.init();
+    if (A.treemode)
+      tree_walk(W, A.eb, A.ee, valid, x.x, x.y, x.z, acc, st);
+    else
+      direct_sum(W, A.eb, A.ee, valid, x.x, x.y, x.z, acc, st);
+    if (valid)
+    {
+      const float self = x.w / A.P.eps;
+      const double phi = ((double)self + (double)acc.s - (double)acc.c) * A.P.G_over_a;
+      A.phi[ti] = (float)phi;
+      if (A.phi_d)
+        A.phi_d[ti] = phi;
+    }
+  }
+  if (A.stats && lane == 0)
+  {
+#pragma unroll
+    for (int k = 0; k < NSTATS; k++)
+      if (st[k])
+        atomicAdd(A.stats + k, st[k]);
+  }
+}
+
+// keys of external target points in a tree's cube (walk order for coherence)
+__global__ void k_point_keys(const float4 *tgt, int32_t n, const double *cube, uint64_t *keys, int32_t *vals)
+{
+  GRID_STRIDE(i, n)
+  {
+    const float4 x = tgt[i];
+    keys[i] = octree_key(x.x, x.y, x.z, cube[0], cube[1], cube[2], cube[3]);
+    vals[i] = (int32_t)i;
+  }
+}
+
+} // namespace hbtx

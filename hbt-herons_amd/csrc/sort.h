@@ -1,0 +1,217 @@
+// Segmented sort of (uint64 key, int32 value) pairs.
+//  * segments of <= SORT_SMALL_MAX elements: one workgroup each, bitonic sort in LDS;
+//  * larger segments: tiled LSD radix sort (8-bit digits) where every tile of
+//    SORT_TILE elements ranks its keys stably with wave ballots (match of the
+//    8 digit bits) and LDS per-digit counters, so all large segments advance
+//    together in one launch per pass.
+#pragma once
+#include "common.h"
+#include "scan.h"
+
+namespace hbtx
+{
+#define SORT_THREADS 256
+#define SORT_SMALL_MAX 2048
+#define SORT_TILE 4096
+#define RADIX_BITS 8
+#define RADIX_BINS 256
+
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_small(uint64_t *__restrict__ keys, int32_t *__restrict__ vals,
+                                                            const int32_t *__restrict__ seg_begin,
+                                                            const int32_t *__restrict__ seg_len, int32_t nseg)
+{
+  __shared__ uint64_t sk[SORT_SMALL_MAX];
+  __shared__ int32_t sv[SORT_SMALL_MAX];
+  const int tid = threadIdx.x;
+  for (int s = blockIdx.x; s < nseg; s += gridDim.x)
+  {
+    const int len = seg_len[s];
+    if (len <= 1 || len > SORT_SMALL_MAX)
+      continue;
+    const int b = seg_begin[s];
+    int P = 1;
+    while (P < len)
+      P <<= 1;
+    for (int i = tid; i < P; i += SORT_THREADS)
+    {
+      sk[i] = (i < len) ? keys[b + i] : ~0ull;
+      sv[i] = (i < len) ? vals[b + i] : 0;
+    }
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1)
+      {
+        for (int i = tid; i < P; i += SORT_THREADS)
+        {
+          const int ixj = i ^ j;
+          if (ixj > i)
+          {
+            const bool up = (i & k) == 0;
+            const uint64_t a = sk[i], c = sk[ixj];
+            if ((a > c) == up)
+            {
+              sk[i] = c;
+              sk[ixj] = a;
+              const int32_t t = sv[i];
+              sv[i] = sv[ixj];
+              sv[ixj] = t;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    for (int i = tid; i < len; i += SORT_THREADS)
+    {
+      keys[b + i] = sk[i];
+      vals[b + i] = sv[i];
+    }
+    __syncthreads();
+  }
+}
+
+// Number of radix tiles of every segment (0 for segments sorted in LDS).
+__global__ void k_sort_ntiles(const int32_t *__restrict__ seg_len, int32_t nseg, int32_t *__restrict__ ntile)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nseg)
+  {
+    const int len = seg_len[s];
+    ntile[s] = (len > SORT_SMALL_MAX) ? (len + SORT_TILE - 1) / SORT_TILE : 0;
+  }
+}
+
+// tiles[t] = (segment, begin, end, 0)
+__global__ void k_sort_tiles(const int32_t *__restrict__ seg_begin, const int32_t *__restrict__ seg_len,
+                             const int32_t *__restrict__ tile_off, int32_t nseg, int4 *__restrict__ tiles)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg)
+    return;
+  const int len = seg_len[s];
+  if (len <= SORT_SMALL_MAX)
+    return;
+  const int b = seg_begin[s], e = b + len;
+  int t = tile_off[s];
+  for (int x = b; x < e; x += SORT_TILE, t++)
+    tiles[t] = make_int4(s, x, min(e, x + SORT_TILE), 0);
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint64_t *__restrict__ kin,
+                                                            const int4 *__restrict__ tiles,
+                                                            const int32_t *__restrict__ ntiles_p, int shift,
+                                                            int32_t *__restrict__ ghist)
+{
+  __shared__ int32_t hist[RADIX_BINS];
+  const int nt = *ntiles_p;
+  const int tid = threadIdx.x;
+  for (int t = blockIdx.x; t < nt; t += gridDim.x)
+  {
+    hist[tid] = 0;
+    __syncthreads();
+    const int4 tl = tiles[t];
+    for (int i = tl.y + tid; i < tl.z; i += SORT_THREADS)
+      atomicAdd(&hist[(int)((kin[i] >> shift) & (RADIX_BINS - 1))], 1);
+    __syncthreads();
+    ghist[(int64_t)t * RADIX_BINS + tid] = hist[tid];
+    __syncthreads();
+  }
+}
+
+// Per large segment: turn tile histograms into global scatter bases.
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_offsets(const int32_t *__restrict__ seg_begin,
+                                                               const int32_t *__restrict__ seg_len,
+                                                               const int32_t *__restrict__ tile_off,
+                                                               const int32_t *__restrict__ ntile, int32_t nseg,
+                                                               int32_t *__restrict__ ghist)
+{
+  __shared__ int32_t lds[SORT_THREADS / HBTX_WAVE];
+  const int d = threadIdx.x;
+  for (int s = blockIdx.x; s < nseg; s += gridDim.x)
+  {
+    if (seg_len[s] <= SORT_SMALL_MAX)
+      continue;
+    const int t0 = tile_off[s], nt = ntile[s];
+    int run = 0;
+    for (int t = t0; t < t0 + nt; t++)
+    {
+      const int64_t q = (int64_t)t * RADIX_BINS + d;
+      const int c = ghist[q];
+      ghist[q] = run;
+      run += c;
+    }
+    int tot;
+    const int base = block_excl_scan(run, lds, &tot) + seg_begin[s];
+    for (int t = t0; t < t0 + nt; t++)
+      ghist[(int64_t)t * RADIX_BINS + d] += base;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *__restrict__ kin,
+                                                               const int32_t *__restrict__ vin,
+                                                               uint64_t *__restrict__ kout,
+                                                               int32_t *__restrict__ vout,
+                                                               const int4 *__restrict__ tiles,
+                                                               const int32_t *__restrict__ ntiles_p, int shift,
+                                                               const int32_t *__restrict__ ghist)
+{
+  __shared__ int32_t run[RADIX_BINS];
+  __shared__ int32_t wcnt[SORT_THREADS / HBTX_WAVE][RADIX_BINS];
+  const int nt = *ntiles_p;
+  const int tid = threadIdx.x;
+  const int w = tid / HBTX_WAVE;
+  const int lane = lane_id();
+  for (int t = blockIdx.x; t < nt; t += gridDim.x)
+  {
+    run[tid] = 0;
+#pragma unroll
+    for (int k = 0; k < SORT_THREADS / HBTX_WAVE; k++)
+      wcnt[k][tid] = 0;
+    __syncthreads();
+    const int4 tl = tiles[t];
+    const int32_t *off = ghist + (int64_t)t * RADIX_BINS;
+    for (int r = 0; r < SORT_TILE / SORT_THREADS; r++)
+    {
+      if (tl.y + r * SORT_THREADS >= tl.z)
+        break; // uniform across the block
+      const int i = tl.y + r * SORT_THREADS + tid;
+      const bool valid = i < tl.z;
+      const uint64_t k = valid ? kin[i] : 0ull;
+      const int32_t v = valid ? vin[i] : 0;
+      const int d = (int)((k >> shift) & (RADIX_BINS - 1));
+      uint64_t m = ballot(valid);
+#pragma unroll
+      for (int bit = 0; bit < RADIX_BITS; bit++)
+      {
+        const bool on = (d >> bit) & 1;
+        const uint64_t bb = ballot(valid && on);
+        m &= on ? bb : ~bb;
+      }
+      const int lrank = __popcll(m & lanemask_lt());
+      if (valid && (m >> lane) == 1ull)
+        wcnt[w][d] = __popcll(m);
+      __syncthreads();
+      if (valid)
+      {
+        int rank = run[d] + lrank;
+        for (int w2 = 0; w2 < w; w2++)
+          rank += wcnt[w2][d];
+        const int dst = off[d] + rank;
+        kout[dst] = k;
+        vout[dst] = v;
+      }
+      __syncthreads();
+      int acc = 0;
+#pragma unroll
+      for (int k2 = 0; k2 < SORT_THREADS / HBTX_WAVE; k2++)
+      {
+        acc += wcnt[k2][tid];
+        wcnt[k2][tid] = 0;
+      }
+      run[tid] += acc;
+      __syncthreads();
+    }
+  }
+}
+
+} // namespace hbtx

@@ -1,0 +1,199 @@
+// GPU Barnes-Hut octree with the reference's structure (single-particle leaves,
+// bounding cube, len halving rule), laid out in depth-first preorder.
+//
+// Sources of every subhalo are contiguous in a dense list and sorted by their
+// 63-bit octree key (tree cube of that subhalo).  For sorted sources j of one
+// segment, c_j = number of leading levels shared by keys j and j+1; the cells
+// of the reference octree are exactly the key prefixes shared by >= 2 sources,
+// and the cells that start at source j are the depths (c_{j-1}, c_j].  With
+// cnt_j = max(0, c_j - c_{j-1}) and cscan its exclusive prefix sum, entry
+// positions are closed-form:
+//   cell of depth l starting at j : j + cscan[j] + (l - c_{j-1} - 1)
+//   particle j                    : j + cscan[j] + cnt_j
+// and every entry's "skip" (first entry after its subtree: the reference's
+// sibling link, src/gravity_tree.cpp:50-80) is the position of the entry group
+// that follows the last source of its key range.
+//
+// Entry records: ent_pm = (centre of mass xyz, mass), ent_aux = (len^2 as float
+// bits, skip), ent_depth = depth (cells) or -1 (particles).  A particle is an
+// entry with len^2 = 0 (never opened), so the walk treats both uniformly.
+#pragma once
+#include "common.h"
+
+namespace hbtx
+{
+
+struct TreeArgs
+{
+  const int32_t *src;        // dense source list (global particle index), sorted per segment
+  const uint64_t *keys;      // sorted keys, aligned with src
+  const int32_t *sub_of;     // subhalo of every particle
+  const int32_t *src_off;    // per subhalo: first dense source
+  const int32_t *nsrc;       // per subhalo: number of sources
+  const int32_t *treemode;   // per subhalo: 1 tree, 0 direct (particle entries only)
+  const int32_t *mode;       // per subhalo: MODE_FULL / MODE_CORR
+  const double *len0;        // per subhalo: root len (cube side, src/oct_tree.tpp:357-360)
+  const float4 *posm;        // particle position + mass
+  const int8_t *ptype;       // particle type or null
+  const float *upscale;      // per subhalo [6] mass upscale factors
+  const int32_t *cnt;        // cells starting at each dense source
+  const int32_t *cscan;      // exclusive scan of cnt (size >= S+1)
+  const int32_t *s_total;    // number of dense sources
+  float4 *ent_pm;
+  int2 *ent_aux;
+  int8_t *ent_depth;
+  int32_t cap;               // entry capacity
+  int32_t *overflow;         // set to 1 if entries would exceed cap
+  DevParams P;
+};
+
+// Source mass as the reference tree sees it: GetMass with the upscale factors
+// during a full iteration (src/subhalo_unbind.cpp:91-101), the true mass in a
+// correction tree (a fresh EnergySnapshot_t, :573-575).
+HBTX_DEV float source_mass(const TreeArgs &A, int p, int s)
+{
+  const float m = A.posm[p].w;
+  if (A.mode[s] != MODE_FULL)
+    return m;
+  const float *up = A.upscale + 6 * s;
+  if (A.P.dm_only)
+    return m * up[1];
+  const int t = A.ptype ? A.ptype[p] : 1;
+  if (donot_subsample(A.P, t))
+    return m;
+  return m * up[t];
+}
+
+HBTX_DEV void pair_levels(const TreeArgs &A, int j, int b, int e, int &cprev, int &cj)
+{
+  cj = (j + 1 < e) ? common_levels(A.keys[j], A.keys[j + 1]) : ((j == b) ? 0 : -1);
+  cprev = (j > b) ? common_levels(A.keys[j - 1], A.keys[j]) : -1;
+}
+
+__global__ void k_tree_count(TreeArgs A, int32_t n_all, int32_t *__restrict__ cnt)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > n_all)
+    return;
+  const int S = *A.s_total;
+  if (j >= S)
+  {
+    cnt[j] = 0;
+    return;
+  }
+  const int p = A.src[j];
+  const int s = A.sub_of[p];
+  if (!A.treemode[s])
+  {
+    cnt[j] = 0;
+    return;
+  }
+  const int b = A.src_off[s], e = b + A.nsrc[s];
+  int cprev, cj;
+  pair_levels(A, j, b, e, cprev, cj);
+  cnt[j] = max(0, cj - cprev);
+}
+
+__global__ void k_tree_emit(TreeArgs A)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int S = *A.s_total;
+  if (j >= S)
+    return;
+  const int p = A.src[j];
+  const int s = A.sub_of[p];
+  const int P = j + A.cscan[j];
+  const float4 x = A.posm[p];
+  const float m = source_mass(A, p, s);
+  if (!A.treemode[s])
+  {
+    if (P < A.cap)
+    {
+      A.ent_pm[P] = make_float4(x.x, x.y, x.z, m);
+      A.ent_aux[P] = make_int2(0, P + 1);
+      A.ent_depth[P] = -1;
+    }
+    else
+      *A.overflow = 1;
+    return;
+  }
+  const int b = A.src_off[s], e = b + A.nsrc[s];
+  int cprev, cj;
+  pair_levels(A, j, b, e, cprev, cj);
+  const int nn = max(0, cj - cprev);
+  const int q = P + nn;
+  if (q >= A.cap)
+  {
+    *A.overflow = 1;
+    return;
+  }
+  A.ent_pm[q] = make_float4(x.x, x.y, x.z, m);
+  A.ent_aux[q] = make_int2(0, q + 1);
+  A.ent_depth[q] = -1;
+  if (nn == 0)
+    return;
+  // len of the first depth, then the reference halving rule (src/gravity_tree.cpp:38-41)
+  double L = A.len0[s];
+  for (int l = 0; l < cprev + 1; l++)
+    L = (L >= A.P.res) ? L * 0.5 : L;
+  const uint64_t kj = A.keys[j];
+  for (int l = cprev + 1; l <= cj; l++)
+  {
+    const int pos = P + (l - cprev - 1);
+    int bend;
+    if (l == 0)
+      bend = e - 1;
+    else
+    {
+      const uint64_t pj = key_prefix(kj, l);
+      int lo = j + 1, hi = e;
+      while (lo < hi)
+      {
+        const int mid = (lo + hi) >> 1;
+        if (key_prefix(A.keys[mid], l) > pj)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      bend = lo - 1;
+    }
+    const int skip = (bend + 1) + A.cscan[bend + 1];
+    const float lf = (float)L;
+    const float len2 = lf * lf;
+    A.ent_aux[pos] = make_int2(__float_as_int(len2), skip);
+    A.ent_depth[pos] = (int8_t)l;
+    A.ent_pm[pos] = make_float4(0.f, 0.f, 0.f, 0.f);
+    L = (L >= A.P.res) ? L * 0.5 : L;
+  }
+}
+
+// Mass and centre of mass of every cell at one depth from its children, in
+// child (sub-index) order with double accumulation of the float child values,
+// as UpdateInternalNodes/ProcessNode do (src/gravity_tree.cpp:25-80).
+__global__ void k_tree_moments(float4 *__restrict__ ent_pm, const int2 *__restrict__ ent_aux,
+                               const int8_t *__restrict__ ent_depth, const int32_t *__restrict__ e_total_p,
+                               int depth)
+{
+  const int E = *e_total_p;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < E; q += gridDim.x * blockDim.x)
+  {
+    if (ent_depth[q] != depth)
+      continue;
+    const int end = min(ent_aux[q].y, E);
+    double m = 0., cx = 0., cy = 0., cz = 0.;
+    int c = q + 1;
+    while (c < end)
+    {
+      const float4 e = ent_pm[c];
+      const double cm = e.w;
+      m += cm;
+      cx += (double)e.x * cm;
+      cy += (double)e.y * cm;
+      cz += (double)e.z * cm;
+      c = max(ent_aux[c].y, c + 1);
+    }
+    ent_pm[q] = make_float4((float)(cx / m), (float)(cy / m), (float)(cz / m), (float)m);
+  }
+}
+
+} // namespace hbtx

@@ -1,0 +1,831 @@
+// Per-round kernels of the batched unbinding (Subhalo_t::Unbind,
+// src/subhalo_unbind.cpp:463-808, advanced for every subhalo at once).
+//
+// Layout: particles of subhalo s occupy global indices [off[s], off[s+1]) in
+// input order ("slots").  state[p] = 0 while p is a candidate (in the bound set
+// being iterated), r > 0 once p was removed in round r.  cur[] holds, in slot
+// layout, the current candidates of each subhalo in walk order (the key order
+// of the last full tree), compacted after every round.
+#pragma once
+#include "common.h"
+#include "scan.h"
+
+namespace hbtx
+{
+
+struct SubArrays
+{
+  int32_t nsub;
+  const int32_t *off;    // [nsub+1]
+  const int32_t *sub_of; // [N]
+  int32_t *status, *corr, *iters, *mode, *treemode;
+  int32_t *nlast, *nbound, *nrem, *nuns, *ntgt, *nsrc;
+  float *mlast, *mbound;
+  float *ref, *oldref, *mb; // [nsub*6]
+  float *upscale;           // [nsub*6]
+  int32_t *src_off, *ent_begin, *ent_end;
+  int32_t *nitems;          // [nsub]
+  double *cube;             // [nsub*4] centre xyz, lenhalf
+  double *len0;             // [nsub]
+  uint32_t *bbox;           // [nsub*6] ordered min xyz, max xyz
+  int32_t *bnd_cnt, *bnd_tr;
+  int32_t *first_idx;       // smallest candidate index (periodic reference particle)
+  int32_t *active_count;    // [1]
+};
+
+struct PartArrays
+{
+  const float4 *posm;
+  const float4 *velu;
+  const int8_t *ptype; // may be null (all type 1)
+  int32_t *state;
+  float *energy;
+  float *selfm;
+  int32_t *cur, *cur2;
+};
+
+HBTX_DEV int ptype_of(const PartArrays &Q, int p)
+{
+  return Q.ptype ? (int)Q.ptype[p] : 1;
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v)
+{
+  GRID_STRIDE(i, n) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v)
+{
+  GRID_STRIDE(i, n) p[i] = v;
+}
+
+// Batch start: every particle is a candidate, frame from the caller
+// (src/subhalo_unbind.cpp:503-541).
+__global__ void k_init_particles(SubArrays S, PartArrays Q, int64_t n)
+{
+  GRID_STRIDE(p, n)
+  {
+    Q.state[p] = 0;
+    Q.cur[p] = (int32_t)p;
+    Q.energy[p] = 0.f;
+    Q.selfm[p] = 0.f;
+  }
+}
+
+__global__ void k_init_subs(SubArrays S, const float *ref_in, const float *mb_in, const uint8_t *has_proposal)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub)
+    return;
+  const int n = S.off[s + 1] - S.off[s];
+  S.status[s] = (n == 0) ? ST_CONVERGED : ST_ACTIVE;
+  S.corr[s] = 0;
+  S.iters[s] = 0;
+  S.nbound[s] = n;
+  S.nlast[s] = n;
+  S.nrem[s] = 0;
+  S.nuns[s] = 0;
+  for (int k = 0; k < 6; k++)
+  {
+    S.ref[6 * s + k] = ref_in[6 * s + k];
+    S.oldref[6 * s + k] = 0.f;
+    S.mb[6 * s + k] = mb_in[6 * s + k];
+    S.upscale[6 * s + k] = 1.f;
+  }
+}
+
+// Mode of the round (:563-600); subsampling keeps MaxSampleSize+Nunsample sources.
+__global__ void k_plan(SubArrays S, DevParams P, int round)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub)
+    return;
+  if (S.status[s] != ST_ACTIVE)
+  {
+    S.mode[s] = MODE_IDLE;
+    S.ntgt[s] = 0;
+    S.nsrc[s] = 0;
+    S.nitems[s] = 0;
+    S.treemode[s] = 0;
+    return;
+  }
+  const int nb = S.nbound[s];
+  int ns;
+  if (S.corr[s])
+  {
+    S.mode[s] = MODE_CORR;
+    ns = S.nrem[s];
+  }
+  else
+  {
+    S.mode[s] = MODE_FULL;
+    S.mlast[s] = S.mbound[s];
+    // SetMassUpscaleFactor (:374-391)
+    if (P.mss == 0 || (int64_t)nb < P.mss + S.nuns[s])
+      ns = nb;
+    else
+      ns = (int)(P.mss + S.nuns[s]);
+  }
+  S.nlast[s] = nb;
+  S.ntgt[s] = nb;
+  S.nsrc[s] = ns;
+  S.treemode[s] = ((int64_t)ns > P.direct_cutoff) ? 1 : 0;
+  S.nitems[s] = (nb + HBTX_WAVE - 1) / HBTX_WAVE;
+  S.iters[s] = round;
+}
+
+// Candidate rank in input order (the E list order between stable partitions):
+// cand[p] = 1 for candidates of full-mode subhaloes (exclusive scan -> rank).
+__global__ void k_mark_candidates(SubArrays S, PartArrays Q, int64_t n, int32_t *flag)
+{
+  GRID_STRIDE(p, n)
+  {
+    const int s = S.sub_of[p];
+    flag[p] = (S.mode[s] == MODE_FULL && Q.state[p] == 0) ? 1 : 0;
+  }
+}
+
+// Source flags: full mode -> candidates with rank < nsrc; correction -> removed
+// in the previous round.  Also the self mass used to cancel the self term.
+__global__ void k_mark_sources(SubArrays S, PartArrays Q, DevParams P, int64_t n, int round, const int32_t *crank,
+                               int32_t *flag)
+{
+  GRID_STRIDE(p, n)
+  {
+    const int s = S.sub_of[p];
+    const int md = S.mode[s];
+    int f = 0;
+    float sm = 0.f;
+    if (md == MODE_FULL && Q.state[p] == 0)
+    {
+      const int rank = crank[p] - crank[S.off[s]];
+      if (rank < S.nsrc[s])
+      {
+        f = 1;
+        const float m = Q.posm[p].w;
+        const float *up = S.upscale + 6 * s;
+        const int t = ptype_of(Q, p);
+        sm = P.dm_only ? m * up[1] : (donot_subsample(P, t) ? m : m * up[t]);
+      }
+    }
+    else if (md == MODE_CORR && Q.state[p] == round - 1)
+      f = 1;
+    flag[p] = f;
+    Q.selfm[p] = sm;
+  }
+}
+
+__global__ void k_scatter_sources(SubArrays S, int64_t n, const int32_t *flag, const int32_t *fscan, int32_t *src)
+{
+  GRID_STRIDE(p, n)
+  {
+    if (flag[p])
+      src[fscan[p]] = (int32_t)p;
+  }
+}
+
+__global__ void k_sub_src(SubArrays S, const int32_t *fscan)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub)
+    return;
+  S.src_off[s] = fscan[S.off[s]];
+  const uint32_t big = float_to_ordered(3.0e38f), small = float_to_ordered(-3.0e38f);
+  for (int k = 0; k < 3; k++)
+  {
+    S.bbox[6 * s + k] = big;
+    S.bbox[6 * s + 3 + k] = small;
+  }
+}
+
+// Bounding box of each tree-mode subhalo's sources (src/oct_tree.tpp:344-355).
+// Waves whose lanes share one subhalo reduce first, then one atomic per wave.
+__global__ void k_bbox(SubArrays S, const float4 *posm, const int32_t *src, const int32_t *s_total)
+{
+  const int ST = *s_total;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < ST; base += (int64_t)gridDim.x * blockDim.x)
+  {
+    const int64_t j = base + threadIdx.x;
+    const bool valid = j < ST;
+    int s = -1;
+    float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid)
+    {
+      const int p = src[j];
+      s = S.sub_of[p];
+      if (!S.treemode[s])
+        s = -1;
+      else
+        x = posm[p];
+    }
+    const int s0 = __shfl(s, 0, HBTX_WAVE);
+    const bool same = ballot(s != s0) == 0ull;
+    if (same)
+    {
+      if (s0 < 0)
+        continue;
+      float mn[3] = {x.x, x.y, x.z}, mx[3] = {x.x, x.y, x.z};
+#pragma unroll
+      for (int d = 1; d < HBTX_WAVE; d <<= 1)
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+        {
+          mn[k] = fminf(mn[k], __shfl_xor(mn[k], d, HBTX_WAVE));
+          mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], d, HBTX_WAVE));
+        }
+      if (lane_id() == 0)
+        for (int k = 0; k < 3; k++)
+        {
+          atomicMin(S.bbox + 6 * s0 + k, float_to_ordered(mn[k]));
+          atomicMax(S.bbox + 6 * s0 + 3 + k, float_to_ordered(mx[k]));
+        }
+    }
+    else if (s >= 0)
+    {
+      const float xs[3] = {x.x, x.y, x.z};
+      for (int k = 0; k < 3; k++)
+      {
+        atomicMin(S.bbox + 6 * s + k, float_to_ordered(xs[k]));
+        atomicMax(S.bbox + 6 * s + 3 + k, float_to_ordered(xs[k]));
+      }
+    }
+  }
+}
+
+// Root cube: Len = max extent, Center = bbox midpoint (src/oct_tree.tpp:357-365).
+__global__ void k_cube(SubArrays S)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub || !S.treemode[s] || S.mode[s] == MODE_IDLE)
+    return;
+  double xmin[3], xmax[3];
+  for (int k = 0; k < 3; k++)
+  {
+    xmin[k] = ordered_to_float(S.bbox[6 * s + k]);
+    xmax[k] = ordered_to_float(S.bbox[6 * s + 3 + k]);
+  }
+  double Len = xmax[0] - xmin[0];
+  for (int j = 1; j < 3; j++)
+    if ((xmax[j] - xmin[j]) > Len)
+      Len = xmax[j] - xmin[j];
+  for (int j = 0; j < 3; j++)
+    S.cube[4 * s + j] = 0.5 * (xmax[j] + xmin[j]);
+  S.cube[4 * s + 3] = 0.5 * Len;
+  S.len0[s] = Len;
+}
+
+__global__ void k_keys(SubArrays S, DevParams P, const float4 *posm, const int32_t *src, const int32_t *s_total,
+                       uint64_t *keys)
+{
+  const int ST = *s_total;
+  GRID_STRIDE(j, ST)
+  {
+    const int p = src[j];
+    const int s = S.sub_of[p];
+    if (S.treemode[s])
+    {
+      const float4 x = posm[p];
+      const double *c = S.cube + 4 * s;
+      const uint64_t k = octree_key(x.x, x.y, x.z, c[0], c[1], c[2], c[3]);
+      keys[j] = hash_digits(k, resolution_depth(c[3], 0.5 * P.res), (uint32_t)(p - S.off[s]));
+    }
+    else
+      keys[j] = 0ull;
+  }
+}
+
+// Sort segments: tree-mode sources only.
+__global__ void k_sort_segments(SubArrays S, int32_t *seg_begin, int32_t *seg_len)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub)
+    return;
+  seg_begin[s] = S.src_off[s];
+  seg_len[s] = (S.mode[s] != MODE_IDLE && S.treemode[s]) ? S.nsrc[s] : 0;
+}
+
+__global__ void k_sub_entries(SubArrays S, const int32_t *cscan, int32_t cap)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub)
+    return;
+  const int b = S.src_off[s], e = b + S.nsrc[s];
+  // clamped so that an overflowing tree (reported to the host) is never read past the buffer
+  S.ent_begin[s] = min(b + cscan[b], cap);
+  S.ent_end[s] = min(e + cscan[e], cap);
+}
+
+__global__ void k_entry_total(const int32_t *s_total, const int32_t *cscan, int32_t *e_total, int32_t cap,
+                              int32_t *overflow)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+  {
+    const int S = *s_total;
+    const int E = S + cscan[S];
+    *e_total = min(E, cap);
+    if (E > cap)
+      *overflow = 1;
+  }
+}
+
+// Work items in subhalo size order (largest first).
+__global__ void k_items_count(SubArrays S, const int32_t *order, int32_t *cnt)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < S.nsub)
+    cnt[i] = S.nitems[order[i]];
+}
+
+__global__ void k_items_emit(SubArrays S, const int32_t *order, const int32_t *iscan, int2 *items)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S.nsub)
+    return;
+  const int s = order[i];
+  const int n = S.nitems[s];
+  const int b = iscan[i];
+  for (int k = 0; k < n; k++)
+    items[b + k] = make_int2(s, k * HBTX_WAVE);
+}
+
+// Partition flags over slots: low 32 bits = bound, high 32 bits = bound tracer.
+__global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, int64_t n, int64_t *flag)
+{
+  GRID_STRIDE(q, n)
+  {
+    const int s = S.sub_of[q];
+    const int k = (int)(q - S.off[s]);
+    int64_t f = 0;
+    if (S.mode[s] != MODE_IDLE && k < S.ntgt[s])
+    {
+      const int p = Q.cur[q];
+      if (Q.energy[p] < 0.f)
+        f = 1 | ((int64_t)(is_tracer(P, ptype_of(Q, p)) ? 1 : 0) << 32);
+    }
+    flag[q] = f;
+  }
+}
+
+// RemoveUnboundParticles (:35-44): stable compaction of the bound candidates;
+// removed particles are tagged with the round.
+__global__ void k_partition_scatter(SubArrays S, PartArrays Q, int64_t n, int round, const int64_t *fscan)
+{
+  GRID_STRIDE(q, n)
+  {
+    const int s = S.sub_of[q];
+    const int k = (int)(q - S.off[s]);
+    if (S.mode[s] != MODE_IDLE && k < S.ntgt[s])
+    {
+      const int p = Q.cur[q];
+      const int64_t f = fscan[q + 1] - fscan[q];
+      if (f & 0xffffffffll)
+      {
+        const int r = (int)((fscan[q] - fscan[S.off[s]]) & 0xffffffffll);
+        Q.cur2[S.off[s] + r] = p;
+      }
+      else
+        Q.state[p] = round;
+    }
+  }
+}
+
+__global__ void k_copy_cur(SubArrays S, PartArrays Q, int64_t n, const int64_t *fscan)
+{
+  GRID_STRIDE(q, n)
+  {
+    const int s = S.sub_of[q];
+    const int k = (int)(q - S.off[s]);
+    if (S.mode[s] != MODE_IDLE)
+    {
+      const int nb = (int)((fscan[S.off[s + 1]] - fscan[S.off[s]]) & 0xffffffffll);
+      if (k < nb)
+        Q.cur[q] = Q.cur2[q];
+    }
+  }
+}
+
+// Disruption test and correction decision (:625-683).
+__global__ void k_round_decide(SubArrays S, DevParams P, const int64_t *fscan)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub || S.mode[s] == MODE_IDLE)
+    return;
+  const int64_t d = fscan[S.off[s + 1]] - fscan[S.off[s]];
+  const int nb = (int)(d & 0xffffffffll);
+  const int ntr = P.dm_only ? nb : (int)(d >> 32);
+  const int nlast = S.nlast[s];
+  S.nbound[s] = nb;
+  S.nrem[s] = nlast - nb;
+  S.bnd_cnt[s] = nb;
+  S.bnd_tr[s] = ntr;
+  if ((int64_t)nb < P.min_part || (int64_t)ntr < P.min_tracer)
+  {
+    S.status[s] = ST_DISRUPTED;
+    for (int k = 0; k < 6; k++)
+      S.ref[6 * s + k] = S.mb[6 * s + k];
+    S.nbound[s] = 0;
+    S.mbound[s] = 0.f;
+    return;
+  }
+  const int ndiff = nlast - nb;
+  if (ndiff < nb)
+    if (P.mss <= 0 || (int64_t)ndiff < P.mss)
+    {
+      S.corr[s] = 1;
+      for (int k = 0; k < 6; k++)
+        S.oldref[6 * s + k] = S.ref[6 * s + k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic segmented reductions over slot ranges [off[s], off[s]+len[s]):
+// fixed tiles, block tree-reduction per tile, per-subhalo sum of tiles in order.
+// ---------------------------------------------------------------------------
+#define RED_THREADS 256
+#define RED_TILE 4096
+#define RED_NV 16
+
+HBTX_DEV void block_reduce_vec(double v[RED_NV], double *lds /* [RED_THREADS/64][RED_NV] */)
+{
+#pragma unroll
+  for (int k = 0; k < RED_NV; k++)
+  {
+    double x = v[k];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+      x += __shfl_down(x, d, HBTX_WAVE);
+    v[k] = x;
+  }
+  const int w = threadIdx.x / HBTX_WAVE;
+  if (lane_id() == 0)
+#pragma unroll
+    for (int k = 0; k < RED_NV; k++)
+      lds[w * RED_NV + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < RED_NV; k++)
+    {
+      double x = 0.;
+      for (int ww = 0; ww < RED_THREADS / HBTX_WAVE; ww++)
+        x += lds[ww * RED_NV + k];
+      v[k] = x;
+    }
+  __syncthreads();
+}
+
+__global__ void k_red_ntiles(const int32_t *len, int32_t nsub, int32_t *ntile)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsub)
+    ntile[s] = (len[s] + RED_TILE - 1) / RED_TILE;
+}
+
+__global__ void k_red_tiles(const int32_t *begin, const int32_t *len, const int32_t *toff, int32_t nsub, int4 *tiles)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsub)
+    return;
+  const int b = begin[s], e = b + len[s];
+  int t = toff[s];
+  for (int x = b; x < e; x += RED_TILE, t++)
+    tiles[t] = make_int4(s, x, min(e, x + RED_TILE), 0);
+}
+
+// Centre-of-mass frame of the bound candidates (CentreOfMassReferenceFrame, :151-198)
+// v[0]=M, v[1..3]=sum m x (float products, or double NEAREST offsets when periodic), v[4..6]=sum m v
+struct ComOp
+{
+  HBTX_DEV static void acc(const SubArrays &S, const PartArrays &Q, const DevParams &P, int s, int q, double v[RED_NV])
+  {
+    const int p = Q.cur[q];
+    const float4 x = Q.posm[p];
+    const float4 u = Q.velu[p];
+    const float m = x.w;
+    v[0] += m;
+    if (P.periodic)
+    {
+      const float4 r0 = Q.posm[S.first_idx[s]];
+      v[1] += nearest_d((double)x.x - (double)r0.x, P.box, P.box_half) * m;
+      v[2] += nearest_d((double)x.y - (double)r0.y, P.box, P.box_half) * m;
+      v[3] += nearest_d((double)x.z - (double)r0.z, P.box, P.box_half) * m;
+    }
+    else
+    {
+      v[1] += x.x * m;
+      v[2] += x.y * m;
+      v[3] += x.z * m;
+    }
+    v[4] += u.x * m;
+    v[5] += u.y * m;
+    v[6] += u.z * m;
+  }
+};
+
+template <class Op>
+__global__ __launch_bounds__(RED_THREADS) void k_red_tile(SubArrays S, PartArrays Q, DevParams P, const int4 *tiles,
+                                                         const int32_t *ntiles_p, double *partial)
+{
+  __shared__ double lds[(RED_THREADS / HBTX_WAVE) * RED_NV];
+  const int nt = *ntiles_p;
+  for (int t = blockIdx.x; t < nt; t += gridDim.x)
+  {
+    const int4 tl = tiles[t];
+    double v[RED_NV];
+#pragma unroll
+    for (int k = 0; k < RED_NV; k++)
+      v[k] = 0.;
+    for (int q = tl.y + threadIdx.x; q < tl.z; q += RED_THREADS)
+      Op::acc(S, Q, P, tl.x, q, v);
+    block_reduce_vec(v, lds);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < RED_NV; k++)
+        partial[(int64_t)t * RED_NV + k] = v[k];
+  }
+}
+
+__global__ void k_red_finalize(const int32_t *toff, const int32_t *ntile, int32_t nsub, const double *partial,
+                               double *out)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsub)
+    return;
+  double v[RED_NV];
+  for (int k = 0; k < RED_NV; k++)
+    v[k] = 0.;
+  const int t0 = toff[s], n = ntile[s];
+  for (int t = t0; t < t0 + n; t++)
+    for (int k = 0; k < RED_NV; k++)
+      v[k] += partial[(int64_t)t * RED_NV + k];
+  for (int k = 0; k < RED_NV; k++)
+    out[(int64_t)s * RED_NV + k] = v[k];
+}
+
+// Reduction range of the CoM: bound candidates of subhaloes still active after the decision.
+__global__ void k_com_ranges(SubArrays S, int32_t *begin, int32_t *len)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub)
+    return;
+  begin[s] = S.off[s];
+  len[s] = (S.mode[s] != MODE_IDLE && S.status[s] == ST_ACTIVE) ? S.nbound[s] : 0;
+}
+
+// smallest input index among bound candidates (the E list's first particle)
+__global__ void k_first_index(SubArrays S, PartArrays Q, int64_t n)
+{
+  GRID_STRIDE(q, n)
+  {
+    const int s = S.sub_of[q];
+    const int k = (int)(q - S.off[s]);
+    if (S.mode[s] != MODE_IDLE && S.status[s] == ST_ACTIVE && k < S.nbound[s])
+      atomicMin(S.first_idx + s, Q.cur[q]);
+  }
+}
+
+// Frame update and convergence test (:687-690).
+__global__ void k_round_finish(SubArrays S, DevParams P, const double *red)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub || S.mode[s] == MODE_IDLE || S.status[s] != ST_ACTIVE)
+    return;
+  const double *v = red + (int64_t)s * RED_NV;
+  const double M = v[0];
+  double wp[3] = {v[1] / M, v[2] / M, v[3] / M};
+  if (P.periodic)
+  {
+    // Q.posm is not visible here; the reference position is added on the host-side
+    // kernel k_round_finish_periodic.
+  }
+  for (int k = 0; k < 3; k++)
+  {
+    S.ref[6 * s + k] = (float)wp[k];
+    S.ref[6 * s + 3 + k] = (float)(v[4 + k] / M);
+  }
+  S.mbound[s] = (float)M;
+  const int64_t nb = S.nbound[s];
+  const int64_t nlast = S.nlast[s];
+  if (nb >= nlast * P.prec)
+    S.status[s] = ST_CONVERGED;
+  else
+    atomicAdd(S.active_count, 1);
+}
+
+__global__ void k_round_finish_periodic(SubArrays S, PartArrays Q, DevParams P, const double *red)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub || S.mode[s] == MODE_IDLE || S.status[s] == ST_DISRUPTED)
+    return;
+  const double *v = red + (int64_t)s * RED_NV;
+  const double M = v[0];
+  const float4 r0 = Q.posm[S.first_idx[s]];
+  S.ref[6 * s + 0] = (float)(v[1] / M + (double)r0.x);
+  S.ref[6 * s + 1] = (float)(v[2] / M + (double)r0.y);
+  S.ref[6 * s + 2] = (float)(v[3] / M + (double)r0.z);
+}
+
+// ---------------------------------------------------------------------------
+// Final ordering and per-subhalo properties
+// ---------------------------------------------------------------------------
+
+// Sort key of every particle: bound first by energy, then the unbound batches
+// with the latest round first, each by energy (:673, :710, :744-750).
+__global__ void k_final_keys(SubArrays S, PartArrays Q, int64_t n, uint64_t *keys, int32_t *vals)
+{
+  GRID_STRIDE(p, n)
+  {
+    const int s = S.sub_of[p];
+    uint64_t key;
+    if (S.status[s] == ST_CONVERGED)
+    {
+      const int st = Q.state[p];
+      const uint64_t group = (st == 0) ? 0ull : (uint64_t)(0x10000 - st);
+      key = (group << 32) | (uint64_t)float_to_ordered(Q.energy[p]);
+    }
+    else
+      key = (uint64_t)(p - S.off[s]);
+    keys[p] = key;
+    vals[p] = (int32_t)p;
+  }
+}
+
+__global__ void k_final_segments(SubArrays S, int32_t *seg_begin, int32_t *seg_len)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub)
+    return;
+  seg_begin[s] = S.off[s];
+  seg_len[s] = (S.status[s] == ST_CONVERGED) ? (S.off[s + 1] - S.off[s]) : 0;
+}
+
+// After the final sort: vals[q] = particle in final slot q.
+// AverageKinematics (:200-249) sums over the bound part in final order:
+// v[0]=sum E m, v[1]=sum u m, v[2]=sum K m (2x), v[3..5]=L m, v[6]=M,
+// v[7..12]=per-type mass, v[13] unused.
+struct FinalOp
+{
+  const int32_t *vals;
+};
+
+__global__ __launch_bounds__(RED_THREADS) void k_final_tile(SubArrays S, PartArrays Q, DevParams P, const int4 *tiles,
+                                                           const int32_t *ntiles_p, const int32_t *vals,
+                                                           double *partial, int32_t *type_cnt, int32_t *first_tracer)
+{
+  __shared__ double lds[(RED_THREADS / HBTX_WAVE) * RED_NV];
+  const int nt = *ntiles_p;
+  for (int t = blockIdx.x; t < nt; t += gridDim.x)
+  {
+    const int4 tl = tiles[t];
+    const int s = tl.x;
+    double v[RED_NV];
+#pragma unroll
+    for (int k = 0; k < RED_NV; k++)
+      v[k] = 0.;
+    const float *rf = S.ref + 6 * s;
+    for (int q = tl.y + threadIdx.x; q < tl.z; q += RED_THREADS)
+    {
+      const int p = vals[q];
+      const float4 x = Q.posm[p];
+      const float4 u = Q.velu[p];
+      const float m = x.w;
+      const int ty = ptype_of(Q, p);
+      v[0] += Q.energy[p] * m;
+      if (P.thermal)
+        v[1] += u.w * m;
+      double dx[3] = {(double)(x.x - rf[0]), (double)(x.y - rf[1]), (double)(x.z - rf[2])};
+      double dv[3];
+      const double xv[3] = {u.x, u.y, u.z};
+      for (int j = 0; j < 3; j++)
+      {
+        if (P.periodic)
+          dx[j] = nearest_d(dx[j], P.box, P.box_half);
+        dx[j] *= P.a;
+        dv[j] = (double)((float)xv[j] - rf[3 + j]) + P.hz * dx[j];
+        v[2] += dv[j] * dv[j] * m;
+      }
+      v[3] += (dx[1] * dv[2] - dx[2] * dv[1]) * m;
+      v[4] += (dx[2] * dv[0] - dx[0] * dv[2]) * m;
+      v[5] += (dx[0] * dv[1] - dx[1] * dv[0]) * m;
+      v[6] += m;
+      v[7 + ty] += m;
+      if (!P.dm_only)
+      {
+        atomicAdd(type_cnt + 6 * s + ty, 1);
+        if (is_tracer(P, ty))
+          atomicMin(first_tracer + s, q - S.off[s]);
+      }
+    }
+    block_reduce_vec(v, lds);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < RED_NV; k++)
+        partial[(int64_t)t * RED_NV + k] = v[k];
+  }
+}
+
+// Core phase-space selection (GetCorePhaseSpaceProperties, src/subhalo_merge.cpp:94-181):
+// tracer flags of the bound part in final order (scanned for ranks).
+__global__ void k_core_flags(SubArrays S, PartArrays Q, DevParams P, int64_t n, const int32_t *vals, int32_t *flag)
+{
+  GRID_STRIDE(q, n)
+  {
+    const int s = S.sub_of[q];
+    const int k = (int)(q - S.off[s]);
+    int f = 0;
+    if (S.status[s] == ST_CONVERGED && k < S.nbound[s])
+      f = is_tracer(P, ptype_of(Q, vals[q])) ? 1 : 0;
+    flag[q] = f;
+  }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_core_tile(SubArrays S, PartArrays Q, DevParams P, const int4 *tiles,
+                                                          const int32_t *ntiles_p, const int32_t *vals,
+                                                          const int32_t *tscan, double *partial)
+{
+  __shared__ double lds[(RED_THREADS / HBTX_WAVE) * RED_NV];
+  const int nt = *ntiles_p;
+  for (int t = blockIdx.x; t < nt; t += gridDim.x)
+  {
+    const int4 tl = tiles[t];
+    const int s = tl.x;
+    const int b = S.off[s];
+    const int nb = S.nbound[s];
+    const int64_t core = max((int64_t)(nb * P.core_factor), P.core_min);
+    const int ttot = tscan[b + nb] - tscan[b];
+    double v[RED_NV];
+#pragma unroll
+    for (int k = 0; k < RED_NV; k++)
+      v[k] = 0.;
+    const float4 o = Q.posm[vals[b]];
+    for (int q = tl.y + threadIdx.x; q < tl.z; q += RED_THREADS)
+    {
+      const int k = q - b;
+      const int rt = tscan[q] - tscan[b];
+      const bool tr = (tscan[q + 1] - tscan[q]) != 0;
+      const bool sel = tr ? (rt < core) : (ttot < core && (int64_t)(k - rt) < core - ttot);
+      if (!sel)
+        continue;
+      const int p = vals[q];
+      const float4 x = Q.posm[p];
+      const float4 u = Q.velu[p];
+      const double m = x.w;
+      const double xs[3] = {x.x, x.y, x.z}, os[3] = {o.x, o.y, o.z}, vs[3] = {u.x, u.y, u.z};
+      v[0] += m;
+      for (int d = 0; d < 3; d++)
+      {
+        double dx = xs[d];
+        if (P.periodic)
+          dx = nearest_d(dx - os[d], P.box, P.box_half);
+        v[1 + d] += m * dx;
+        v[4 + d] += m * dx * dx;
+        v[7 + d] += m * vs[d];
+        v[10 + d] += m * vs[d] * vs[d];
+      }
+    }
+    block_reduce_vec(v, lds);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < RED_NV; k++)
+        partial[(int64_t)t * RED_NV + k] = v[k];
+  }
+}
+
+__global__ void k_final_ranges(SubArrays S, int32_t *begin, int32_t *len, int core_only, DevParams P)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub)
+    return;
+  begin[s] = S.off[s];
+  len[s] = (S.status[s] == ST_CONVERGED) ? S.nbound[s] : 0;
+}
+
+// ParticlePotentialEnergies (GetPotentialEnergy, :114-135) of the bound part.
+__global__ void k_potential_energy(SubArrays S, PartArrays Q, DevParams P, int64_t n, const int32_t *vals, float *out)
+{
+  GRID_STRIDE(q, n)
+  {
+    const int s = S.sub_of[q];
+    const int k = (int)(q - S.off[s]);
+    float E = 0.f;
+    if (S.status[s] == ST_CONVERGED && k < S.nbound[s])
+    {
+      const int p = vals[q];
+      const float *rf = S.ref + 6 * s;
+      const float4 x = Q.posm[p];
+      const float4 u = Q.velu[p];
+      E = Q.energy[p];
+      if (P.thermal)
+        E -= u.w;
+      const float xs[3] = {x.x, x.y, x.z}, vs[3] = {u.x, u.y, u.z};
+      for (int j = 0; j < 3; j++)
+      {
+        double dx = (double)(xs[j] - rf[j]);
+        if (P.periodic)
+          dx = nearest_d(dx, P.box, P.box_half);
+        dx *= P.a;
+        const double dv = (double)(vs[j] - rf[3 + j]) + P.hz * dx;
+        E = (float)((double)E - 0.5 * dv * dv);
+      }
+    }
+    out[q] = E;
+  }
+}
+
+} // namespace hbtx

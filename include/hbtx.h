@@ -1,0 +1,166 @@
+/* hbtx — MI355X-native gravitational unbinding: the C-ABI drop-in boundary.
+ *
+ * Every entry point takes plain pointers and sizes; the library never aborts and
+ * reports failures as negative status codes with a message in hbtx_last_error().
+ * Host arrays are owned by the caller; a context owns device buffers, its HIP
+ * stream and its cached graphs.  One context per host thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   hbtx_unbind_batch        Subhalo_t::Unbind(const Snapshot_t&)        src/subhalo.h:152, src/subhalo_unbind.cpp:463-808
+ *                            (per subhalo, HBTX_PLAIN) and the batch loop of
+ *                            SubhaloSnapshot_t::RefineParticles           src/subhalo_unbind.cpp:855-919 (default)
+ *   hbtx_tree_build          GravityTree_t::Reserve + OctTree_t::Build     src/oct_tree.tpp:11-156, src/gravity_tree.cpp:50-80
+ *   hbtx_tree_potential      GravityTree_t::EvaluatePotential              src/gravity_tree.cpp:82-166
+ *   hbtx_tree_binding_energy GravityTree_t::BindingEnergy                  src/gravity_tree.cpp:168-178
+ *   hbtx_tree_free           OctTree_t::Clear                              src/oct_tree.tpp:158-164
+ *   hbtx_params              HBTConfig fields used by unbinding            src/config_parser.h:98-172, :200-212, :641-649
+ */
+#ifndef HBTX_H
+#define HBTX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HBTX_OK 0
+#define HBTX_E_INVALID (-1)
+#define HBTX_E_ALLOC (-2)
+#define HBTX_E_HIP (-3)
+#define HBTX_E_CAPACITY (-4)
+#define HBTX_E_NOT_IMPLEMENTED (-5)
+
+/* Flags for hbtx_unbind_batch / hbtx_batch_run */
+#define HBTX_BATCHED 0u          /* all subhaloes advance together, device-side control (default) */
+#define HBTX_PLAIN 1u            /* one subhalo at a time, host-driven iteration (Subhalo_t::Unbind loop) */
+#define HBTX_WARMSTART 2u        /* start from the proposed bound set (hbtx_batch_set_proposal) */
+#define HBTX_SAVE_POTENTIAL 4u   /* fill out_potential (ParticlePotentialEnergies) */
+#define HBTX_NO_GRAPH 8u         /* launch kernels directly instead of replaying the captured round graph */
+
+typedef struct hbtx_ctx hbtx_ctx;
+typedef struct hbtx_tree hbtx_tree;
+
+/* Unbinding parameters: the same names, meaning and defaults as the reference's
+ * HBTConfig (src/config_parser.h:98-172), plus the epoch and two tuning knobs. */
+typedef struct
+{
+  double G;                       /* PhysicalConst::G = 43.0071 (Msun/h 1e10, Mpc/h, km/s) */
+  float scale_factor;             /* Cosmology.ScaleFactor */
+  float hz;                       /* Cosmology.Hz, internal units */
+  float softening;                /* SofteningHalo (comoving) */
+  float max_phys_softening;       /* MaxPhysicalSofteningHalo, -1 = unset */
+  float theta;                    /* TreeNodeOpenAngle */
+  float bound_mass_precision;     /* BoundMassPrecision */
+  int64_t max_sample_size;        /* MaxSampleSizeOfPotentialEstimate, 0 disables subsampling */
+  int32_t refine_most_bound;      /* RefineMostBoundParticle */
+  float bound_fraction_center_refinement; /* BoundFractionCenterRefinement */
+  int64_t min_num_part;           /* MinNumPartOfSub */
+  int64_t min_num_tracer;         /* MinNumTracerPartOfSub */
+  int32_t tracer_mask;            /* TracerParticleBitMask */
+  int32_t donot_subsample_mask;   /* DoNotSubsampleParticleBitMask */
+  int32_t periodic;               /* PeriodicBoundaryOn */
+  float box_size;                 /* BoxSize */
+  int32_t thermal;                /* UNBIND_WITH_THERMAL_ENERGY semantics */
+  int32_t dm_only;                /* DM_ONLY semantics (no types; every particle a tracer) */
+  int32_t upscale_per_type;       /* PotentialEstimateUpscaleMassesPerType */
+  float sub_core_size_factor;     /* SubCoreSizeFactor */
+  int64_t sub_core_size_min;      /* SubCoreSizeMin */
+  int64_t direct_cutoff;          /* sources <= cutoff use the exact O(N^2) sum; 0 = tree everywhere,
+                                     -1 = autotuned value */
+  int32_t reserved[2];
+} hbtx_params;
+
+/* Per-subhalo outputs of Subhalo_t::Unbind (src/subhalo.h:26-216 fields). */
+typedef struct
+{
+  int64_t nbound;
+  float mbound;
+  int64_t nbound_type[6];
+  float mbound_type[6];
+  float com_pos[3];      /* ComovingAveragePosition */
+  float com_vel[3];      /* PhysicalAverageVelocity */
+  float mostbound_pos[3];/* ComovingMostBoundPosition */
+  float mostbound_vel[3];/* PhysicalMostBoundVelocity */
+  float spec_pot;        /* SpecificSelfPotentialEnergy */
+  float spec_kin;        /* SpecificSelfKineticEnergy */
+  float spec_am[3];      /* SpecificAngularMomentum */
+  int64_t tracer_index;  /* TracerIndex */
+  int64_t mostbound_id;  /* MostBoundParticleId (-1 when disrupted) */
+  float core_pos[3], core_vel[3];       /* CoreComovingPosition, CorePhysicalVelocity */
+  float core_sigma_r, core_sigma_v;     /* CoreComovingSigmaR, CorePhysicalSigmaV */
+  int32_t iterations;    /* NumberUnbindingIterations */
+  int32_t disrupted;     /* 1 if Nbound fell below MinNumPartOfSub / MinNumTracerPartOfSub */
+} hbtx_sub_result;
+
+void hbtx_default_params(hbtx_params *p);
+
+int hbtx_create(hbtx_ctx **ctx, int device);
+void hbtx_destroy(hbtx_ctx *ctx);
+const char *hbtx_last_error(const hbtx_ctx *ctx);
+int hbtx_device_sync(hbtx_ctx *ctx);
+
+/* One-call unbinding of nsub subhaloes stored back to back (CSR offsets[nsub+1]).
+ * Inputs are structure-of-arrays host buffers: pos/vel float[n][3], mass float[n],
+ * u float[n] or NULL (specific internal energy), type int32[n] or NULL (all DM),
+ * id int64[n] or NULL, ref_pos/ref_vel float[nsub][3] (the pre-set frame,
+ * src/subhalo_unbind.cpp:464), mb_pos/mb_vel float[nsub][3] or NULL (most-bound
+ * position/velocity used when a subhalo disrupts, :660-661).
+ * With max_sample_size > 0 each subhalo must arrive partitioned and shuffled as
+ * PrepareParticlesForSubsampling leaves it (:430-443).
+ * Outputs: out_order int64[n] (local input index of the particle in each final
+ * slot: bound sorted by energy, then unbound batches, latest first), out_energy
+ * float[n] (energy of the particle in each slot), out_potential float[n] or NULL
+ * (ParticlePotentialEnergies of the bound part, needs HBTX_SAVE_POTENTIAL),
+ * out_res[nsub]. */
+int hbtx_unbind_batch(hbtx_ctx *ctx, const hbtx_params *prm, int64_t n, const float *pos, const float *vel,
+                      const float *mass, const float *u, const int32_t *type, const int64_t *id, int64_t nsub,
+                      const int64_t *offsets, const float *ref_pos, const float *ref_vel, const float *mb_pos,
+                      const float *mb_vel, uint32_t flags, int64_t *out_order, float *out_energy,
+                      float *out_potential, hbtx_sub_result *out_res);
+
+/* Device-resident batch: upload once, run (timed on the context stream with HIP
+ * events, inputs already in HBM), fetch.  Same array meanings as above. */
+int hbtx_batch_load(hbtx_ctx *ctx, int64_t n, const float *pos, const float *vel, const float *mass,
+                    const float *u, const int32_t *type, const int64_t *id, int64_t nsub, const int64_t *offsets,
+                    const float *ref_pos, const float *ref_vel, const float *mb_pos, const float *mb_vel);
+/* Optional warm start: proposed bound flag per particle (uint8[n], 1 = proposed bound). */
+int hbtx_batch_set_proposal(hbtx_ctx *ctx, const uint8_t *proposal);
+int hbtx_batch_run(hbtx_ctx *ctx, const hbtx_params *prm, uint32_t flags, double *elapsed_ms);
+int hbtx_batch_fetch(hbtx_ctx *ctx, int64_t *out_order, float *out_energy, float *out_potential,
+                     hbtx_sub_result *out_res);
+/* Kernel-level counters of the last run: stats[0..15] (see hbtx_stat_names). */
+int hbtx_batch_stats(hbtx_ctx *ctx, double *stats, int nstats);
+const char *hbtx_stat_name(int i);
+
+/* GravityTree_t replacement.  Build over n sources (pos float[n][3], mass float[n]);
+ * *ncells receives the number of tree cells (OctTree_t::Build's return value). */
+int hbtx_tree_build(hbtx_ctx *ctx, const hbtx_params *prm, int64_t n, const float *pos, const float *mass,
+                    hbtx_tree **tree, int64_t *ncells);
+/* Specific physical potential at n_tgt points (targetMass = tgt_mass[i], or 0
+ * when tgt_mass is NULL, removing the self term of a target that is a source). */
+int hbtx_tree_potential(hbtx_ctx *ctx, hbtx_tree *tree, int64_t n_tgt, const float *tgt_pos,
+                        const float *tgt_mass, double *out_pot);
+/* Potential + 0.5 |v - V + H a (x - X)|^2 at n_tgt points, one reference frame. */
+int hbtx_tree_binding_energy(hbtx_ctx *ctx, hbtx_tree *tree, int64_t n_tgt, const float *tgt_pos,
+                             const float *tgt_vel, const float *ref_pos, const float *ref_vel,
+                             const float *tgt_mass, double *out_energy);
+/* Depth-first preorder export of the tree (tests): per entry kind (0 particle,
+ * 1 cell), source index or -1, mass, centre of mass [3], len, skip, depth. */
+int hbtx_tree_export(hbtx_ctx *ctx, hbtx_tree *tree, int64_t cap, int32_t *kind, int64_t *pidx, float *mass,
+                     float *com, float *len, int64_t *skip, int32_t *depth, int64_t *nentries);
+void hbtx_tree_free(hbtx_tree *tree);
+
+/* Exact softened potential by direct summation on the GPU (sources = targets allowed). */
+int hbtx_direct_potential(hbtx_ctx *ctx, const hbtx_params *prm, int64_t n_src, const float *src_pos,
+                          const float *src_mass, int64_t n_tgt, const float *tgt_pos, const float *tgt_mass,
+                          double *out_pot);
+
+/* Segmented sort of uint64 keys with int32 payloads (used by the tree build and
+ * the final binding-energy order; exported for tests). */
+int hbtx_segmented_sort(hbtx_ctx *ctx, int64_t n, uint64_t *keys, int32_t *vals, int64_t nseg,
+                        const int64_t *seg_offsets);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
