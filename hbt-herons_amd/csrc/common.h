@@ -62,6 +62,20 @@ struct DevParams
   int64_t direct_cutoff;
 };
 
+// One depth-first tree entry (32 bytes, one scalar load).  Cells carry the centre
+// of mass and mass; particles their position and (scaled) mass.  crit2 is the
+// squared opening radius len^2 / theta^2 (0 for particles, which are never
+// opened); len2 keeps len^2 (float bits) for export; skip is the first entry
+// after this entry's subtree.
+struct __attribute__((aligned(32))) Entry
+{
+  float x, y, z, m;
+  float crit2;
+  int32_t skip;
+  float len2;
+  int32_t pad;
+};
+
 // Loads through the constant address space: a wave-uniform index then becomes a
 // scalar (s_load) access.  Only for buffers that no kernel writes while it reads them.
 HBTX_DEV float4 const_load(const float4 *p, int i)
@@ -75,6 +89,27 @@ HBTX_DEV int2 const_load(const int2 *p, int i)
   typedef const __attribute__((address_space(4))) int ci;
   const ci *q = (const ci *)(uintptr_t)(p + i);
   return make_int2(q[0], q[1]);
+}
+// walk view of an entry: position+mass, opening radius^2, skip
+struct WEntry
+{
+  float x, y, z, m, crit2;
+  int32_t skip;
+};
+HBTX_DEV WEntry const_load_entry(const Entry *p, int i)
+{
+  typedef const __attribute__((address_space(4))) float cf;
+  typedef const __attribute__((address_space(4))) int ci;
+  const cf *q = (const cf *)(uintptr_t)(p + i);
+  const ci *qi = (const ci *)(uintptr_t)(p + i);
+  WEntry e;
+  e.x = q[0];
+  e.y = q[1];
+  e.z = q[2];
+  e.m = q[3];
+  e.crit2 = q[4];
+  e.skip = qi[5];
+  return e;
 }
 
 HBTX_DEV uint64_t ballot(bool p)
@@ -200,19 +235,18 @@ HBTX_HD uint64_t key_prefix(uint64_t k, int l)
   return (l == 0) ? 0ull : (k >> (3 * (HBTX_KEY_LEVELS - l)));
 }
 
-// Softened pair potential term -m * W(r) / ... in units where the final sum is
-// multiplied by G/a: r >= h gives -m/r; otherwise the GADGET-2 cubic-spline
-// polynomial of src/gravity_tree.cpp:150-162.
-HBTX_DEV float spline_near(float r2, float m, float hinv)
+// Softened pair potential inside the kernel radius h, in units where the final
+// sum is multiplied by G/a: the GADGET-2 cubic-spline polynomial of
+// src/gravity_tree.cpp:150-162 (outside h the term is -m/r).  rinv = 1/sqrt(r2).
+HBTX_DEV float spline_near(float r2, float rinv, float m, float hinv)
 {
-  float r = __fsqrt_rn(r2);
-  float u = r * hinv;
-  float wp;
-  if (u < 0.5f)
-    wp = -2.8f + u * u * (5.333333333333f + u * u * (6.4f * u - 9.6f));
-  else
-    wp = -3.2f + 0.066666666667f / u + u * u * (10.666666666667f + u * (-16.0f + u * (9.6f - 2.133333333333f * u)));
-  return m * hinv * wp;
+  const float r = (r2 > 0.f) ? r2 * rinv : 0.f;
+  const float u = r * hinv;
+  const float u2 = u * u;
+  const float inner = -2.8f + u2 * (5.333333333333f + u2 * (6.4f * u - 9.6f));
+  const float outer = -3.2f + 0.066666666667f * __builtin_amdgcn_rcpf(u) +
+                      u2 * (10.666666666667f + u * (-16.0f + u * (9.6f - 2.133333333333f * u)));
+  return m * hinv * ((u < 0.5f) ? inner : outer);
 }
 
 // Kahan-compensated accumulation of FP32 terms.

@@ -109,14 +109,13 @@ struct Batch
   DArr<float4> posm, velu;
   DArr<int8_t> ptype;
   DArr<int32_t> sub_of, sub_zero, off, off_pairs, order, order_zero;
-  DArr<int32_t> state, cur, cur2, src, fvals, ford;
+  DArr<int32_t> state, cur, cur2, src, src_sub, fvals, ford;
   DArr<float> energy, selfm, fenergy, fpot;
   DArr<uint8_t> proposal;
   DArr<int32_t> flag, fscan, crank, cnt, cscan;
   DArr<int64_t> flag64, fscan64;
   DArr<uint64_t> keys;
-  DArr<float4> ent_pm;
-  DArr<int2> ent_aux;
+  DArr<Entry> ent; // ent_cap + 1 entries (the walk may prefetch one past the end)
   DArr<int8_t> ent_depth;
   int32_t ent_cap = 0;
   DArr<int2> items;
@@ -156,8 +155,7 @@ struct hbtx_tree
   int32_t ne = 0;
   int64_t ncells = 0;
   DevParams P;
-  DArr<float4> ent_pm;
-  DArr<int2> ent_aux;
+  DArr<Entry> ent;
   DArr<int8_t> ent_depth;
   DArr<int32_t> src_of_entry;
   DArr<double> cube;
@@ -182,6 +180,7 @@ struct hbtx_ctx
   DArr<unsigned long long> stats;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, wev0 = nullptr, wev1 = nullptr;
   double walk_ms = 0.0;
+  bool count_work = false;
   double stage_ms[8] = {0};
   int rounds = 0;
   Batch B;
@@ -343,6 +342,7 @@ void batch_reserve(hbtx_ctx *c, Batch &B)
   B.cur.alloc(n);
   B.cur2.alloc(n);
   B.src.alloc(n);
+  B.src_sub.alloc(n);
   B.fvals.alloc(n);
   B.ford.alloc(n);
   B.energy.alloc(n);
@@ -359,9 +359,8 @@ void batch_reserve(hbtx_ctx *c, Batch &B)
   B.fscan64.alloc(n + 1);
   B.keys.alloc(n);
   B.ent_cap = (int32_t)std::min<int64_t>(3 * n + 64, (int64_t)0x7ffffff0);
-  B.ent_pm.alloc(B.ent_cap);
-  B.ent_aux.alloc(B.ent_cap);
-  B.ent_depth.alloc(B.ent_cap);
+  B.ent.alloc((int64_t)B.ent_cap + 1);
+  B.ent_depth.alloc(((int64_t)B.ent_cap + 16) & ~(int64_t)15);
   B.items.alloc(n / HBTX_WAVE + ns + 1);
   B.icnt.alloc(ns + 1);
   B.iscan.alloc(ns + 1);
@@ -454,7 +453,7 @@ TreeArgs tree_args(Batch &B, View &V, const DevParams &P)
   TreeArgs T;
   T.src = B.src.p;
   T.keys = B.keys.p;
-  T.sub_of = V.S.sub_of;
+  T.src_sub = B.src_sub.p;
   T.src_off = V.S.src_off;
   T.nsrc = V.S.nsrc;
   T.treemode = V.S.treemode;
@@ -466,8 +465,7 @@ TreeArgs tree_args(Batch &B, View &V, const DevParams &P)
   T.cnt = B.cnt.p;
   T.cscan = B.cscan.p;
   T.s_total = B.fscan.p + V.n;
-  T.ent_pm = B.ent_pm.p;
-  T.ent_aux = B.ent_aux.p;
+  T.ent = B.ent.p;
   T.ent_depth = B.ent_depth.p;
   T.cap = B.ent_cap;
   T.overflow = B.misc.p + 2;
@@ -497,41 +495,43 @@ void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, 
   }
   k_mark_sources<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, round, crank, B.flag.p);
   scan32(c, B.flag.p, B.fscan.p, n);
-  k_scatter_sources<<<gb, 256, 0, st>>>(V.S, n, B.flag.p, B.fscan.p, B.src.p);
+  k_scatter_sources<<<gb, 256, 0, st>>>(V.S, n, B.flag.p, B.fscan.p, B.src.p, B.src_sub.p);
   k_sub_src<<<gs, 256, 0, st>>>(V.S, B.fscan.p);
   const int32_t *s_total = B.fscan.p + n;
-  k_bbox<<<gb, 256, 0, st>>>(V.S, V.Q.posm, B.src.p, s_total);
+  k_tree_src_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p);
+  const int32_t *nbt = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, n);
+  k_bbox_tiles<<<red_grid(c, n, V.nsub), 256, 0, st>>>(V.S, V.Q.posm, B.src.p, c->red_tiles.p, nbt);
   k_cube<<<gs, 256, 0, st>>>(V.S);
-  k_keys<<<gb, 256, 0, st>>>(V.S, P, V.Q.posm, B.src.p, s_total, B.keys.p);
+  k_keys<<<gb, 256, 0, st>>>(V.S, P, V.Q.posm, B.src.p, B.src_sub.p, s_total, B.keys.p);
   k_sort_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
   seg_sort(c, B.keys.p, B.src.p, V.nsub, B.seg_begin.p, B.seg_len.p, n, 63);
+  k_targets_from_sources<<<gb, 256, 0, st>>>(V.S, V.Q, B.src.p, B.src_sub.p, s_total);
   TreeArgs T = tree_args(B, V, P);
-  k_tree_count<<<blocks_for(n + 1, 256, 1 << 24), 256, 0, st>>>(T, (int32_t)n, B.cnt.p);
+  k_fill_i32<<<1, 64, 0, st>>>(B.misc.p + 4, 1, 0);
+  k_tree_count<<<blocks_for(n + 1, 256, 1 << 24), 256, 0, st>>>(T, (int32_t)n, B.cnt.p, B.misc.p + 4);
   scan32(c, B.cnt.p, B.cscan.p, n);
-  // exact entry count of this round's trees; grow the entry buffers if needed
+  // exact entry count and depth of this round's trees; grow the entry buffers if needed
   k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, 0x7fffffff, B.misc.p + 2);
-  int32_t etot = 0;
-  HIPCHK(hipMemcpyAsync(&etot, B.misc.p + 3, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  int32_t hm[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(hm, B.misc.p + 3, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  const int32_t etot = hm[0], maxdepth = std::min(hm[1], HBTX_KEY_LEVELS);
   REQUIRE(etot >= 0, HBTX_E_CAPACITY, "tree entry count overflows int32");
   if (etot > B.ent_cap)
   {
     const int64_t cap = std::min<int64_t>((int64_t)etot + etot / 4 + 64, (int64_t)0x7ffffff0);
-    B.ent_pm.release();
-    B.ent_aux.release();
+    B.ent.release();
     B.ent_depth.release();
-    B.ent_pm.alloc(cap);
-    B.ent_aux.alloc(cap);
-    B.ent_depth.alloc(cap);
+    B.ent.alloc(cap + 1);
+    B.ent_depth.alloc((cap + 16) & ~(int64_t)15);
     B.ent_cap = (int32_t)cap;
     T = tree_args(B, V, P);
   }
   k_tree_emit<<<blocks_for(n, 256, 1 << 24), 256, 0, st>>>(T);
   k_sub_entries<<<gs, 256, 0, st>>>(V.S, B.cscan.p, B.ent_cap);
   k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, B.ent_cap, B.misc.p + 2);
-  for (int d = HBTX_KEY_LEVELS; d >= 0; d--)
-    k_tree_moments<<<blocks_for(2 * n, 256, 8192), 256, 0, st>>>(B.ent_pm.p, B.ent_aux.p, B.ent_depth.p,
-                                                                 B.misc.p + 3, d);
+  for (int d = maxdepth; d >= 0; d--)
+    k_tree_moments<<<blocks_for(2 * n / 16 + 1, 256, 8192), 256, 0, st>>>(B.ent.p, B.ent_depth.p, B.misc.p + 3, d);
   HIPCHK(hipGetLastError());
 }
 
@@ -548,8 +548,7 @@ WalkArgs walk_args(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   W.treemode = V.S.treemode;
   W.ent_begin = V.S.ent_begin;
   W.ent_end = V.S.ent_end;
-  W.ent_pm = B.ent_pm.p;
-  W.ent_aux = B.ent_aux.p;
+  W.ent = B.ent.p;
   W.posm = V.Q.posm;
   W.velu = V.Q.velu;
   W.selfm = V.Q.selfm;
@@ -568,13 +567,27 @@ void round_walk(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   const int gs = blocks_for(V.nsub, 256, 1 << 20);
   k_items_count<<<gs, 256, 0, st>>>(V.S, V.order, B.icnt.p);
   scan32(c, B.icnt.p, B.iscan.p, V.nsub);
-  k_items_emit<<<gs, 256, 0, st>>>(V.S, V.order, B.iscan.p, B.items.p);
+  k_items_emit<<<blocks_for((int64_t)V.nsub * HBTX_WAVE, 256, 1 << 24), 256, 0, st>>>(V.S, V.order, B.iscan.p,
+                                                                                       B.items.p);
   k_fill_i32<<<1, 64, 0, st>>>(B.misc.p + 1, 1, 0);
   WalkArgs W = walk_args(c, B, V, P);
   const int64_t maxitems = V.n / HBTX_WAVE + V.nsub + 1;
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxitems, 32 * c->num_cu));
   HIPCHK(hipEventRecord(c->wev0, st));
-  k_walk_energy<<<grid, 64, 0, st>>>(W);
+  if (P.periodic)
+  {
+    if (c->count_work)
+      k_walk_energy<true, true><<<grid, 64, 0, st>>>(W);
+    else
+      k_walk_energy<true, false><<<grid, 64, 0, st>>>(W);
+  }
+  else
+  {
+    if (c->count_work)
+      k_walk_energy<false, true><<<grid, 64, 0, st>>>(W);
+    else
+      k_walk_energy<false, false><<<grid, 64, 0, st>>>(W);
+  }
   HIPCHK(hipEventRecord(c->wev1, st));
   HIPCHK(hipGetLastError());
 }
@@ -619,8 +632,8 @@ __global__ void k_final_local(SubArrays S, int64_t n, const int32_t *vals, const
 }
 
 __global__ void k_final_sub(SubArrays S, PartArrays Q, DevParams P, const int32_t *vals, const double *kin,
-                            const double *core, const int32_t *type_cnt, const int32_t *first_tracer,
-                            const float *ref_in, hbtx_sub_result *out)
+                            const double *core, const int32_t *first_tracer, const float *ref_in,
+                            hbtx_sub_result *out)
 {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S.nsub)
@@ -694,7 +707,7 @@ __global__ void k_final_sub(SubArrays S, PartArrays Q, DevParams P, const int32_
   {
     for (int k = 0; k < 6; k++)
     {
-      R.nbound_type[k] = type_cnt[6 * s + k];
+      R.nbound_type[k] = (int64_t)v[13 + k];
       R.mbound_type[k] = (float)v[7 + k];
     }
     const int ft = first_tracer[s];
@@ -734,22 +747,17 @@ void run_final(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot
   k_final_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
   seg_sort(c, B.keys.p, V.fvals, V.nsub, B.seg_begin.p, B.seg_len.p, n, 48);
   k_final_local<<<gb, 256, 0, st>>>(V.S, n, V.fvals, V.Q.energy, V.ford, V.fenergy);
-  // kinematics / types over the bound part
-  k_final_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p, 0, P);
-  k_fill_i32<<<blocks_for(6 * V.nsub), 256, 0, st>>>(V.type_cnt, 6 * (int64_t)V.nsub, 0);
-  k_fill_i32<<<gs, 256, 0, st>>>(V.first_tracer, V.nsub, 0x7fffffff);
+  // kinematics / types over the bound particles (input order)
+  k_final_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p);
   const int32_t *ntiles = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, n);
-  k_final_tile<<<red_grid(c, n, V.nsub), RED_THREADS, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, ntiles, V.fvals,
-                                                                c->red_partial.p, V.type_cnt, V.first_tracer);
+  k_final_tile<<<red_grid(c, n, V.nsub), RED_THREADS, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, ntiles,
+                                                                c->red_partial.p);
   k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
-  // core phase space
-  k_core_flags<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, V.fvals, B.flag.p);
-  scan32(c, B.flag.p, B.fscan.p, n);
-  k_core_tile<<<red_grid(c, n, V.nsub), RED_THREADS, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, ntiles, V.fvals,
-                                                               B.fscan.p, c->red_partial.p);
-  k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red2);
-  k_final_sub<<<gs, 256, 0, st>>>(V.S, V.Q, P, V.fvals, V.red, V.red2, V.type_cnt, V.first_tracer,
-                                  B.ref_in.p + 6 * V.s0, V.dres);
+  const int gw = blocks_for((int64_t)V.nsub * HBTX_WAVE, 256, 1 << 24);
+  k_first_tracer<<<gw, 256, 0, st>>>(V.S, V.Q, P, V.fvals, V.first_tracer);
+  k_core_wave<<<gw, 256, 0, st>>>(V.S, V.Q, P, V.fvals, V.red2);
+  k_final_sub<<<gs, 256, 0, st>>>(V.S, V.Q, P, V.fvals, V.red, V.red2, V.first_tracer, B.ref_in.p + 6 * V.s0,
+                                  V.dres);
   if (want_pot)
     k_potential_energy<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, V.fvals, V.fpot);
   HIPCHK(hipGetLastError());
@@ -893,6 +901,7 @@ void run_batch(hbtx_ctx *c, const hbtx_params &prm, uint32_t flags, double *elap
   c->rounds = 0;
   HIPCHK(hipEventRecord(c->ev0, st));
   const bool want_pot = (flags & HBTX_SAVE_POTENTIAL) != 0;
+  c->count_work = (flags & HBTX_COUNT) != 0;
   if (flags & HBTX_PLAIN)
   {
     for (int32_t s = 0; s < B.nsub; s++)
@@ -984,13 +993,11 @@ void build_tree(hbtx_ctx *c, const hbtx_params &prm, int64_t n, const float *pos
   T->ncells = ne - n;
   T->P = P;
   T->treemode = treemode;
-  T->ent_pm.alloc(ne);
-  T->ent_aux.alloc(ne);
+  T->ent.alloc((int64_t)ne + 1);
   T->ent_depth.alloc(ne);
   T->src_of_entry.alloc(n);
   T->cube.alloc(4);
-  HIPCHK(hipMemcpyAsync(T->ent_pm.p, B.ent_pm.p, ne * sizeof(float4), hipMemcpyDeviceToDevice, st));
-  HIPCHK(hipMemcpyAsync(T->ent_aux.p, B.ent_aux.p, ne * sizeof(int2), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(T->ent.p, B.ent.p, ((int64_t)ne + 1) * sizeof(Entry), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(T->ent_depth.p, B.ent_depth.p, ne, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(T->src_of_entry.p, B.src.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(T->cube.p, B.cube.p, 4 * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -1036,8 +1043,7 @@ void tree_points(hbtx_ctx *c, hbtx_tree *T, int64_t nt, const float *tpos, const
   A.eb = 0;
   A.ee = T->ne;
   A.treemode = T->treemode;
-  A.ent_pm = T->ent_pm.p;
-  A.ent_aux = T->ent_aux.p;
+  A.ent = T->ent.p;
   A.phi = phi.p;
   A.phi_d = phid.p;
   A.counter = counter.p;
@@ -1046,7 +1052,20 @@ void tree_points(hbtx_ctx *c, hbtx_tree *T, int64_t nt, const float *tpos, const
   HIPCHK(hipMemsetAsync(c->stats.p, 0, NSTATS * sizeof(unsigned long long), st));
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nt / HBTX_WAVE + 1, 32 * c->num_cu));
   HIPCHK(hipEventRecord(c->wev0, st));
-  k_points_potential<<<grid, 64, 0, st>>>(A);
+  if (T->P.periodic)
+  {
+    if (c->count_work)
+      k_points_potential<true, true><<<grid, 64, 0, st>>>(A);
+    else
+      k_points_potential<true, false><<<grid, 64, 0, st>>>(A);
+  }
+  else
+  {
+    if (c->count_work)
+      k_points_potential<false, true><<<grid, 64, 0, st>>>(A);
+    else
+      k_points_potential<false, false><<<grid, 64, 0, st>>>(A);
+  }
   HIPCHK(hipEventRecord(c->wev1, st));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, phid.p, nt * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1337,12 +1356,10 @@ int hbtx_tree_export(hbtx_ctx *c, hbtx_tree *T, int64_t cap, int32_t *kind, int6
     if (nentries)
       *nentries = ne;
     REQUIRE(cap >= ne, HBTX_E_CAPACITY, "export capacity too small");
-    std::vector<float4> pm(ne);
-    std::vector<int2> aux(ne);
+    std::vector<Entry> en(ne);
     std::vector<int8_t> dep(ne);
     std::vector<int32_t> src(T->n);
-    HIPCHK(hipMemcpy(pm.data(), T->ent_pm.p, ne * sizeof(float4), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(aux.data(), T->ent_aux.p, ne * sizeof(int2), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(en.data(), T->ent.p, ne * sizeof(Entry), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(dep.data(), T->ent_depth.p, ne, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(src.data(), T->src_of_entry.p, T->n * sizeof(int32_t), hipMemcpyDeviceToHost));
     int64_t pi = 0;
@@ -1356,19 +1373,17 @@ int hbtx_tree_export(hbtx_ctx *c, hbtx_tree *T, int64_t cap, int32_t *kind, int6
       if (part)
         pi++;
       if (mass)
-        mass[q] = pm[q].w;
+        mass[q] = en[q].m;
       if (com)
       {
-        com[3 * q] = pm[q].x;
-        com[3 * q + 1] = pm[q].y;
-        com[3 * q + 2] = pm[q].z;
+        com[3 * q] = en[q].x;
+        com[3 * q + 1] = en[q].y;
+        com[3 * q + 2] = en[q].z;
       }
-      float l2;
-      std::memcpy(&l2, &aux[q].x, sizeof(float));
       if (len)
-        len[q] = part ? 0.f : std::sqrt(l2);
+        len[q] = part ? 0.f : std::sqrt(en[q].len2);
       if (skip)
-        skip[q] = aux[q].y;
+        skip[q] = en[q].skip;
       if (depth)
         depth[q] = dep[q];
     }

@@ -13,8 +13,7 @@ struct PointArgs
   int32_t ntgt;
   int32_t eb, ee;          // entry range
   int32_t treemode;
-  const float4 *__restrict__ ent_pm;
-  const int2 *__restrict__ ent_aux;
+  const Entry *ent;
   float *phi;              // specific potential, scattered to the target's input index
   double *phi_d;           // optional double output
   int32_t *counter;
@@ -22,18 +21,12 @@ struct PointArgs
   DevParams P;
 };
 
+template <bool PERIODIC, bool STATS>
 __global__ __launch_bounds__(64) void k_points_potential(PointArgs A)
 {
   const int lane = lane_id();
-  unsigned long long st[NSTATS];
-#pragma unroll
-  for (int k = 0; k < NSTATS; k++)
-    st[k] = 0;
+  unsigned long long st_steps = 0, st_visits = 0, st_accepts = 0, st_pairs = 0, st_near = 0;
   const int nitems = (A.ntgt + HBTX_WAVE - 1) / HBTX_WAVE;
-  WalkArgs W;
-  W.ent_pm = A.ent_pm;
-  W.ent_aux = A.ent_aux;
-  W.P = A.P;
   for (;;)
   {
     int it = 0;
@@ -50,10 +43,25 @@ __global__ __launch_bounds__(64) void k_points_potential(PointArgs A)
       x = A.tgt[ti];
     Kahan acc;
     acc.init();
-    if (A.treemode)
-      tree_walk(W, A.eb, A.ee, valid, x.x, x.y, x.z, acc, st);
-    else
-      direct_sum(W, A.eb, A.ee, valid, x.x, x.y, x.z, acc, st);
+    WalkCounts wc = {0u, 0u, 0u, 0u};
+    if (A.ee > A.eb)
+    {
+      if (A.treemode)
+        tree_walk<PERIODIC, STATS>(A.ent, A.eb, A.ee, valid, x.x, x.y, x.z, A.P, acc, wc);
+      else
+      {
+        direct_sum<PERIODIC>(A.ent, A.eb, A.ee, x.x, x.y, x.z, A.P, acc, wc);
+        if (STATS)
+          st_pairs += (unsigned long long)__popcll(wave_mask(valid)) * (unsigned long long)(A.ee - A.eb);
+      }
+    }
+    if (STATS)
+    {
+      st_steps += wc.steps;
+      st_visits += wc.visits;
+      st_accepts += wc.accepts;
+      st_near += wc.nnear;
+    }
     if (valid)
     {
       const float self = x.w / A.P.eps;
@@ -63,12 +71,13 @@ __global__ __launch_bounds__(64) void k_points_potential(PointArgs A)
         A.phi_d[ti] = phi;
     }
   }
-  if (A.stats && lane == 0)
+  if (STATS && A.stats && lane == 0)
   {
-#pragma unroll
-    for (int k = 0; k < NSTATS; k++)
-      if (st[k])
-        atomicAdd(A.stats + k, st[k]);
+    atomicAdd(A.stats + STAT_TREE_STEPS, st_steps);
+    atomicAdd(A.stats + STAT_TREE_VISITS, st_visits);
+    atomicAdd(A.stats + STAT_TREE_ACCEPT, st_accepts);
+    atomicAdd(A.stats + STAT_DIRECT_PAIRS, st_pairs);
+    atomicAdd(A.stats + STAT_NEAR, st_near);
   }
 }
 

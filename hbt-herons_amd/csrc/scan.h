@@ -48,17 +48,18 @@ HBTX_DEV T block_excl_scan(T v, T *lds_wave, T *total)
   return wbase + incl - v;
 }
 
+// Per-tile totals with coalesced loads (thread t reads elements t, t+256, ...).
 template <typename T>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const T *__restrict__ in, int64_t n,
                                                              T *__restrict__ partials)
 {
   __shared__ T lds[SCAN_THREADS / HBTX_WAVE];
-  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x;
   T s = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; k++)
   {
-    int64_t i = base + k;
+    const int64_t i = base + (int64_t)k * SCAN_THREADS;
     if (i < n)
       s += in[i];
   }
@@ -101,34 +102,48 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_partials(T *__restrict__ 
   }
 }
 
+// Tile scan: each wave owns 1024 consecutive elements, read in 16 coalesced
+// passes of 64 and scanned in registers; wave totals are combined through LDS.
 template <typename T>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const T *in, T *out, int64_t n,
                                                            const T *__restrict__ partials)
 {
   __shared__ T lds[SCAN_THREADS / HBTX_WAVE];
-  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  const int w = threadIdx.x / HBTX_WAVE;
+  const int lane = lane_id();
+  const int64_t wbase = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)w * (HBTX_WAVE * SCAN_ITEMS);
   T v[SCAN_ITEMS];
-  T s = 0;
+  T run = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; k++)
   {
-    int64_t i = base + k;
-    v[k] = (i < n) ? in[i] : (T)0;
-    s += v[k];
+    const int64_t i = wbase + (int64_t)k * HBTX_WAVE + lane;
+    const T x = (i < n) ? in[i] : (T)0;
+    const T incl = wave_incl_scan(x);
+    v[k] = run + incl - x; // exclusive prefix inside the wave's span
+    run += __shfl(incl, HBTX_WAVE - 1, HBTX_WAVE);
   }
-  T tot;
-  T ex = block_excl_scan(s, lds, &tot) + partials[blockIdx.x];
+  if (lane == 0)
+    lds[w] = run;
+  __syncthreads();
+  T off = partials[blockIdx.x];
+  for (int k = 0; k < w; k++)
+    off += lds[k];
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; k++)
   {
-    int64_t i = base + k;
+    const int64_t i = wbase + (int64_t)k * HBTX_WAVE + lane;
     if (i < n)
-      out[i] = ex;
-    ex += v[k];
+      out[i] = off + v[k];
   }
   // the last block also writes the grand total at out[n]
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1)
-    out[n] = ex;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+  {
+    T tot = partials[blockIdx.x];
+    for (int k = 0; k < SCAN_THREADS / HBTX_WAVE; k++)
+      tot += lds[k];
+    out[n] = tot;
+  }
 }
 
 } // namespace hbtx

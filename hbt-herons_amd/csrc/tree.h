@@ -14,9 +14,10 @@
 // sibling link, src/gravity_tree.cpp:50-80) is the position of the entry group
 // that follows the last source of its key range.
 //
-// Entry records: ent_pm = (centre of mass xyz, mass), ent_aux = (len^2 as float
-// bits, skip), ent_depth = depth (cells) or -1 (particles).  A particle is an
-// entry with len^2 = 0 (never opened), so the walk treats both uniformly.
+// Entry records (struct Entry): centre of mass / position, mass, squared
+// opening radius len^2/theta^2, len^2, skip; ent_depth = depth (cells) or -1
+// (particles).  A particle is an entry with opening radius 0 (never opened), so
+// the walk treats both uniformly.
 #pragma once
 #include "common.h"
 
@@ -27,7 +28,7 @@ struct TreeArgs
 {
   const int32_t *src;        // dense source list (global particle index), sorted per segment
   const uint64_t *keys;      // sorted keys, aligned with src
-  const int32_t *sub_of;     // subhalo of every particle
+  const int32_t *src_sub;    // subhalo of every dense source
   const int32_t *src_off;    // per subhalo: first dense source
   const int32_t *nsrc;       // per subhalo: number of sources
   const int32_t *treemode;   // per subhalo: 1 tree, 0 direct (particle entries only)
@@ -39,8 +40,7 @@ struct TreeArgs
   const int32_t *cnt;        // cells starting at each dense source
   const int32_t *cscan;      // exclusive scan of cnt (size >= S+1)
   const int32_t *s_total;    // number of dense sources
-  float4 *ent_pm;
-  int2 *ent_aux;
+  Entry *ent;
   int8_t *ent_depth;
   int32_t cap;               // entry capacity
   int32_t *overflow;         // set to 1 if entries would exceed cap
@@ -70,28 +70,36 @@ HBTX_DEV void pair_levels(const TreeArgs &A, int j, int b, int e, int &cprev, in
   cprev = (j > b) ? common_levels(A.keys[j - 1], A.keys[j]) : -1;
 }
 
-__global__ void k_tree_count(TreeArgs A, int32_t n_all, int32_t *__restrict__ cnt)
+// (also records the deepest cell level in *maxdepth)
+__global__ void k_tree_count(TreeArgs A, int32_t n_all, int32_t *__restrict__ cnt, int32_t *maxdepth)
 {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j > n_all)
-    return;
   const int S = *A.s_total;
-  if (j >= S)
+  int depth = 0;
+  if (j <= n_all)
   {
-    cnt[j] = 0;
-    return;
+    int c = 0;
+    if (j < S)
+    {
+      const int s = A.src_sub[j];
+      if (A.treemode[s])
+      {
+        const int b = A.src_off[s], e = b + A.nsrc[s];
+        int cprev, cj;
+        pair_levels(A, j, b, e, cprev, cj);
+        c = max(0, cj - cprev);
+        if (c > 0)
+          depth = cj;
+      }
+    }
+    cnt[j] = c;
   }
-  const int p = A.src[j];
-  const int s = A.sub_of[p];
-  if (!A.treemode[s])
-  {
-    cnt[j] = 0;
-    return;
-  }
-  const int b = A.src_off[s], e = b + A.nsrc[s];
-  int cprev, cj;
-  pair_levels(A, j, b, e, cprev, cj);
-  cnt[j] = max(0, cj - cprev);
+  // deepest cell level: wave maximum, one atomic per wave
+#pragma unroll
+  for (int d = 1; d < HBTX_WAVE; d <<= 1)
+    depth = max(depth, __shfl_xor(depth, d, HBTX_WAVE));
+  if (lane_id() == 0 && depth > 0)
+    atomicMax(maxdepth, depth);
 }
 
 __global__ void k_tree_emit(TreeArgs A)
@@ -101,7 +109,7 @@ __global__ void k_tree_emit(TreeArgs A)
   if (j >= S)
     return;
   const int p = A.src[j];
-  const int s = A.sub_of[p];
+  const int s = A.src_sub[j];
   const int P = j + A.cscan[j];
   const float4 x = A.posm[p];
   const float m = source_mass(A, p, s);
@@ -109,8 +117,7 @@ __global__ void k_tree_emit(TreeArgs A)
   {
     if (P < A.cap)
     {
-      A.ent_pm[P] = make_float4(x.x, x.y, x.z, m);
-      A.ent_aux[P] = make_int2(0, P + 1);
+      A.ent[P] = Entry{x.x, x.y, x.z, m, 0.f, P + 1, 0.f, 0};
       A.ent_depth[P] = -1;
     }
     else
@@ -127,8 +134,7 @@ __global__ void k_tree_emit(TreeArgs A)
     *A.overflow = 1;
     return;
   }
-  A.ent_pm[q] = make_float4(x.x, x.y, x.z, m);
-  A.ent_aux[q] = make_int2(0, q + 1);
+  A.ent[q] = Entry{x.x, x.y, x.z, m, 0.f, q + 1, 0.f, 0};
   A.ent_depth[q] = -1;
   if (nn == 0)
     return;
@@ -160,9 +166,10 @@ __global__ void k_tree_emit(TreeArgs A)
     const int skip = (bend + 1) + A.cscan[bend + 1];
     const float lf = (float)L;
     const float len2 = lf * lf;
-    A.ent_aux[pos] = make_int2(__float_as_int(len2), skip);
+    // open if len^2 > r^2 theta^2 (src/gravity_tree.cpp:139)  <=>  r^2 < len^2 / theta^2
+    const float crit2 = (A.P.theta2 > 0.f) ? len2 / A.P.theta2 : __int_as_float(0x7f800000);
+    A.ent[pos] = Entry{0.f, 0.f, 0.f, 0.f, crit2, skip, len2, 0};
     A.ent_depth[pos] = (int8_t)l;
-    A.ent_pm[pos] = make_float4(0.f, 0.f, 0.f, 0.f);
     L = (L >= A.P.res) ? L * 0.5 : L;
   }
 }
@@ -170,29 +177,47 @@ __global__ void k_tree_emit(TreeArgs A)
 // Mass and centre of mass of every cell at one depth from its children, in
 // child (sub-index) order with double accumulation of the float child values,
 // as UpdateInternalNodes/ProcessNode do (src/gravity_tree.cpp:25-80).
-__global__ void k_tree_moments(float4 *__restrict__ ent_pm, const int2 *__restrict__ ent_aux,
-                               const int8_t *__restrict__ ent_depth, const int32_t *__restrict__ e_total_p,
-                               int depth)
+__global__ void k_tree_moments(Entry *__restrict__ ent, const int8_t *__restrict__ ent_depth,
+                               const int32_t *__restrict__ e_total_p, int depth)
 {
   const int E = *e_total_p;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < E; q += gridDim.x * blockDim.x)
+  // 16 depth bytes per thread and load; ent_depth is allocated in multiples of 16
+  for (int q0 = (blockIdx.x * blockDim.x + threadIdx.x) * 16; q0 < E; q0 += gridDim.x * blockDim.x * 16)
   {
-    if (ent_depth[q] != depth)
+    const int4 dv = *(const int4 *)(ent_depth + q0);
+    const int dw[4] = {dv.x, dv.y, dv.z, dv.w};
+    const uint32_t pat = 0x01010101u * (uint32_t)(uint8_t)depth;
+    bool any = false;
+    for (int k = 0; k < 4; k++)
+    {
+      const uint32_t x = (uint32_t)dw[k] ^ pat; // a zero byte marks a cell of this depth
+      any |= ((x - 0x01010101u) & ~x & 0x80808080u) != 0u;
+    }
+    if (!any)
       continue;
-    const int end = min(ent_aux[q].y, E);
+    for (int k = 0; k < 16; k++)
+    {
+    const int q = q0 + k;
+    if (q >= E || ent_depth[q] != depth)
+      continue;
+    const int end = min(ent[q].skip, E);
     double m = 0., cx = 0., cy = 0., cz = 0.;
     int c = q + 1;
     while (c < end)
     {
-      const float4 e = ent_pm[c];
-      const double cm = e.w;
+      const Entry e = ent[c];
+      const double cm = e.m;
       m += cm;
       cx += (double)e.x * cm;
       cy += (double)e.y * cm;
       cz += (double)e.z * cm;
-      c = max(ent_aux[c].y, c + 1);
+      c = max(e.skip, c + 1);
     }
-    ent_pm[q] = make_float4((float)(cx / m), (float)(cy / m), (float)(cz / m), (float)m);
+    ent[q].x = (float)(cx / m);
+    ent[q].y = (float)(cy / m);
+    ent[q].z = (float)(cz / m);
+    ent[q].m = (float)m;
+    }
   }
 }
 

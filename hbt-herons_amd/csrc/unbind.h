@@ -174,12 +174,16 @@ __global__ void k_mark_sources(SubArrays S, PartArrays Q, DevParams P, int64_t n
   }
 }
 
-__global__ void k_scatter_sources(SubArrays S, int64_t n, const int32_t *flag, const int32_t *fscan, int32_t *src)
+__global__ void k_scatter_sources(SubArrays S, int64_t n, const int32_t *flag, const int32_t *fscan, int32_t *src,
+                                  int32_t *src_sub)
 {
   GRID_STRIDE(p, n)
   {
     if (flag[p])
+    {
       src[fscan[p]] = (int32_t)p;
+      src_sub[fscan[p]] = S.sub_of[p];
+    }
   }
 }
 
@@ -251,6 +255,72 @@ __global__ void k_bbox(SubArrays S, const float4 *posm, const int32_t *src, cons
   }
 }
 
+// Bounding box per tile of a dense source range (tiles never span subhaloes):
+// block min/max reduction, then six atomics per tile (min/max are order-free).
+__global__ __launch_bounds__(256) void k_bbox_tiles(SubArrays S, const float4 *posm, const int32_t *src,
+                                                    const int4 *tiles, const int32_t *ntiles_p)
+{
+  __shared__ float lds[6][256 / HBTX_WAVE];
+  const int nt = *ntiles_p;
+  for (int t = blockIdx.x; t < nt; t += gridDim.x)
+  {
+    const int4 tl = tiles[t];
+    const int s = tl.x;
+    float mn[3] = {3.0e38f, 3.0e38f, 3.0e38f}, mx[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (int j = tl.y + threadIdx.x; j < tl.z; j += blockDim.x)
+    {
+      const float4 x = posm[src[j]];
+      mn[0] = fminf(mn[0], x.x);
+      mn[1] = fminf(mn[1], x.y);
+      mn[2] = fminf(mn[2], x.z);
+      mx[0] = fmaxf(mx[0], x.x);
+      mx[1] = fmaxf(mx[1], x.y);
+      mx[2] = fmaxf(mx[2], x.z);
+    }
+#pragma unroll
+    for (int d = 1; d < HBTX_WAVE; d <<= 1)
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+      {
+        mn[k] = fminf(mn[k], __shfl_xor(mn[k], d, HBTX_WAVE));
+        mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], d, HBTX_WAVE));
+      }
+    const int w = threadIdx.x / HBTX_WAVE;
+    if (lane_id() == 0)
+      for (int k = 0; k < 3; k++)
+      {
+        lds[k][w] = mn[k];
+        lds[3 + k][w] = mx[k];
+      }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+      for (int ww = 1; ww < (int)blockDim.x / HBTX_WAVE; ww++)
+        for (int k = 0; k < 3; k++)
+        {
+          lds[k][0] = fminf(lds[k][0], lds[k][ww]);
+          lds[3 + k][0] = fmaxf(lds[3 + k][0], lds[3 + k][ww]);
+        }
+      for (int k = 0; k < 3; k++)
+      {
+        atomicMin(S.bbox + 6 * s + k, float_to_ordered(lds[k][0]));
+        atomicMax(S.bbox + 6 * s + 3 + k, float_to_ordered(lds[3 + k][0]));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Dense source ranges of tree-mode subhaloes (for the bounding-box tiles).
+__global__ void k_tree_src_ranges(SubArrays S, int32_t *begin, int32_t *len)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub)
+    return;
+  begin[s] = S.src_off[s];
+  len[s] = (S.mode[s] != MODE_IDLE && S.treemode[s]) ? S.nsrc[s] : 0;
+}
+
 // Root cube: Len = max extent, Center = bbox midpoint (src/oct_tree.tpp:357-365).
 __global__ void k_cube(SubArrays S)
 {
@@ -273,23 +343,23 @@ __global__ void k_cube(SubArrays S)
   S.len0[s] = Len;
 }
 
-__global__ void k_keys(SubArrays S, DevParams P, const float4 *posm, const int32_t *src, const int32_t *s_total,
-                       uint64_t *keys)
+__global__ void k_keys(SubArrays S, DevParams P, const float4 *posm, const int32_t *src, const int32_t *src_sub,
+                       const int32_t *s_total, uint64_t *keys)
 {
   const int ST = *s_total;
   GRID_STRIDE(j, ST)
   {
-    const int p = src[j];
-    const int s = S.sub_of[p];
-    if (S.treemode[s])
+    const int s = src_sub[j];
+    if (!S.treemode[s])
     {
-      const float4 x = posm[p];
-      const double *c = S.cube + 4 * s;
-      const uint64_t k = octree_key(x.x, x.y, x.z, c[0], c[1], c[2], c[3]);
-      keys[j] = hash_digits(k, resolution_depth(c[3], 0.5 * P.res), (uint32_t)(p - S.off[s]));
-    }
-    else
       keys[j] = 0ull;
+      continue;
+    }
+    const int p = src[j];
+    const float4 x = posm[p];
+    const double *c = S.cube + 4 * s;
+    const uint64_t k = octree_key(x.x, x.y, x.z, c[0], c[1], c[2], c[3]);
+    keys[j] = hash_digits(k, resolution_depth(c[3], 0.5 * P.res), (uint32_t)(p - S.off[s]));
   }
 }
 
@@ -301,6 +371,21 @@ __global__ void k_sort_segments(SubArrays S, int32_t *seg_begin, int32_t *seg_le
     return;
   seg_begin[s] = S.src_off[s];
   seg_len[s] = (S.mode[s] != MODE_IDLE && S.treemode[s]) ? S.nsrc[s] : 0;
+}
+
+// Walk order of a full round: when every target is a source, the targets take
+// the key order of the freshly sorted sources (spatially coherent waves); the
+// compactions of later rounds keep that order.
+__global__ void k_targets_from_sources(SubArrays S, PartArrays Q, const int32_t *src, const int32_t *src_sub,
+                                       const int32_t *s_total)
+{
+  const int ST = *s_total;
+  GRID_STRIDE(j, ST)
+  {
+    const int s = src_sub[j];
+    if (S.mode[s] == MODE_FULL && S.treemode[s] && S.nsrc[s] == S.ntgt[s])
+      Q.cur[S.off[s] + (int)(j - S.src_off[s])] = src[j];
+  }
 }
 
 __global__ void k_sub_entries(SubArrays S, const int32_t *cscan, int32_t cap)
@@ -335,15 +420,16 @@ __global__ void k_items_count(SubArrays S, const int32_t *order, int32_t *cnt)
     cnt[i] = S.nitems[order[i]];
 }
 
+// one wave per subhalo writes its items
 __global__ void k_items_emit(SubArrays S, const int32_t *order, const int32_t *iscan, int2 *items)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / HBTX_WAVE);
   if (i >= S.nsub)
     return;
   const int s = order[i];
   const int n = S.nitems[s];
   const int b = iscan[i];
-  for (int k = 0; k < n; k++)
+  for (int k = lane_id(); k < n; k += HBTX_WAVE)
     items[b + k] = make_int2(s, k * HBTX_WAVE);
 }
 
@@ -442,7 +528,7 @@ __global__ void k_round_decide(SubArrays S, DevParams P, const int64_t *fscan)
 // ---------------------------------------------------------------------------
 #define RED_THREADS 256
 #define RED_TILE 4096
-#define RED_NV 16
+#define RED_NV 20
 
 HBTX_DEV void block_reduce_vec(double v[RED_NV], double *lds /* [RED_THREADS/64][RED_NV] */)
 {
@@ -491,13 +577,16 @@ __global__ void k_red_tiles(const int32_t *begin, const int32_t *len, const int3
     tiles[t] = make_int4(s, x, min(e, x + RED_TILE), 0);
 }
 
-// Centre-of-mass frame of the bound candidates (CentreOfMassReferenceFrame, :151-198)
+// Centre-of-mass frame of the bound candidates (CentreOfMassReferenceFrame, :151-198),
+// summed in input order over particles still flagged as candidates:
 // v[0]=M, v[1..3]=sum m x (float products, or double NEAREST offsets when periodic), v[4..6]=sum m v
 struct ComOp
 {
   HBTX_DEV static void acc(const SubArrays &S, const PartArrays &Q, const DevParams &P, int s, int q, double v[RED_NV])
   {
-    const int p = Q.cur[q];
+    const int p = q;
+    if (Q.state[p] != 0)
+      return;
     const float4 x = Q.posm[p];
     const float4 u = Q.velu[p];
     const float m = x.w;
@@ -567,7 +656,7 @@ __global__ void k_com_ranges(SubArrays S, int32_t *begin, int32_t *len)
   if (s >= S.nsub)
     return;
   begin[s] = S.off[s];
-  len[s] = (S.mode[s] != MODE_IDLE && S.status[s] == ST_ACTIVE) ? S.nbound[s] : 0;
+  len[s] = (S.mode[s] != MODE_IDLE && S.status[s] == ST_ACTIVE) ? (S.off[s + 1] - S.off[s]) : 0;
 }
 
 // smallest input index among bound candidates (the E list's first particle)
@@ -657,18 +746,12 @@ __global__ void k_final_segments(SubArrays S, int32_t *seg_begin, int32_t *seg_l
   seg_len[s] = (S.status[s] == ST_CONVERGED) ? (S.off[s + 1] - S.off[s]) : 0;
 }
 
-// After the final sort: vals[q] = particle in final slot q.
-// AverageKinematics (:200-249) sums over the bound part in final order:
-// v[0]=sum E m, v[1]=sum u m, v[2]=sum K m (2x), v[3..5]=L m, v[6]=M,
-// v[7..12]=per-type mass, v[13] unused.
-struct FinalOp
-{
-  const int32_t *vals;
-};
+// AverageKinematics (:200-249) and CountParticleTypes sums over the bound particles
+// of converged subhaloes (input order): v[0]=sum E m, v[1]=sum u m, v[2]=sum K m (2x),
+// v[3..5]=L m, v[6]=M, v[7..12]=per-type mass, v[13..18]=per-type count.
 
 __global__ __launch_bounds__(RED_THREADS) void k_final_tile(SubArrays S, PartArrays Q, DevParams P, const int4 *tiles,
-                                                           const int32_t *ntiles_p, const int32_t *vals,
-                                                           double *partial, int32_t *type_cnt, int32_t *first_tracer)
+                                                           const int32_t *ntiles_p, double *partial)
 {
   __shared__ double lds[(RED_THREADS / HBTX_WAVE) * RED_NV];
   const int nt = *ntiles_p;
@@ -681,9 +764,10 @@ __global__ __launch_bounds__(RED_THREADS) void k_final_tile(SubArrays S, PartArr
     for (int k = 0; k < RED_NV; k++)
       v[k] = 0.;
     const float *rf = S.ref + 6 * s;
-    for (int q = tl.y + threadIdx.x; q < tl.z; q += RED_THREADS)
+    for (int p = tl.y + threadIdx.x; p < tl.z; p += RED_THREADS)
     {
-      const int p = vals[q];
+      if (Q.state[p] != 0)
+        continue;
       const float4 x = Q.posm[p];
       const float4 u = Q.velu[p];
       const float m = x.w;
@@ -706,78 +790,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_final_tile(SubArrays S, PartArr
       v[4] += (dx[2] * dv[0] - dx[0] * dv[2]) * m;
       v[5] += (dx[0] * dv[1] - dx[1] * dv[0]) * m;
       v[6] += m;
-      v[7 + ty] += m;
-      if (!P.dm_only)
-      {
-        atomicAdd(type_cnt + 6 * s + ty, 1);
-        if (is_tracer(P, ty))
-          atomicMin(first_tracer + s, q - S.off[s]);
-      }
-    }
-    block_reduce_vec(v, lds);
-    if (threadIdx.x == 0)
-      for (int k = 0; k < RED_NV; k++)
-        partial[(int64_t)t * RED_NV + k] = v[k];
-  }
-}
-
-// Core phase-space selection (GetCorePhaseSpaceProperties, src/subhalo_merge.cpp:94-181):
-// tracer flags of the bound part in final order (scanned for ranks).
-__global__ void k_core_flags(SubArrays S, PartArrays Q, DevParams P, int64_t n, const int32_t *vals, int32_t *flag)
-{
-  GRID_STRIDE(q, n)
-  {
-    const int s = S.sub_of[q];
-    const int k = (int)(q - S.off[s]);
-    int f = 0;
-    if (S.status[s] == ST_CONVERGED && k < S.nbound[s])
-      f = is_tracer(P, ptype_of(Q, vals[q])) ? 1 : 0;
-    flag[q] = f;
-  }
-}
-
-__global__ __launch_bounds__(RED_THREADS) void k_core_tile(SubArrays S, PartArrays Q, DevParams P, const int4 *tiles,
-                                                          const int32_t *ntiles_p, const int32_t *vals,
-                                                          const int32_t *tscan, double *partial)
-{
-  __shared__ double lds[(RED_THREADS / HBTX_WAVE) * RED_NV];
-  const int nt = *ntiles_p;
-  for (int t = blockIdx.x; t < nt; t += gridDim.x)
-  {
-    const int4 tl = tiles[t];
-    const int s = tl.x;
-    const int b = S.off[s];
-    const int nb = S.nbound[s];
-    const int64_t core = max((int64_t)(nb * P.core_factor), P.core_min);
-    const int ttot = tscan[b + nb] - tscan[b];
-    double v[RED_NV];
 #pragma unroll
-    for (int k = 0; k < RED_NV; k++)
-      v[k] = 0.;
-    const float4 o = Q.posm[vals[b]];
-    for (int q = tl.y + threadIdx.x; q < tl.z; q += RED_THREADS)
-    {
-      const int k = q - b;
-      const int rt = tscan[q] - tscan[b];
-      const bool tr = (tscan[q + 1] - tscan[q]) != 0;
-      const bool sel = tr ? (rt < core) : (ttot < core && (int64_t)(k - rt) < core - ttot);
-      if (!sel)
-        continue;
-      const int p = vals[q];
-      const float4 x = Q.posm[p];
-      const float4 u = Q.velu[p];
-      const double m = x.w;
-      const double xs[3] = {x.x, x.y, x.z}, os[3] = {o.x, o.y, o.z}, vs[3] = {u.x, u.y, u.z};
-      v[0] += m;
-      for (int d = 0; d < 3; d++)
+      for (int k = 0; k < 6; k++)
       {
-        double dx = xs[d];
-        if (P.periodic)
-          dx = nearest_d(dx - os[d], P.box, P.box_half);
-        v[1 + d] += m * dx;
-        v[4 + d] += m * dx * dx;
-        v[7 + d] += m * vs[d];
-        v[10 + d] += m * vs[d] * vs[d];
+        v[7 + k] += (ty == k) ? (double)m : 0.;
+        v[13 + k] += (ty == k) ? 1.0 : 0.;
       }
     }
     block_reduce_vec(v, lds);
@@ -787,13 +804,98 @@ __global__ __launch_bounds__(RED_THREADS) void k_core_tile(SubArrays S, PartArra
   }
 }
 
-__global__ void k_final_ranges(SubArrays S, int32_t *begin, int32_t *len, int core_only, DevParams P)
+// TracerIndex: first tracer in final order (CountParticleTypes, src/subhalo.cpp:311-392);
+// one wave per converged subhalo.
+__global__ void k_first_tracer(SubArrays S, PartArrays Q, DevParams P, const int32_t *vals, int32_t *first_tracer)
+{
+  const int s = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / HBTX_WAVE);
+  if (s >= S.nsub)
+    return;
+  int ft = 0x7fffffff;
+  if (S.status[s] == ST_CONVERGED)
+  {
+    const int b = S.off[s], nb = S.nbound[s];
+    for (int base = 0; base < nb; base += HBTX_WAVE)
+    {
+      const int k = base + lane_id();
+      const bool tr = k < nb && is_tracer(P, ptype_of(Q, vals[b + k]));
+      const uint64_t m = ballot(tr);
+      if (m)
+      {
+        ft = base + __ffsll((long long)m) - 1;
+        break;
+      }
+    }
+  }
+  if (lane_id() == 0)
+    first_tracer[s] = ft;
+}
+
+// Core phase space (GetCorePhaseSpaceProperties, src/subhalo_merge.cpp:94-181):
+// the first CoreSize tracers in final order, then non-tracers; one wave per subhalo
+// walks the final order in chunks of 64.  out[13*s..]: m, m x[3], m x^2[3], m v[3], m v^2[3].
+__global__ void k_core_wave(SubArrays S, PartArrays Q, DevParams P, const int32_t *vals, double *out)
+{
+  const int s = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / HBTX_WAVE);
+  if (s >= S.nsub)
+    return;
+  double v[13];
+  for (int k = 0; k < 13; k++)
+    v[k] = 0.;
+  if (S.status[s] == ST_CONVERGED && S.nbound[s] > 0)
+  {
+    const int b = S.off[s], nb = S.nbound[s];
+    const int64_t core = max((int64_t)(nb * P.core_factor), P.core_min);
+    const float4 o = Q.posm[vals[b]];
+    const double os[3] = {o.x, o.y, o.z};
+    int64_t num = 0;
+    for (int pass = 0; pass < 2 && num < core; pass++)
+      for (int base = 0; base < nb && num < core; base += HBTX_WAVE)
+      {
+        const int k = base + lane_id();
+        const int p = (k < nb) ? vals[b + k] : 0;
+        const bool tr = (k < nb) && is_tracer(P, ptype_of(Q, p));
+        const bool want = (k < nb) && (pass == 0 ? tr : !tr);
+        const uint64_t m = ballot(want);
+        const int64_t rank = __popcll(m & lanemask_lt());
+        if (want && num + rank < core)
+        {
+          const float4 x = Q.posm[p];
+          const float4 u = Q.velu[p];
+          const double ms = x.w;
+          const double xs[3] = {x.x, x.y, x.z}, vs[3] = {u.x, u.y, u.z};
+          v[0] += ms;
+          for (int d = 0; d < 3; d++)
+          {
+            double dx = xs[d];
+            if (P.periodic)
+              dx = nearest_d(dx - os[d], P.box, P.box_half);
+            v[1 + d] += ms * dx;
+            v[4 + d] += ms * dx * dx;
+            v[7 + d] += ms * vs[d];
+            v[10 + d] += ms * vs[d] * vs[d];
+          }
+        }
+        num += min((int64_t)__popcll(m), core - num);
+      }
+  }
+  for (int k = 0; k < 13; k++)
+  {
+    double x = v[k];
+    for (int d = 32; d >= 1; d >>= 1)
+      x += __shfl_down(x, d, HBTX_WAVE);
+    if (lane_id() == 0)
+      out[(int64_t)RED_NV * s + k] = x;
+  }
+}
+
+__global__ void k_final_ranges(SubArrays S, int32_t *begin, int32_t *len)
 {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S.nsub)
     return;
   begin[s] = S.off[s];
-  len[s] = (S.status[s] == ST_CONVERGED) ? S.nbound[s] : 0;
+  len[s] = (S.status[s] == ST_CONVERGED) ? (S.off[s + 1] - S.off[s]) : 0;
 }
 
 // ParticlePotentialEnergies (GetPotentialEnergy, :114-135) of the bound part.

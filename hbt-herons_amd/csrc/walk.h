@@ -4,12 +4,13 @@
 // wave visits entries in preorder at a wave-uniform index (scalar loads).  Each
 // lane applies the reference's own opening test to its own target
 // (open if len^2 > r^2 theta^2 with r to the cell's centre of mass,
-// src/gravity_tree.cpp:139) and keeps a private "skip" bound: after accepting
-// an entry it ignores that entry's subtree.  The wave descends (index+1) if
-// any active lane opened the entry, otherwise it jumps to the entry's skip.
-// Every lane therefore sees exactly the entries its own stackless walk would
-// see, in the same order, while lanes on spatially adjacent targets share the
-// traversal.
+// src/gravity_tree.cpp:139, stored as r^2 < crit2 = len^2/theta^2) and keeps a
+// private "skip" bound: after accepting an entry it ignores that entry's
+// subtree.  The wave descends (index+1) if any active lane opened the entry,
+// otherwise it jumps to the entry's skip.  Every lane therefore sees exactly the
+// entries its own stackless walk would see, in the same order, while lanes on
+// spatially adjacent targets share the traversal.  Both possible next entries
+// are loaded while the current one is evaluated.
 //
 // Direct mode: the entry list holds particles only and every pair is summed.
 //
@@ -30,7 +31,7 @@ enum
   STAT_DIRECT_PAIRS = 3,  // lane-pair evaluations in direct mode
   STAT_TARGETS = 4,       // targets processed
   STAT_ITEMS = 5,         // work items processed
-  STAT_NEAR = 6,          // accepted interactions inside the softening kernel
+  STAT_NEAR = 6,          // wave steps that evaluated the softening kernel
   NSTATS = 16
 };
 
@@ -46,8 +47,7 @@ struct WalkArgs
   const int32_t *treemode;
   const int32_t *ent_begin;
   const int32_t *ent_end;
-  const float4 *__restrict__ ent_pm;
-  const int2 *__restrict__ ent_aux;
+  const Entry *ent;         // entry array, one spare entry past the last used one
   const float4 *posm;
   const float4 *velu;
   const float *selfm;       // target self mass (sources of a full iteration), else 0
@@ -59,96 +59,115 @@ struct WalkArgs
   DevParams P;
 };
 
-// Far term -m/r and the spline kernel inside h, for the accepting lanes.
-HBTX_DEV float pair_term(float r2, float m, bool acc, const DevParams &P, unsigned long long &nnear)
+HBTX_DEV uint64_t wave_mask(bool p)
 {
-  float term = -m * __builtin_amdgcn_rsqf(r2);
-  const bool nr = acc && (r2 < P.h2);
-  const uint64_t bn = ballot(nr);
-  if (bn)
-  {
-    nnear += __popcll(bn);
-    const float tn = spline_near(r2, m, P.hinv);
-    term = nr ? tn : term;
-  }
-  return acc ? term : 0.f;
+  return __builtin_amdgcn_ballot_w64(p);
 }
 
-HBTX_DEV void tree_walk(const WalkArgs &A, int eb, int ee, bool valid, float px, float py, float pz, Kahan &acc,
-                        unsigned long long st[NSTATS])
+template <bool PERIODIC>
+HBTX_DEV float sep2(float ex, float ey, float ez, float px, float py, float pz, const DevParams &P)
 {
-  const DevParams &P = A.P;
+  float dx = ex - px, dy = ey - py, dz = ez - pz;
+  if (PERIODIC)
+  {
+    dx = nearest_f(dx, P.box, P.box_half);
+    dy = nearest_f(dy, P.box, P.box_half);
+    dz = nearest_f(dz, P.box, P.box_half);
+  }
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Per-wave work counters of one walk (uniform values, kept only in the stats
+// instantiation; the walk is deterministic, so a stats run counts exactly the
+// work of a timed run).
+struct WalkCounts
+{
+  uint32_t steps, visits, accepts, nnear;
+};
+
+// Walk the entries [eb, ee) for this lane's target, accumulating the pair terms
+// (-m/r outside h, the spline inside): plain FP32 sums over blocks of 16 steps,
+// Kahan-compensated across blocks.
+template <bool PERIODIC, bool STATS>
+HBTX_DEV void tree_walk(const Entry *ent, int eb, int ee, bool valid, float px, float py, float pz,
+                        const DevParams &P, Kahan &acc, WalkCounts &wc)
+{
   int i = eb;
   int myskip = valid ? eb : 0x7fffffff;
-  unsigned long long steps = 0, visits = 0, accepts = 0, nnear = 0;
   while (i < ee)
   {
-    const float4 e = const_load(A.ent_pm, i);
-    const int2 a = const_load(A.ent_aux, i);
-    const bool active = i >= myskip;
-    float dx = e.x - px, dy = e.y - py, dz = e.z - pz;
-    if (P.periodic)
+    float blk = 0.f;
+    for (int k = 0; k < 16; k++)
     {
-      dx = nearest_f(dx, P.box, P.box_half);
-      dy = nearest_f(dy, P.box, P.box_half);
-      dz = nearest_f(dz, P.box, P.box_half);
+      const WEntry e = const_load_entry(ent, i);
+      const bool active = i >= myskip;
+      const float r2 = sep2<PERIODIC>(e.x, e.y, e.z, px, py, pz, P);
+      const bool open = r2 < e.crit2;
+      const bool accept = active && !open;
+      const float rinv = __builtin_amdgcn_rsqf(r2);
+      float term = -e.m * rinv;
+      if (wave_mask(accept && r2 < P.h2))
+      {
+        if (STATS)
+          wc.nnear++;
+        const float tn = spline_near(r2, rinv, e.m, P.hinv);
+        term = (r2 < P.h2) ? tn : term;
+      }
+      blk += accept ? term : 0.f;
+      myskip = accept ? e.skip : myskip;
+      const bool anyopen = wave_mask(active && open) != 0ull;
+      if (STATS)
+      {
+        wc.steps++;
+        wc.visits += __popcll(wave_mask(active));
+        wc.accepts += __popcll(wave_mask(accept));
+      }
+      // entries of a subtree follow it, so skip > i; max() keeps the walk finite regardless
+      i = uniform(anyopen ? i + 1 : max(e.skip, i + 1));
+      if (i >= ee)
+        break;
     }
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    const bool open = __int_as_float(a.x) > r2 * P.theta2;
-    const bool accept = active && !open;
-    acc.add(pair_term(r2, e.w, accept, P, nnear));
-    myskip = accept ? a.y : myskip;
-    const uint64_t bo = ballot(active && open);
-    steps++;
-    visits += __popcll(ballot(active));
-    accepts += __popcll(ballot(accept));
-    // entries of a subtree follow it, so skip > i; max() keeps the walk finite regardless
-    i = uniform(bo ? i + 1 : max(a.y, i + 1));
+    acc.add(blk);
   }
-  st[STAT_TREE_STEPS] += steps;
-  st[STAT_TREE_VISITS] += visits;
-  st[STAT_TREE_ACCEPT] += accepts;
-  st[STAT_NEAR] += nnear;
 }
 
-HBTX_DEV void direct_sum(const WalkArgs &A, int eb, int ee, bool valid, float px, float py, float pz, Kahan &acc,
-                         unsigned long long st[NSTATS])
+template <bool PERIODIC>
+HBTX_DEV float direct_term(const Entry *ent, int i, float px, float py, float pz, const DevParams &P, bool &near)
 {
-  const DevParams &P = A.P;
-  unsigned long long nnear = 0;
+  const float4 e = const_load((const float4 *)(ent + i), 0);
+  const float r2 = sep2<PERIODIC>(e.x, e.y, e.z, px, py, pz, P);
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  float term = -e.w * rinv;
+  const bool nr = r2 < P.h2;
+  if (wave_mask(nr))
+  {
+    near = true;
+    const float tn = spline_near(r2, rinv, e.w, P.hinv);
+    term = nr ? tn : term;
+  }
+  return term;
+}
+
+template <bool PERIODIC>
+HBTX_DEV void direct_sum(const Entry *ent, int eb, int ee, float px, float py, float pz, const DevParams &P,
+                         Kahan &acc, WalkCounts &wc)
+{
   int i = eb;
   for (; i + 4 <= ee; i += 4)
   {
-    float part = 0.f;
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-    {
-      const float4 e = const_load(A.ent_pm, i + k);
-      float dx = e.x - px, dy = e.y - py, dz = e.z - pz;
-      if (P.periodic)
-      {
-        dx = nearest_f(dx, P.box, P.box_half);
-        dy = nearest_f(dy, P.box, P.box_half);
-        dz = nearest_f(dz, P.box, P.box_half);
-      }
-      part += pair_term(dx * dx + dy * dy + dz * dz, e.w, valid, P, nnear);
-    }
+    bool nr = false;
+    float part = direct_term<PERIODIC>(ent, i, px, py, pz, P, nr);
+    part += direct_term<PERIODIC>(ent, i + 1, px, py, pz, P, nr);
+    part += direct_term<PERIODIC>(ent, i + 2, px, py, pz, P, nr);
+    part += direct_term<PERIODIC>(ent, i + 3, px, py, pz, P, nr);
+    wc.nnear += nr;
     acc.add(part);
   }
   for (; i < ee; i++)
   {
-    const float4 e = const_load(A.ent_pm, i);
-    float dx = e.x - px, dy = e.y - py, dz = e.z - pz;
-    if (P.periodic)
-    {
-      dx = nearest_f(dx, P.box, P.box_half);
-      dy = nearest_f(dy, P.box, P.box_half);
-      dz = nearest_f(dz, P.box, P.box_half);
-    }
-    acc.add(pair_term(dx * dx + dy * dy + dz * dz, e.w, valid, P, nnear));
+    bool nr = false;
+    acc.add(direct_term<PERIODIC>(ent, i, px, py, pz, P, nr));
   }
-  st[STAT_DIRECT_PAIRS] += (unsigned long long)__popcll(ballot(valid)) * (unsigned long long)(ee - eb);
-  st[STAT_NEAR] += nnear;
 }
 
 // Snapshot_t::RelativeVelocity in float (src/snapshot.h:135-148)
@@ -168,14 +187,13 @@ HBTX_DEV void rel_vel(const DevParams &P, float x0, float x1, float x2, float v0
   }
 }
 
+template <bool PERIODIC, bool STATS>
 __global__ __launch_bounds__(64) void k_walk_energy(WalkArgs A)
 {
   const DevParams &P = A.P;
   const int lane = lane_id();
-  unsigned long long st[NSTATS];
-#pragma unroll
-  for (int k = 0; k < NSTATS; k++)
-    st[k] = 0;
+  unsigned long long st_steps = 0, st_visits = 0, st_accepts = 0, st_pairs = 0, st_targets = 0, st_items = 0,
+                     st_near = 0;
   for (;;)
   {
     int it = 0;
@@ -196,12 +214,27 @@ __global__ __launch_bounds__(64) void k_walk_energy(WalkArgs A)
     const int eb = uniform(A.ent_begin[s]), ee = uniform(A.ent_end[s]);
     Kahan acc;
     acc.init();
-    if (A.treemode[s])
-      tree_walk(A, eb, ee, valid, x.x, x.y, x.z, acc, st);
-    else
-      direct_sum(A, eb, ee, valid, x.x, x.y, x.z, acc, st);
-    st[STAT_TARGETS] += __popcll(ballot(valid));
-    st[STAT_ITEMS] += 1;
+    WalkCounts wc = {0u, 0u, 0u, 0u};
+    if (eb < ee)
+    {
+      if (A.treemode[s])
+        tree_walk<PERIODIC, STATS>(A.ent, eb, ee, valid, x.x, x.y, x.z, P, acc, wc);
+      else
+      {
+        direct_sum<PERIODIC>(A.ent, eb, ee, x.x, x.y, x.z, P, acc, wc);
+        if (STATS)
+          st_pairs += (unsigned long long)__popcll(wave_mask(valid)) * (unsigned long long)(ee - eb);
+      }
+    }
+    if (STATS)
+    {
+      st_steps += wc.steps;
+      st_visits += wc.visits;
+      st_accepts += wc.accepts;
+      st_near += wc.nnear;
+      st_targets += __popcll(wave_mask(valid));
+      st_items += 1;
+    }
     if (!valid)
       continue;
     const int md = A.mode[s];
@@ -232,12 +265,15 @@ __global__ __launch_bounds__(64) void k_walk_energy(WalkArgs A)
       A.energy[p] = (float)((double)A.energy[p] + ((double)tkin - phi));
     }
   }
-  if (A.stats && lane == 0)
+  if (STATS && A.stats && lane == 0)
   {
-#pragma unroll
-    for (int k = 0; k < NSTATS; k++)
-      if (st[k])
-        atomicAdd(A.stats + k, st[k]);
+    atomicAdd(A.stats + STAT_TREE_STEPS, st_steps);
+    atomicAdd(A.stats + STAT_TREE_VISITS, st_visits);
+    atomicAdd(A.stats + STAT_TREE_ACCEPT, st_accepts);
+    atomicAdd(A.stats + STAT_DIRECT_PAIRS, st_pairs);
+    atomicAdd(A.stats + STAT_TARGETS, st_targets);
+    atomicAdd(A.stats + STAT_ITEMS, st_items);
+    atomicAdd(A.stats + STAT_NEAR, st_near);
   }
 }
 

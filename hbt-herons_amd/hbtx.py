@@ -26,6 +26,7 @@ PLAIN = 1
 WARMSTART = 2
 SAVE_POTENTIAL = 4
 NO_GRAPH = 8
+COUNT = 16
 
 
 class HbtxError(RuntimeError):

@@ -36,6 +36,8 @@ extern "C" {
 #define HBTX_WARMSTART 2u        /* start from the proposed bound set (hbtx_batch_set_proposal) */
 #define HBTX_SAVE_POTENTIAL 4u   /* fill out_potential (ParticlePotentialEnergies) */
 #define HBTX_NO_GRAPH 8u         /* launch kernels directly instead of replaying the captured round graph */
+#define HBTX_COUNT 16u           /* count walk work (entries visited, interactions) for hbtx_batch_stats;
+                                    results are identical, timing is not representative */
 
 typedef struct hbtx_ctx hbtx_ctx;
 typedef struct hbtx_tree hbtx_tree;
@@ -128,7 +130,7 @@ int hbtx_batch_set_proposal(hbtx_ctx *ctx, const uint8_t *proposal);
 int hbtx_batch_run(hbtx_ctx *ctx, const hbtx_params *prm, uint32_t flags, double *elapsed_ms);
 int hbtx_batch_fetch(hbtx_ctx *ctx, int64_t *out_order, float *out_energy, float *out_potential,
                      hbtx_sub_result *out_res);
-/* Kernel-level counters of the last run: stats[0..15] (see hbtx_stat_names). */
+/* Kernel-level counters of the last run (work counts need HBTX_COUNT): stats[0..15], see hbtx_stat_name. */
 int hbtx_batch_stats(hbtx_ctx *ctx, double *stats, int nstats);
 const char *hbtx_stat_name(int i);
 
