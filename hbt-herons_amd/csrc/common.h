@@ -27,7 +27,8 @@ enum : int32_t
 {
   MODE_IDLE = 0,
   MODE_FULL = 1,  // all current candidates are targets, sources = candidates (or subsample)
-  MODE_CORR = 2   // correction: sources = particles removed in the previous round
+  MODE_CORR = 2,  // correction: sources = particles removed in the previous round
+  MODE_REFINE = 3 // centre refinement: self-binding energies of the most bound particles
 };
 
 // Device copy of the unbinding parameters (derived values precomputed on the host

@@ -10,6 +10,7 @@
 #include "walk.h"
 #include "unbind.h"
 #include "points.h"
+#include "subsample.h"
 
 #include <algorithm>
 #include <cmath>
@@ -110,7 +111,7 @@ struct Batch
   DArr<int8_t> ptype;
   DArr<int32_t> sub_of, sub_zero, off, off_pairs, order, order_zero;
   DArr<int32_t> state, cur, cur2, src, src_sub, fvals, ford;
-  DArr<float> energy, selfm, fenergy, fpot;
+  DArr<float> energy, selfm, fenergy, fpot, eref;
   DArr<uint8_t> proposal;
   DArr<int32_t> flag, fscan, crank, cnt, cscan;
   DArr<int64_t> flag64, fscan64;
@@ -122,7 +123,7 @@ struct Batch
   DArr<int32_t> icnt, iscan;
   // subhaloes
   DArr<int32_t> status, corr, iters, mode, treemode, nlast, nbound, nrem, nuns, ntgt, nsrc, src_off, ent_begin,
-    ent_end, nitems, bnd_cnt, bnd_tr, first_idx, seg_begin, seg_len, rbegin, rlen, type_cnt, first_tracer;
+    ent_end, nitems, bnd_cnt, bnd_tr, first_idx, seg_begin, seg_len, rbegin, rlen, type_cnt, first_tracer, aux32;
   DArr<float> mlast, mbound, ref, oldref, mb, upscale, ref_in, mb_in;
   DArr<double> cube, len0, red, red2;
   DArr<uint32_t> bbox;
@@ -349,6 +350,7 @@ void batch_reserve(hbtx_ctx *c, Batch &B)
   B.selfm.alloc(n);
   B.fenergy.alloc(n);
   B.fpot.alloc(n);
+  B.eref.alloc(n);
   B.proposal.alloc(n);
   B.flag.alloc(n + 1);
   B.fscan.alloc(n + 1);
@@ -366,7 +368,8 @@ void batch_reserve(hbtx_ctx *c, Batch &B)
   B.iscan.alloc(ns + 1);
   DArr<int32_t> *i32s[] = {&B.status, &B.corr, &B.iters, &B.mode, &B.treemode, &B.nlast, &B.nbound, &B.nrem,
                            &B.nuns, &B.ntgt, &B.nsrc, &B.src_off, &B.ent_begin, &B.ent_end, &B.nitems, &B.bnd_cnt,
-                           &B.bnd_tr, &B.first_idx, &B.seg_begin, &B.seg_len, &B.rbegin, &B.rlen, &B.first_tracer};
+                           &B.bnd_tr, &B.first_idx, &B.seg_begin, &B.seg_len, &B.rbegin, &B.rlen, &B.first_tracer,
+                           &B.aux32};
   for (auto *a : i32s)
     a->alloc(ns + 1);
   B.type_cnt.alloc(6 * (int64_t)ns + 6);
@@ -473,6 +476,8 @@ TreeArgs tree_args(Batch &B, View &V, const DevParams &P)
   return T;
 }
 
+void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P);
+
 // Sources of the round -> dense list, then (tree-mode subhaloes) the octree.
 void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int round)
 {
@@ -480,20 +485,32 @@ void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, 
   const int64_t n = V.n;
   const int gb = blocks_for(n);
   const int gs = blocks_for(V.nsub, 256, 1 << 20);
-  const int32_t *crank = nullptr;
   if (P.mss > 0)
   {
+    // candidate ranks in E-list order and the mass upscale factors (:290-391)
     k_mark_candidates<<<gb, 256, 0, st>>>(V.S, V.Q, n, B.flag.p);
     scan32(c, B.flag.p, B.crank.p, n);
-    crank = B.crank.p;
+    k_upscale_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p);
+    const int32_t *nt = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, n);
+    k_upscale_tiles<<<red_grid(c, n, V.nsub), 256, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, nt, B.crank.p,
+                                                            c->red_partial.p);
+    k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red2);
+    k_upscale_finish<<<gs, 256, 0, st>>>(V.S, P, V.red2);
   }
   else
-  {
     // rank is irrelevant without subsampling (every candidate is a source)
     k_fill_i32<<<gb, 256, 0, st>>>(B.crank.p, n + 1, 0);
-    crank = B.crank.p;
-  }
-  k_mark_sources<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, round, crank, B.flag.p);
+  k_mark_sources<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, round, B.crank.p, B.flag.p);
+  build_sources_tree(c, B, V, P);
+}
+
+// Flags in B.flag -> dense source list, tree cube, keys, sort, octree entries.
+void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
+{
+  hipStream_t st = c->stream;
+  const int64_t n = V.n;
+  const int gb = blocks_for(n);
+  const int gs = blocks_for(V.nsub, 256, 1 << 20);
   scan32(c, B.flag.p, B.fscan.p, n);
   k_scatter_sources<<<gb, 256, 0, st>>>(V.S, n, B.flag.p, B.fscan.p, B.src.p, B.src_sub.p);
   k_sub_src<<<gs, 256, 0, st>>>(V.S, B.fscan.p);
@@ -555,6 +572,7 @@ WalkArgs walk_args(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   W.ref = V.S.ref;
   W.oldref = V.S.oldref;
   W.energy = V.Q.energy;
+  W.eref = B.eref.p + V.p0;
   W.phi_out = nullptr;
   W.stats = c->stats.p;
   W.P = P;
@@ -602,6 +620,12 @@ void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int rou
   scan64(c, B.flag64.p, B.fscan64.p, n);
   k_partition_scatter<<<gb, 256, 0, st>>>(V.S, V.Q, n, round, B.fscan64.p);
   k_copy_cur<<<gb, 256, 0, st>>>(V.S, V.Q, n, B.fscan64.p);
+  if (P.mss > 0)
+  {
+    k_fill_i32<<<gs, 256, 0, st>>>(B.aux32.p, V.nsub, 0);
+    k_count_unsampled<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, B.aux32.p);
+    k_set_unsampled<<<gs, 256, 0, st>>>(V.S, B.aux32.p);
+  }
   k_round_decide<<<gs, 256, 0, st>>>(V.S, P, B.fscan64.p);
   k_com_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p);
   if (P.periodic)
@@ -746,6 +770,18 @@ void run_final(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot
   k_final_keys<<<gb, 256, 0, st>>>(V.S, V.Q, n, B.keys.p, V.fvals);
   k_final_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
   seg_sort(c, B.keys.p, V.fvals, V.nsub, B.seg_begin.p, B.seg_len.p, n, 48);
+  if (P.refine)
+  {
+    // RefineBindingEnergyOrder (:395-425): tree over the most bound particles, true masses
+    k_refine_plan<<<gs, 256, 0, st>>>(V.S, P);
+    k_slot_of<<<gb, 256, 0, st>>>(V.S, n, V.fvals, V.Q.cur2);
+    k_mark_refine<<<gb, 256, 0, st>>>(V.S, V.Q, n, V.Q.cur2, B.flag.p);
+    build_sources_tree(c, B, V, P);
+    round_walk(c, B, V, P);
+    k_refine_keys<<<gb, 256, 0, st>>>(V.S, n, V.fvals, B.eref.p + V.p0, B.keys.p, B.seg_begin.p, B.seg_len.p);
+    k_refine_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
+    seg_sort(c, B.keys.p, V.fvals, V.nsub, B.seg_begin.p, B.seg_len.p, n, 32);
+  }
   k_final_local<<<gb, 256, 0, st>>>(V.S, n, V.fvals, V.Q.energy, V.ford, V.fenergy);
   // kinematics / types over the bound particles (input order)
   k_final_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p);
@@ -763,6 +799,20 @@ void run_final(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot
   HIPCHK(hipGetLastError());
 }
 
+__global__ void k_all_ranges(SubArrays S, int32_t *begin, int32_t *len)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub)
+    return;
+  begin[s] = S.off[s];
+  len[s] = S.off[s + 1] - S.off[s];
+}
+
+void k_final_ranges_all(hbtx_ctx *c, Batch &B, View &V)
+{
+  k_all_ranges<<<blocks_for(V.nsub, 256, 1 << 20), 256, 0, c->stream>>>(V.S, B.rbegin.p, B.rlen.p);
+}
+
 // Iterate one view to convergence. Returns rounds used.
 int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot)
 {
@@ -771,7 +821,17 @@ int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot)
   const int gs = blocks_for(V.nsub, 256, 1 << 20);
   k_init_particles<<<gb, 256, 0, st>>>(V.S, V.Q, V.n);
   k_init_subs<<<gs, 256, 0, st>>>(V.S, B.ref_in.p + 6 * V.s0, B.mb_in.p + 6 * V.s0, nullptr);
-  k_fill_i32<<<1, 64, 0, st>>>(B.misc.p, 4, 0);
+  k_fill_i32<<<1, 64, 0, st>>>(B.misc.p, 8, 0);
+  {
+    // initial Mbound (:537-541) and the leading do-not-subsample run (:522-524)
+    k_final_ranges_all(c, B, V);
+    const int32_t *nt = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, V.n);
+    k_fill_i32<<<gs, 256, 0, st>>>(B.aux32.p, V.nsub, 0x7fffffff);
+    k_init_mass_tiles<<<red_grid(c, V.n, V.nsub), 256, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, nt, c->red_partial.p,
+                                                                B.aux32.p);
+    k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
+    k_init_mass_finish<<<gs, 256, 0, st>>>(V.S, P, V.red, B.aux32.p);
+  }
   HIPCHK(hipGetLastError());
   int round = 0;
   const int max_rounds = 2000;
@@ -891,8 +951,7 @@ void run_batch(hbtx_ctx *c, const hbtx_params &prm, uint32_t flags, double *elap
   hbtx_params p = prm;
   if (p.direct_cutoff < 0)
     p.direct_cutoff = tuned_direct_cutoff();
-  REQUIRE(p.max_sample_size == 0, HBTX_E_NOT_IMPLEMENTED,
-          "MaxSampleSizeOfPotentialEstimate > 0 (subsampling) is not supported by this build");
+  REQUIRE(p.max_sample_size >= 0, HBTX_E_INVALID, "MaxSampleSizeOfPotentialEstimate must be >= 0");
   REQUIRE(!(flags & HBTX_WARMSTART), HBTX_E_NOT_IMPLEMENTED, "warm start is not supported by this build");
   const DevParams P = make_dev_params(p);
   hipStream_t st = c->stream;

@@ -383,7 +383,7 @@ __global__ void k_targets_from_sources(SubArrays S, PartArrays Q, const int32_t 
   GRID_STRIDE(j, ST)
   {
     const int s = src_sub[j];
-    if (S.mode[s] == MODE_FULL && S.treemode[s] && S.nsrc[s] == S.ntgt[s])
+    if ((S.mode[s] == MODE_FULL || S.mode[s] == MODE_REFINE) && S.treemode[s] && S.nsrc[s] == S.ntgt[s])
       Q.cur[S.off[s] + (int)(j - S.src_off[s])] = src[j];
   }
 }

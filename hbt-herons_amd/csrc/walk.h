@@ -54,6 +54,7 @@ struct WalkArgs
   const float *ref;         // per subhalo frame [6]
   const float *oldref;      // per subhalo previous frame [6]
   float *energy;
+  float *eref;              // refinement energies (MODE_REFINE targets)
   float *phi_out;           // optional: specific potential per particle (float), may be null
   unsigned long long *stats;
   DevParams P;
@@ -238,21 +239,26 @@ __global__ __launch_bounds__(64) void k_walk_energy(WalkArgs A)
     if (!valid)
       continue;
     const int md = A.mode[s];
-    const float self = (md == MODE_FULL) ? (A.selfm[p] / P.eps) : 0.f;
+    const float self = (md == MODE_FULL || md == MODE_REFINE) ? (A.selfm[p] / P.eps) : 0.f;
     const double phi = ((double)self + (double)acc.s - (double)acc.c) * P.G_over_a;
     if (A.phi_out)
       A.phi_out[p] = (float)phi;
     const float4 v = A.velu[p];
     const float *rf = A.ref + 6 * s;
-    if (md == MODE_FULL)
+    if (md == MODE_FULL || md == MODE_REFINE)
     {
       float dv[3];
       rel_vel(P, x.x, x.y, x.z, v.x, v.y, v.z, rf, rf + 3, dv);
       const float k2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
       float E = (float)((double)k2 * 0.5 + phi);
-      if (P.thermal)
-        E += v.w;
-      A.energy[p] = E;
+      if (md == MODE_REFINE)
+        A.eref[p] = E; // RefineBindingEnergyOrder: BindingEnergy with the true mass, no thermal term
+      else
+      {
+        if (P.thermal)
+          E += v.w;
+        A.energy[p] = E;
+      }
     }
     else
     {
