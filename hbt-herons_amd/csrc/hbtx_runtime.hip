@@ -11,6 +11,7 @@
 #include "unbind.h"
 #include "points.h"
 #include "subsample.h"
+#include "learn.h"
 
 #include <algorithm>
 #include <cmath>
@@ -124,7 +125,14 @@ struct Batch
   // subhaloes
   DArr<int32_t> status, corr, iters, mode, treemode, nlast, nbound, nrem, nuns, ntgt, nsrc, src_off, ent_begin,
     ent_end, nitems, bnd_cnt, bnd_tr, first_idx, seg_begin, seg_len, rbegin, rlen, type_cnt, first_tracer, aux32;
-  DArr<float> mlast, mbound, ref, oldref, mb, upscale, ref_in, mb_in;
+  DArr<float> mlast, mbound, ref, oldref, mb, upscale, ref_in, mb_in, minit;
+  DArr<int32_t> warm, npro, reset_;
+  DArr<float> feat, logit;
+  // previous snapshot's outputs for the learned warm start (same slot = same particle)
+  DArr<int32_t> prev_rank;
+  DArr<uint8_t> prev_bound;
+  DArr<float> prev_energy, prev_com, prev_mb, prev_rs;
+  bool has_prev = false;
   DArr<double> cube, len0, red, red2;
   DArr<uint32_t> bbox;
   DArr<int32_t> misc; // [0] active, [1] work counter, [2] overflow, [3] entry total
@@ -161,6 +169,14 @@ struct hbtx_tree
   DArr<int32_t> src_of_entry;
   DArr<double> cube;
   int32_t treemode = 1;
+};
+
+struct hbtx_mlp
+{
+  DArr<float> master, m, v, wf, grad, partial;
+  DArr<uint16_t> wb;
+  DArr<double> lossp, loss;
+  int64_t step = 0;
 };
 
 struct hbtx_ctx
@@ -373,7 +389,10 @@ void batch_reserve(hbtx_ctx *c, Batch &B)
   for (auto *a : i32s)
     a->alloc(ns + 1);
   B.type_cnt.alloc(6 * (int64_t)ns + 6);
-  DArr<float> *f1[] = {&B.mlast, &B.mbound};
+  DArr<int32_t> *w3[] = {&B.warm, &B.npro, &B.reset_};
+  for (auto *a : w3)
+    a->alloc(ns + 1);
+  DArr<float> *f1[] = {&B.mlast, &B.mbound, &B.minit};
   for (auto *a : f1)
     a->alloc(ns + 1);
   DArr<float> *f6[] = {&B.ref, &B.oldref, &B.mb, &B.upscale, &B.ref_in, &B.mb_in};
@@ -429,6 +448,10 @@ View make_view(Batch &B, int64_t p0, int64_t n, int32_t s0, int32_t nsub, bool w
   S.bnd_tr = B.bnd_tr.p + s0;
   S.first_idx = B.first_idx.p + s0;
   S.active_count = B.misc.p + 0;
+  S.warm = B.warm.p + s0;
+  S.npro = B.npro.p + s0;
+  S.reset = B.reset_.p + s0;
+  S.minit = B.minit.p + s0;
   PartArrays &Q = V.Q;
   Q.posm = B.posm.p + p0;
   Q.velu = B.velu.p + p0;
@@ -438,6 +461,7 @@ View make_view(Batch &B, int64_t p0, int64_t n, int32_t s0, int32_t nsub, bool w
   Q.selfm = B.selfm.p + p0;
   Q.cur = B.cur.p + p0;
   Q.cur2 = B.cur2.p + p0;
+  Q.proposal = B.has_proposal ? B.proposal.p + p0 : nullptr;
   V.order = whole ? B.order.p : B.order_zero.p;
   V.ford = B.ford.p + p0;
   V.fenergy = B.fenergy.p + p0;
@@ -610,7 +634,7 @@ void round_walk(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   HIPCHK(hipGetLastError());
 }
 
-void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int round)
+void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int round, bool warm_round)
 {
   hipStream_t st = c->stream;
   const int64_t n = V.n;
@@ -626,7 +650,12 @@ void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int rou
     k_count_unsampled<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, B.aux32.p);
     k_set_unsampled<<<gs, 256, 0, st>>>(V.S, B.aux32.p);
   }
-  k_round_decide<<<gs, 256, 0, st>>>(V.S, P, B.fscan64.p);
+  if (warm_round)
+  {
+    k_fill_i32<<<gs, 256, 0, st>>>(B.aux32.p, V.nsub, 0);
+    k_warm_count<<<gb, 256, 0, st>>>(V.S, V.Q, n, B.aux32.p);
+  }
+  k_round_decide<<<gs, 256, 0, st>>>(V.S, P, B.fscan64.p, warm_round ? B.aux32.p : nullptr);
   k_com_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p);
   if (P.periodic)
   {
@@ -640,6 +669,11 @@ void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int rou
   k_round_finish<<<gs, 256, 0, st>>>(V.S, P, V.red);
   if (P.periodic)
     k_round_finish_periodic<<<gs, 256, 0, st>>>(V.S, V.Q, P, V.red);
+  if (warm_round)
+  {
+    k_warm_reset_particles<<<gb, 256, 0, st>>>(V.S, V.Q, n);
+    k_warm_reset_subs<<<gs, 256, 0, st>>>(V.S, B.ref_in.p + 6 * V.s0);
+  }
   HIPCHK(hipGetLastError());
 }
 
@@ -814,7 +848,7 @@ void k_final_ranges_all(hbtx_ctx *c, Batch &B, View &V)
 }
 
 // Iterate one view to convergence. Returns rounds used.
-int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot)
+int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, bool warm)
 {
   hipStream_t st = c->stream;
   const int gb = blocks_for(V.n);
@@ -830,7 +864,7 @@ int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot)
     k_init_mass_tiles<<<red_grid(c, V.n, V.nsub), 256, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, nt, c->red_partial.p,
                                                                 B.aux32.p);
     k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
-    k_init_mass_finish<<<gs, 256, 0, st>>>(V.S, P, V.red, B.aux32.p);
+    k_init_mass_finish<<<gs, 256, 0, st>>>(V.S, P, V.red, B.aux32.p, warm ? 1 : 0);
   }
   HIPCHK(hipGetLastError());
   int round = 0;
@@ -842,8 +876,15 @@ int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot)
     k_fill_i32<<<1, 64, 0, st>>>(V.S.active_count, 1, 0);
     k_plan<<<gs, 256, 0, st>>>(V.S, P, round);
     round_sources_and_tree(c, B, V, P, round);
+    if (warm && round == 1)
+    {
+      // warm round: every target in key order of the proposal's tree cube
+      k_warm_keys<<<gb, 256, 0, st>>>(V.S, V.Q, V.n, B.keys.p, B.seg_begin.p, B.seg_len.p);
+      k_warm_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
+      seg_sort(c, B.keys.p, V.Q.cur, V.nsub, B.seg_begin.p, B.seg_len.p, V.n, 63);
+    }
     round_walk(c, B, V, P);
-    round_partition(c, B, V, P, round);
+    round_partition(c, B, V, P, round, warm && round == 1);
     int32_t h_misc[4];
     HIPCHK(hipMemcpyAsync(h_misc, B.misc.p, sizeof(h_misc), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -872,6 +913,7 @@ void upload_batch(hbtx_ctx *c, Batch &B, int64_t n, const float *pos, const floa
   B.has_type = type != nullptr;
   B.has_u = u != nullptr;
   B.has_proposal = false;
+  B.has_prev = false;
   B.h_off.assign(offsets, offsets + nsub + 1);
   B.h_count.resize(nsub);
   B.max_count = 0;
@@ -952,7 +994,9 @@ void run_batch(hbtx_ctx *c, const hbtx_params &prm, uint32_t flags, double *elap
   if (p.direct_cutoff < 0)
     p.direct_cutoff = tuned_direct_cutoff();
   REQUIRE(p.max_sample_size >= 0, HBTX_E_INVALID, "MaxSampleSizeOfPotentialEstimate must be >= 0");
-  REQUIRE(!(flags & HBTX_WARMSTART), HBTX_E_NOT_IMPLEMENTED, "warm start is not supported by this build");
+  const bool warm = (flags & HBTX_WARMSTART) != 0;
+  REQUIRE(!warm || B.has_proposal, HBTX_E_INVALID, "warm start needs a proposal (hbtx_batch_set_proposal / propose)");
+  REQUIRE(!warm || p.max_sample_size == 0, HBTX_E_INVALID, "warm start is exclusive with subsampling");
   const DevParams P = make_dev_params(p);
   hipStream_t st = c->stream;
   HIPCHK(hipMemsetAsync(c->stats.p, 0, NSTATS * sizeof(unsigned long long), st));
@@ -966,13 +1010,13 @@ void run_batch(hbtx_ctx *c, const hbtx_params &prm, uint32_t flags, double *elap
     for (int32_t s = 0; s < B.nsub; s++)
     {
       View V = make_view(B, B.h_off[s], B.h_count[s], s, 1, false);
-      c->rounds += run_view(c, B, V, P, want_pot);
+      c->rounds += run_view(c, B, V, P, want_pot, warm);
     }
   }
   else
   {
     View V = make_view(B, 0, B.n, 0, B.nsub, true);
-    c->rounds = run_view(c, B, V, P, want_pot);
+    c->rounds = run_view(c, B, V, P, want_pot, warm);
   }
   HIPCHK(hipEventRecord(c->ev1, st));
   HIPCHK(hipEventSynchronize(c->ev1));
@@ -1498,6 +1542,319 @@ int hbtx_segmented_sort(hbtx_ctx *c, int64_t n, uint64_t *keys, int32_t *vals, i
     HIPCHK(hipMemcpyAsync(keys, dk.p, n * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(vals, dv.p, n * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+  });
+}
+
+} // extern "C"
+
+namespace
+{
+uint64_t splitmix(uint64_t &x)
+{
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+double urand(uint64_t &x)
+{
+  return (double)(splitmix(x) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+void mlp_alloc(hbtx_mlp *M)
+{
+  M->master.alloc(MLP_NPAR);
+  M->m.alloc(MLP_NPAR);
+  M->v.alloc(MLP_NPAR);
+  M->wf.alloc(MLP_NPAR);
+  M->grad.alloc(MLP_NPAR);
+  M->wb.alloc(MLP_NPAR);
+  M->loss.alloc(1);
+}
+
+void mlp_upload(hbtx_ctx *c, hbtx_mlp *M, const float *master)
+{
+  hipStream_t st = c->stream;
+  HIPCHK(hipMemcpyAsync(M->master.p, master, MLP_NPAR * sizeof(float), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(M->m.p, 0, MLP_NPAR * sizeof(float), st));
+  HIPCHK(hipMemsetAsync(M->v.p, 0, MLP_NPAR * sizeof(float), st));
+  k_mlp_round<<<(MLP_NPAR + 255) / 256, 256, 0, st>>>(M->master.p, M->wf.p, M->wb.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  M->step = 0;
+}
+
+// forward+backward of nb samples (rows idx[0..nb) of dX, or rows 0..nb) -> M->grad, M->loss
+void mlp_grad_device(hbtx_ctx *c, hbtx_mlp *M, const float *dX, const uint8_t *dy, const int32_t *idx, int64_t nb)
+{
+  hipStream_t st = c->stream;
+  const int nblk = (int)((nb + MLP_BLOCK - 1) / MLP_BLOCK);
+  M->partial.alloc((int64_t)nblk * MLP_NPAR);
+  M->lossp.alloc(nblk);
+  k_mlp_grad<<<nblk, MLP_BLOCK, 0, st>>>(M->wf.p, dX, dy, idx, nb, 1.f / (float)nb, M->partial.p, M->lossp.p);
+  k_mlp_reduce<<<(MLP_NPAR + 255) / 256, 256, 0, st>>>(M->partial.p, nblk, M->grad.p, M->lossp.p, M->loss.p);
+  HIPCHK(hipGetLastError());
+}
+} // namespace
+
+extern "C" {
+
+int hbtx_mlp_create(hbtx_ctx *c, uint64_t seed, hbtx_mlp **out)
+{
+  if (!c || !out)
+    return HBTX_E_INVALID;
+  *out = nullptr;
+  hbtx_mlp *M = new (std::nothrow) hbtx_mlp();
+  if (!M)
+    return HBTX_E_ALLOC;
+  try
+  {
+    HIPCHK(hipSetDevice(c->device));
+    mlp_alloc(M);
+    std::vector<float> w(MLP_NPAR, 0.f);
+    uint64_t x = seed;
+    const double l1 = std::sqrt(6.0 / MLP_IN), l2 = std::sqrt(6.0 / MLP_H), l3 = std::sqrt(3.0 / MLP_H);
+    for (int i = 0; i < MLP_H * MLP_IN; i++)
+      w[MLP_OFF_W1 + i] = (float)((2.0 * urand(x) - 1.0) * l1);
+    for (int i = 0; i < MLP_H * MLP_H; i++)
+      w[MLP_OFF_W2 + i] = (float)((2.0 * urand(x) - 1.0) * l2);
+    for (int i = 0; i < MLP_H; i++)
+      w[MLP_OFF_W3 + i] = (float)((2.0 * urand(x) - 1.0) * l3);
+    mlp_upload(c, M, w.data());
+  }
+  catch (const Error &e)
+  {
+    delete M;
+    return fail(c, e.code, e.what());
+  }
+  *out = M;
+  return HBTX_OK;
+}
+
+void hbtx_mlp_free(hbtx_mlp *M)
+{
+  delete M;
+}
+
+int hbtx_mlp_get_weights(hbtx_ctx *c, hbtx_mlp *M, float *master, uint16_t *bf16)
+{
+  if (!c || !M)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (master)
+      HIPCHK(hipMemcpy(master, M->master.p, MLP_NPAR * sizeof(float), hipMemcpyDeviceToHost));
+    if (bf16)
+      HIPCHK(hipMemcpy(bf16, M->wb.p, MLP_NPAR * sizeof(uint16_t), hipMemcpyDeviceToHost));
+  });
+}
+
+int hbtx_mlp_set_weights(hbtx_ctx *c, hbtx_mlp *M, const float *master)
+{
+  if (!c || !M || !master)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    mlp_upload(c, M, master);
+  });
+}
+
+int hbtx_mlp_gradient(hbtx_ctx *c, hbtx_mlp *M, int64_t n, const float *X, const uint8_t *y, float *grad,
+                      double *loss)
+{
+  if (!c || !M || n <= 0 || !X || !y)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    DArr<float> dX;
+    DArr<uint8_t> dy;
+    dX.alloc(n * MLP_IN);
+    dy.alloc(n);
+    HIPCHK(hipMemcpyAsync(dX.p, X, n * MLP_IN * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dy.p, y, n, hipMemcpyHostToDevice, st));
+    mlp_grad_device(c, M, dX.p, dy.p, nullptr, n);
+    double l = 0.0;
+    if (grad)
+      HIPCHK(hipMemcpyAsync(grad, M->grad.p, MLP_NPAR * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&l, M->loss.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (loss)
+      *loss = l / (double)n;
+  });
+}
+
+int hbtx_mlp_train(hbtx_ctx *c, hbtx_mlp *M, int64_t n, const float *X, const uint8_t *y, int32_t epochs,
+                   int32_t batch, float lr, uint64_t seed, double *mean_loss)
+{
+  if (!c || !M || n <= 0 || !X || !y || epochs <= 0 || batch <= 0)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    REQUIRE(n < (int64_t)0x7fffffff, HBTX_E_INVALID, "too many training samples");
+    hipStream_t st = c->stream;
+    DArr<float> dX;
+    DArr<uint8_t> dy;
+    DArr<int32_t> didx;
+    dX.alloc(n * MLP_IN);
+    dy.alloc(n);
+    didx.alloc(n);
+    HIPCHK(hipMemcpyAsync(dX.p, X, n * MLP_IN * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dy.p, y, n, hipMemcpyHostToDevice, st));
+    std::vector<int32_t> perm(n);
+    uint64_t rs = seed;
+    const float b1 = 0.9f, b2 = 0.999f;
+    double last = 0.0;
+    std::vector<double> lbatch;
+    for (int ep = 0; ep < epochs; ep++)
+    {
+      std::iota(perm.begin(), perm.end(), 0);
+      for (int64_t i = n - 1; i > 0; i--)
+      {
+        const int64_t j = (int64_t)(splitmix(rs) % (uint64_t)(i + 1));
+        std::swap(perm[i], perm[j]);
+      }
+      HIPCHK(hipMemcpyAsync(didx.p, perm.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+      DArr<double> losses;
+      const int64_t nsteps = (n + batch - 1) / batch;
+      losses.alloc(nsteps);
+      for (int64_t k = 0; k < nsteps; k++)
+      {
+        const int64_t b0 = k * batch, nb = std::min<int64_t>(batch, n - b0);
+        mlp_grad_device(c, M, dX.p, dy.p, didx.p + b0, nb);
+        M->step++;
+        const float c1 = 1.f - std::pow(b1, (float)M->step), c2 = 1.f - std::pow(b2, (float)M->step);
+        k_mlp_adam<<<(MLP_NPAR + 255) / 256, 256, 0, st>>>(M->master.p, M->m.p, M->v.p, M->grad.p, M->wf.p,
+                                                            M->wb.p, lr, b1, b2, 1e-8f, c1, c2);
+        HIPCHK(hipMemcpyAsync(losses.p + k, M->loss.p, sizeof(double), hipMemcpyDeviceToDevice, st));
+      }
+      lbatch.resize(nsteps);
+      HIPCHK(hipMemcpyAsync(lbatch.data(), losses.p, nsteps * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      double tot = 0.0;
+      for (double v : lbatch)
+        tot += v;
+      last = tot / (double)n;
+    }
+    if (mean_loss)
+      *mean_loss = last;
+  });
+}
+
+int hbtx_mlp_predict(hbtx_ctx *c, hbtx_mlp *M, int64_t n, const float *X, float *logit)
+{
+  if (!c || !M || n < 0 || (n > 0 && (!X || !logit)))
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    if (n == 0)
+      return HBTX_OK;
+    hipStream_t st = c->stream;
+    DArr<float> dX, dl;
+    dX.alloc(n * MLP_IN);
+    dl.alloc(n);
+    HIPCHK(hipMemcpyAsync(dX.p, X, n * MLP_IN * sizeof(float), hipMemcpyHostToDevice, st));
+    k_mlp_forward<<<blocks_for(n, 256, 1 << 30), 256, 0, st>>>(M->wf.p, dX.p, n, dl.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(logit, dl.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  });
+}
+
+int hbtx_batch_set_previous(hbtx_ctx *c, const int32_t *prev_rank, const uint8_t *prev_bound,
+                            const float *prev_energy, const float *prev_com_pos, const float *prev_com_vel,
+                            const float *prev_mbound, const float *prev_rscale)
+{
+  if (!c || !prev_rank || !prev_bound || !prev_energy || !prev_com_pos || !prev_com_vel || !prev_mbound ||
+      !prev_rscale)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    Batch &B = c->B;
+    REQUIRE(B.n > 0, HBTX_E_INVALID, "no batch loaded");
+    hipStream_t st = c->stream;
+    const int64_t n = B.n, ns = B.nsub;
+    B.prev_rank.alloc(n);
+    B.prev_bound.alloc(n);
+    B.prev_energy.alloc(n);
+    B.prev_com.alloc(6 * ns);
+    B.prev_mb.alloc(ns);
+    B.prev_rs.alloc(ns);
+    std::vector<float> com(6 * ns);
+    for (int64_t s = 0; s < ns; s++)
+      for (int k = 0; k < 3; k++)
+      {
+        com[6 * s + k] = prev_com_pos[3 * s + k];
+        com[6 * s + 3 + k] = prev_com_vel[3 * s + k];
+      }
+    HIPCHK(hipMemcpyAsync(B.prev_rank.p, prev_rank, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(B.prev_bound.p, prev_bound, n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(B.prev_energy.p, prev_energy, n * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(B.prev_com.p, com.data(), 6 * ns * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(B.prev_mb.p, prev_mbound, ns * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(B.prev_rs.p, prev_rscale, ns * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    B.has_prev = true;
+  });
+}
+
+int hbtx_batch_features(hbtx_ctx *c, const hbtx_params *prm, float *out_features)
+{
+  if (!c || !prm)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    Batch &B = c->B;
+    REQUIRE(B.n > 0 && B.has_prev, HBTX_E_INVALID, "load a batch and its previous-snapshot outputs first");
+    const DevParams P = make_dev_params(*prm);
+    hipStream_t st = c->stream;
+    const int64_t n = B.n;
+    B.feat.alloc(n * MLP_IN);
+    View V = make_view(B, 0, n, 0, B.nsub, true);
+    PrevArrays A{B.prev_rank.p, B.prev_bound.p, B.prev_energy.p, B.prev_com.p, B.prev_mb.p, B.prev_rs.p};
+    k_features<<<blocks_for(n), 256, 0, st>>>(V.S, V.Q, P, A, n, B.feat.p);
+    HIPCHK(hipGetLastError());
+    if (out_features)
+    {
+      HIPCHK(hipMemcpyAsync(out_features, B.feat.p, n * MLP_IN * sizeof(float), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+  });
+}
+
+int hbtx_batch_propose(hbtx_ctx *c, hbtx_mlp *M, float threshold, int64_t *nproposed, double *elapsed_ms)
+{
+  if (!c || !M || !(threshold > 0.f && threshold < 1.f))
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    Batch &B = c->B;
+    REQUIRE(B.n > 0 && B.feat.p, HBTX_E_INVALID, "compute the batch features first");
+    hipStream_t st = c->stream;
+    const int64_t n = B.n;
+    B.logit.alloc(n);
+    HIPCHK(hipEventRecord(c->wev0, st));
+    k_mlp_forward<<<blocks_for(n, 256, 1 << 30), 256, 0, st>>>(M->wf.p, B.feat.p, n, B.logit.p);
+    k_proposal_from_logit<<<blocks_for(n), 256, 0, st>>>(B.logit.p, n, std::log(threshold / (1.f - threshold)),
+                                                          B.proposal.p);
+    HIPCHK(hipEventRecord(c->wev1, st));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventSynchronize(c->wev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->wev0, c->wev1));
+    if (elapsed_ms)
+      *elapsed_ms = ms;
+    B.has_proposal = true;
+    if (nproposed)
+    {
+      std::vector<uint8_t> h(n);
+      HIPCHK(hipMemcpy(h.data(), B.proposal.p, n, hipMemcpyDeviceToHost));
+      int64_t cnt = 0;
+      for (int64_t i = 0; i < n; i++)
+        cnt += h[i];
+      *nproposed = cnt;
+    }
   });
 }
 

@@ -41,9 +41,23 @@ __global__ __launch_bounds__(256) void k_init_mass_tiles(SubArrays S, PartArrays
     int mymin = 0x7fffffff;
     for (int p = tl.y + threadIdx.x; p < tl.z; p += blockDim.x)
     {
-      v[0] += Q.posm[p].w;
+      const float4 x = Q.posm[p];
+      v[0] += x.w;
       if (!donot_subsample(P, ptype_of(Q, p)))
         mymin = min(mymin, p - S.off[s]);
+      if (Q.proposal && Q.proposal[p])
+      {
+        // proposed set: count, mass, and the mass-weighted frame of the warm round
+        const float4 u = Q.velu[p];
+        v[1] += 1.0;
+        v[2] += x.w;
+        v[3] += x.x * x.w;
+        v[4] += x.y * x.w;
+        v[5] += x.z * x.w;
+        v[6] += u.x * x.w;
+        v[7] += u.y * x.w;
+        v[8] += u.z * x.w;
+      }
     }
     __syncthreads();
     atomicMin(&lmin, mymin);
@@ -59,13 +73,28 @@ __global__ __launch_bounds__(256) void k_init_mass_tiles(SubArrays S, PartArrays
   }
 }
 
-__global__ void k_init_mass_finish(SubArrays S, DevParams P, const double *red, const int32_t *first_sampled)
+__global__ void k_init_mass_finish(SubArrays S, DevParams P, const double *red, const int32_t *first_sampled,
+                                   int warm_start)
 {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S.nsub)
     return;
   const int n = S.off[s + 1] - S.off[s];
-  S.mbound[s] = (float)red[(int64_t)RED_NV * s];
+  const double *v = red + (int64_t)RED_NV * s;
+  S.mbound[s] = (float)v[0];
+  S.minit[s] = (float)v[0];
+  if (warm_start && n > 0 && (int64_t)v[1] >= P.min_part)
+  {
+    // warm round in the proposed set's own centre-of-mass frame
+    S.warm[s] = 1;
+    S.npro[s] = (int)v[1];
+    if (!P.periodic)
+      for (int k = 0; k < 3; k++)
+      {
+        S.ref[6 * s + k] = (float)(v[3 + k] / v[2]);
+        S.ref[6 * s + 3 + k] = (float)(v[6 + k] / v[2]);
+      }
+  }
   // Nunsample (:522-524): only when subsampling can apply to this subhalo
   S.nuns[s] = (P.mss > 0 && n > P.mss) ? min(first_sampled[s], n) : 0;
 }

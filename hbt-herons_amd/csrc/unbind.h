@@ -31,6 +31,10 @@ struct SubArrays
   int32_t *bnd_cnt, *bnd_tr;
   int32_t *first_idx;       // smallest candidate index (periodic reference particle)
   int32_t *active_count;    // [1]
+  // learned warm start: warm = 0 off, 1 first (proposal-sourced) round pending, 2 done;
+  // npro = proposed particles; reset = proposal rejected, restart from all particles
+  int32_t *warm, *npro, *reset;
+  float *minit;             // initial bound mass (all particles)
 };
 
 struct PartArrays
@@ -38,6 +42,7 @@ struct PartArrays
   const float4 *posm;
   const float4 *velu;
   const int8_t *ptype; // may be null (all type 1)
+  const uint8_t *proposal; // warm-start proposal (1 = proposed bound), may be null
   int32_t *state;
   float *energy;
   float *selfm;
@@ -79,6 +84,9 @@ __global__ void k_init_subs(SubArrays S, const float *ref_in, const float *mb_in
   const int n = S.off[s + 1] - S.off[s];
   S.status[s] = (n == 0) ? ST_CONVERGED : ST_ACTIVE;
   S.corr[s] = 0;
+  S.warm[s] = 0;
+  S.npro[s] = 0;
+  S.reset[s] = 0;
   S.iters[s] = 0;
   S.nbound[s] = n;
   S.nlast[s] = n;
@@ -126,6 +134,13 @@ __global__ void k_plan(SubArrays S, DevParams P, int round)
       ns = (int)(P.mss + S.nuns[s]);
   }
   S.nlast[s] = nb;
+  if (S.warm[s] == 1)
+  {
+    // warm round: every particle is a target, the proposed set is the source set,
+    // and convergence is measured against the proposed set
+    ns = S.npro[s];
+    S.nlast[s] = ns;
+  }
   S.ntgt[s] = nb;
   S.nsrc[s] = ns;
   S.treemode[s] = ((int64_t)ns > P.direct_cutoff) ? 1 : 0;
@@ -155,7 +170,15 @@ __global__ void k_mark_sources(SubArrays S, PartArrays Q, DevParams P, int64_t n
     const int md = S.mode[s];
     int f = 0;
     float sm = 0.f;
-    if (md == MODE_FULL && Q.state[p] == 0)
+    if (md == MODE_FULL && Q.state[p] == 0 && S.warm[s] == 1)
+    {
+      if (Q.proposal[p])
+      {
+        f = 1;
+        sm = Q.posm[p].w;
+      }
+    }
+    else if (md == MODE_FULL && Q.state[p] == 0)
     {
       const int rank = crank[p] - crank[S.off[s]];
       if (rank < S.nsrc[s])
@@ -168,7 +191,8 @@ __global__ void k_mark_sources(SubArrays S, PartArrays Q, DevParams P, int64_t n
       }
     }
     else if (md == MODE_CORR && Q.state[p] == round - 1)
-      f = 1;
+      // after a warm round only the proposed particles were sources
+      f = (round == 2 && S.warm[s] == 2 && Q.proposal) ? (int)Q.proposal[p] : 1;
     flag[p] = f;
     Q.selfm[p] = sm;
   }
@@ -490,7 +514,7 @@ __global__ void k_copy_cur(SubArrays S, PartArrays Q, int64_t n, const int64_t *
 }
 
 // Disruption test and correction decision (:625-683).
-__global__ void k_round_decide(SubArrays S, DevParams P, const int64_t *fscan)
+__global__ void k_round_decide(SubArrays S, DevParams P, const int64_t *fscan, const int32_t *warm_np)
 {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S.nsub || S.mode[s] == MODE_IDLE)
@@ -503,6 +527,39 @@ __global__ void k_round_decide(SubArrays S, DevParams P, const int64_t *fscan)
   S.nrem[s] = nlast - nb;
   S.bnd_cnt[s] = nb;
   S.bnd_tr[s] = ntr;
+  if (S.warm[s] == 1)
+  {
+    // Proposal verification.  Accepted when no particle outside the proposed set
+    // is bound to it and at least half of it is bound (then the correction
+    // sequence continues from the proposed set); a few (<= 0.1%) bound outsiders
+    // get one exact round from the bound set; anything else, including too few
+    // bound particles, restarts the plain iteration from every particle.
+    const int np = warm_np ? warm_np[s] : 0;
+    const int n = S.off[s + 1] - S.off[s];
+    const int npro = S.npro[s];
+    S.warm[s] = 2;
+    const bool viable =
+      (int64_t)nb >= P.min_part && (int64_t)ntr >= P.min_tracer && 2 * (int64_t)nb >= (int64_t)npro;
+    if (viable && np == 0)
+    {
+      const int nrem = npro - nb; // proposed but unbound: the next correction's sources
+      S.nrem[s] = nrem;
+      if (nrem < nb)
+      {
+        S.corr[s] = 1;
+        for (int k = 0; k < 6; k++)
+          S.oldref[6 * s + k] = S.ref[6 * s + k];
+      }
+    }
+    else if (viable && (int64_t)np * 1000 <= (int64_t)nb)
+      S.nlast[s] = n;
+    else
+    {
+      S.reset[s] = 1;
+      S.nlast[s] = n;
+    }
+    return;
+  }
   if ((int64_t)nb < P.min_part || (int64_t)ntr < P.min_tracer)
   {
     S.status[s] = ST_DISRUPTED;
@@ -520,6 +577,73 @@ __global__ void k_round_decide(SubArrays S, DevParams P, const int64_t *fscan)
       for (int k = 0; k < 6; k++)
         S.oldref[6 * s + k] = S.ref[6 * s + k];
     }
+}
+
+// Warm round: bound particles that were not proposed.
+__global__ void k_warm_count(SubArrays S, PartArrays Q, int64_t n, int32_t *cnt)
+{
+  GRID_STRIDE(p, n)
+  {
+    const int s = S.sub_of[p];
+    if (S.mode[s] == MODE_FULL && S.warm[s] == 1 && Q.state[p] == 0 && !Q.proposal[p])
+      atomicAdd(cnt + s, 1);
+  }
+}
+
+// Rejected proposals: back to the starting state of a plain unbinding.
+__global__ void k_warm_reset_particles(SubArrays S, PartArrays Q, int64_t n)
+{
+  GRID_STRIDE(p, n)
+  {
+    const int s = S.sub_of[p];
+    if (S.reset[s])
+    {
+      Q.state[p] = 0;
+      Q.cur[p] = (int32_t)p;
+    }
+  }
+}
+
+__global__ void k_warm_reset_subs(SubArrays S, const float *ref_in)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub || !S.reset[s])
+    return;
+  const int n = S.off[s + 1] - S.off[s];
+  S.nbound[s] = n;
+  S.nlast[s] = n;
+  S.mbound[s] = S.minit[s];
+  S.corr[s] = 0;
+  for (int k = 0; k < 6; k++)
+    S.ref[6 * s + k] = ref_in[6 * s + k];
+  S.status[s] = ST_ACTIVE;
+  S.reset[s] = 0;
+}
+
+// Warm-round walk order: every target sorted by its key in the proposal's tree cube.
+__global__ void k_warm_keys(SubArrays S, PartArrays Q, int64_t n, uint64_t *keys, int32_t *seg_begin, int32_t *seg_len)
+{
+  GRID_STRIDE(q, n)
+  {
+    const int s = S.sub_of[q];
+    if (S.warm[s] == 1 && S.mode[s] == MODE_FULL && S.treemode[s] && S.nsrc[s] < S.ntgt[s])
+    {
+      const float4 x = Q.posm[Q.cur[q]];
+      const double *c = S.cube + 4 * s;
+      keys[q] = octree_key(x.x, x.y, x.z, c[0], c[1], c[2], c[3]);
+    }
+  }
+}
+
+__global__ void k_warm_segments(SubArrays S, int32_t *seg_begin, int32_t *seg_len)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S.nsub)
+    return;
+  seg_begin[s] = S.off[s];
+  seg_len[s] = (S.warm[s] == 1 && S.mode[s] == MODE_FULL && S.treemode[s] && S.nsrc[s] < S.ntgt[s])
+                 ? (S.off[s + 1] - S.off[s])
+                 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -693,7 +817,7 @@ __global__ void k_round_finish(SubArrays S, DevParams P, const double *red)
   S.mbound[s] = (float)M;
   const int64_t nb = S.nbound[s];
   const int64_t nlast = S.nlast[s];
-  if (nb >= nlast * P.prec)
+  if (!S.reset[s] && nb >= nlast * P.prec)
     S.status[s] = ST_CONVERGED;
   else
     atomicAdd(S.active_count, 1);

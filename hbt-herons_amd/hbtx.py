@@ -139,6 +139,18 @@ def lib():
         L.hbtx_tree_free.restype = None
         L.hbtx_direct_potential.argtypes = [P, C.POINTER(Params), i64, P, P, i64, P, P, P]
         L.hbtx_segmented_sort.argtypes = [P, i64, P, P, i64, P]
+        L.hbtx_mlp_create.argtypes = [P, C.c_uint64, C.POINTER(C.c_void_p)]
+        L.hbtx_mlp_free.argtypes = [P]
+        L.hbtx_mlp_free.restype = None
+        L.hbtx_mlp_get_weights.argtypes = [P, P, P, P]
+        L.hbtx_mlp_set_weights.argtypes = [P, P, P]
+        L.hbtx_mlp_gradient.argtypes = [P, P, i64, P, P, P, C.POINTER(C.c_double)]
+        L.hbtx_mlp_train.argtypes = [P, P, i64, P, P, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
+                                     C.POINTER(C.c_double)]
+        L.hbtx_mlp_predict.argtypes = [P, P, i64, P, P]
+        L.hbtx_batch_set_previous.argtypes = [P, P, P, P, P, P, P, P]
+        L.hbtx_batch_features.argtypes = [P, C.POINTER(Params), P]
+        L.hbtx_batch_propose.argtypes = [P, P, C.c_float, C.POINTER(i64), C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -224,6 +236,32 @@ class Context:
                                             flags, _p(order), _p(energy), _p(pot), _p(res)))
         return dict(order=order, energy=energy, potential=pot, result=res)
 
+    # --- learned warm start ----------------------------------------------------
+    def set_previous(self, prev):
+        """Upload the previous snapshot's outputs (a dict from learned.previous_state)."""
+        rank = np.ascontiguousarray(prev["rank"], dtype=np.int32)
+        bound = np.ascontiguousarray(prev["bound"], dtype=np.uint8)
+        energy = _f32(prev["energy"])
+        cp, cv = _f32(prev["com_pos"]), _f32(prev["com_vel"])
+        mb, rs = _f32(prev["mbound"]), _f32(prev["rscale"])
+        self._check(lib().hbtx_batch_set_previous(self._h, _p(rank), _p(bound), _p(energy), _p(cp), _p(cv),
+                                                  _p(mb), _p(rs)))
+
+    def features(self, params=None, fetch=True):
+        prm = params if isinstance(params, Params) else make_params(**(params or {}))
+        out = np.zeros((self._n, MLP_NIN), np.float32) if fetch else None
+        self._check(lib().hbtx_batch_features(self._h, C.byref(prm), _p(out)))
+        return out
+
+    def propose(self, mlp, threshold=0.5, count=True):
+        """Proposal = P(bound) > threshold from the batch features; returns
+        (number proposed or None, inference milliseconds)."""
+        npro = C.c_int64(0)
+        ms = C.c_double(0.0)
+        self._check(lib().hbtx_batch_propose(self._h, mlp._h, threshold, C.byref(npro) if count else None,
+                                             C.byref(ms)))
+        return (npro.value if count else None), ms.value
+
     # --- GravityTree_t API -----------------------------------------------------
     def tree(self, pos, mass, params=None):
         return Tree(self, pos, mass, params)
@@ -245,6 +283,74 @@ class Context:
         off = np.ascontiguousarray(offsets, dtype=np.int64)
         self._check(lib().hbtx_segmented_sort(self._h, len(k), _p(k), _p(v), len(off) - 1, _p(off)))
         return k, v
+
+
+MLP_NIN = 8
+MLP_NPAR = 4801
+MLP_LAYOUT = [("W1", (64, 8)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)), ("w3", (64,)), ("b3", (1,))]
+
+
+def mlp_unpack(flat):
+    """Split a flat parameter vector into the named arrays of the 8-64-64-1 network."""
+    out, k = {}, 0
+    for name, shape in MLP_LAYOUT:
+        size = int(np.prod(shape))
+        out[name] = np.asarray(flat[k:k + size]).reshape(shape)
+        k += size
+    return out
+
+
+class MLP:
+    """The warm-start classifier: FP32 master weights, BF16 working copy on the device."""
+
+    def __init__(self, ctx, seed=0):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        ctx._check(lib().hbtx_mlp_create(ctx._h, seed, C.byref(self._h)))
+
+    def weights(self):
+        master = np.zeros(MLP_NPAR, np.float32)
+        bf16 = np.zeros(MLP_NPAR, np.uint16)
+        self.ctx._check(lib().hbtx_mlp_get_weights(self.ctx._h, self._h, _p(master), _p(bf16)))
+        return master, bf16
+
+    def set_weights(self, master):
+        w = np.ascontiguousarray(master, dtype=np.float32)
+        assert w.shape == (MLP_NPAR,)
+        self.ctx._check(lib().hbtx_mlp_set_weights(self.ctx._h, self._h, _p(w)))
+
+    def gradient(self, X, y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.uint8)
+        g = np.zeros(MLP_NPAR, np.float32)
+        loss = C.c_double(0.0)
+        self.ctx._check(lib().hbtx_mlp_gradient(self.ctx._h, self._h, len(X), _p(X), _p(y), _p(g), C.byref(loss)))
+        return g, loss.value
+
+    def train(self, X, y, epochs=3, batch=4096, lr=2e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.uint8)
+        loss = C.c_double(0.0)
+        self.ctx._check(lib().hbtx_mlp_train(self.ctx._h, self._h, len(X), _p(X), _p(y), epochs, batch, lr, seed,
+                                             C.byref(loss)))
+        return loss.value
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros(len(X), np.float32)
+        self.ctx._check(lib().hbtx_mlp_predict(self.ctx._h, self._h, len(X), _p(X), _p(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().hbtx_mlp_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Tree:

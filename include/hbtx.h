@@ -157,6 +157,41 @@ int hbtx_direct_potential(hbtx_ctx *ctx, const hbtx_params *prm, int64_t n_src, 
                           const float *src_mass, int64_t n_tgt, const float *tgt_pos, const float *tgt_mass,
                           double *out_pot);
 
+/* ---------------------------------------------------------------------------
+ * Learned warm start (no counterpart in the reference; it only proposes the
+ * starting set: the proposal is verified by an exact potential computation and
+ * falls back to the plain iteration, src/subhalo_unbind.cpp:555-758).
+ * Per-particle classifier 8 -> 64 -> 64 -> 1 (ReLU, logistic output), FP32 master
+ * weights, BF16 weights in every forward/backward pass, FP32 accumulation.
+ * ------------------------------------------------------------------------- */
+#define HBTX_MLP_NIN 8
+#define HBTX_MLP_NPAR 4801 /* W1[64][8] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3 */
+typedef struct hbtx_mlp hbtx_mlp;
+int hbtx_mlp_create(hbtx_ctx *ctx, uint64_t seed, hbtx_mlp **mlp);
+void hbtx_mlp_free(hbtx_mlp *mlp);
+/* master FP32 weights and their BF16 working copy (either may be NULL) */
+int hbtx_mlp_get_weights(hbtx_ctx *ctx, hbtx_mlp *mlp, float *master, uint16_t *bf16);
+int hbtx_mlp_set_weights(hbtx_ctx *ctx, hbtx_mlp *mlp, const float *master);
+/* mean binary cross-entropy and its gradient over n samples X[n][8], labels y[n] */
+int hbtx_mlp_gradient(hbtx_ctx *ctx, hbtx_mlp *mlp, int64_t n, const float *X, const uint8_t *y, float *grad,
+                      double *loss);
+/* Adam training on the GPU, shuffled minibatches; *mean_loss = mean loss of the last epoch */
+int hbtx_mlp_train(hbtx_ctx *ctx, hbtx_mlp *mlp, int64_t n, const float *X, const uint8_t *y, int32_t epochs,
+                   int32_t batch, float lr, uint64_t seed, double *mean_loss);
+int hbtx_mlp_predict(hbtx_ctx *ctx, hbtx_mlp *mlp, int64_t n, const float *X, float *logit);
+/* Previous snapshot's outputs for the loaded batch (same slot = same particle):
+ * prev_rank int32[n] (slot in the previous final order, -1 if absent), prev_bound
+ * uint8[n], prev_energy float[n]; per subhalo prev_com_pos, prev_com_vel float[nsub][3],
+ * prev_mbound and prev_rscale float[nsub] (rms radius of the previous bound set). */
+int hbtx_batch_set_previous(hbtx_ctx *ctx, const int32_t *prev_rank, const uint8_t *prev_bound,
+                            const float *prev_energy, const float *prev_com_pos, const float *prev_com_vel,
+                            const float *prev_mbound, const float *prev_rscale);
+/* Per-particle features float[n][8] on the device (out_features may be NULL). */
+int hbtx_batch_features(hbtx_ctx *ctx, const hbtx_params *prm, float *out_features);
+/* Inference on the batch features: proposal = P(bound) > threshold, kept on the device
+ * for HBTX_WARMSTART; *elapsed_ms times the inference kernel. */
+int hbtx_batch_propose(hbtx_ctx *ctx, hbtx_mlp *mlp, float threshold, int64_t *nproposed, double *elapsed_ms);
+
 /* Segmented sort of uint64 keys with int32 payloads (used by the tree build and
  * the final binding-energy order; exported for tests). */
 int hbtx_segmented_sort(hbtx_ctx *ctx, int64_t n, uint64_t *keys, int32_t *vals, int64_t nseg,

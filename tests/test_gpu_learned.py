@@ -1,0 +1,181 @@
+"""Learned warm start on the GPU: classifier kernels vs the FP64 oracle, training,
+and warm-started unbinding vs the plain iteration and the CPU oracle.
+
+Gates: gradients within 2e-3 of the largest gradient component of the FP64 oracle
+evaluated at the same BF16-rounded weights (FP32 accumulation over <= 4096
+samples); warm-started bound sets within 0.1% of bound mass of the plain GPU
+unbinding and of the oracle, whatever the proposal; the trained network lowers
+the rounds and the walk work against the plain iteration and an untrained network.
+"""
+import numpy as np
+import pytest
+
+import hbtx
+import learned
+import mlp_oracle as M
+import pyoracle as O
+import synth
+from parity import bound_mass_agreement, energy_errors, order_is_reference_layout
+
+pytestmark = pytest.mark.gpu
+
+
+def random_net(ctx, seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    w = rng.normal(0, scale, M.NPAR).astype(np.float32)
+    net = hbtx.MLP(ctx, seed)
+    net.set_weights(w)
+    return net, w
+
+
+def test_mlp_weights_are_bf16_copies(ctx):
+    net, w = random_net(ctx, 1)
+    master, bf16 = net.weights()
+    assert np.array_equal(master, w)
+    assert np.array_equal(bf16.astype(np.uint32) << 16, M.bf16_round(w).view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [1, 100, 4096])
+def test_mlp_gradient_matches_oracle(ctx, n):
+    net, w = random_net(ctx, 2 + n)
+    rng = np.random.default_rng(n)
+    X = rng.normal(0, 1, (n, M.NIN)).astype(np.float32)
+    y = (rng.random(n) < 0.4).astype(np.uint8)
+    g, loss = net.gradient(X, y)
+    wr = M.bf16_round(w).astype(np.float64)
+    gref, lref = M.gradient(wr, X.astype(np.float64), y)
+    assert abs(loss - lref) <= 1e-5 * max(1.0, abs(lref))
+    scale = np.abs(gref).max()
+    err = np.abs(g - gref).max()
+    assert err <= 2e-3 * scale, (err, scale)
+    # every layer block agrees, not only the largest components
+    k = 0
+    for name, shape in M.LAYOUT:
+        size = int(np.prod(shape))
+        gb, rb = g[k:k + size], gref[k:k + size]
+        assert np.abs(gb - rb).max() <= 2e-3 * max(np.abs(rb).max(), 1e-6 * scale), name
+        k += size
+
+
+def test_mlp_predict_matches_oracle(ctx):
+    net, w = random_net(ctx, 5)
+    rng = np.random.default_rng(5)
+    X = rng.normal(0, 1, (3000, M.NIN)).astype(np.float32)
+    z = net.predict(X)
+    zref, _ = M.forward(M.bf16_round(w).astype(np.float64), X.astype(np.float64))
+    assert np.abs(z - zref).max() <= 1e-4 * max(1.0, np.abs(zref).max())
+
+
+def test_mlp_training_learns_a_separable_rule(ctx):
+    rng = np.random.default_rng(7)
+    X = rng.normal(0, 1, (200_000, M.NIN)).astype(np.float32)
+    y = ((X[:, 0] + 0.5 * X[:, 3] < 0.2) & (X[:, 1] < 1.5)).astype(np.uint8)
+    net = hbtx.MLP(ctx, 3)
+    l0 = net.gradient(X[:20000], y[:20000])[1]
+    loss = net.train(X, y, epochs=3, batch=4096, lr=3e-3, seed=1)
+    assert loss < 0.25 * l0, (loss, l0)
+    Xt = rng.normal(0, 1, (20000, M.NIN)).astype(np.float32)
+    yt = ((Xt[:, 0] + 0.5 * Xt[:, 3] < 0.2) & (Xt[:, 1] < 1.5))
+    acc = np.mean((net.predict(Xt) > 0) == yt)
+    assert acc > 0.97, acc
+
+
+@pytest.fixture(scope="module")
+def evolved():
+    pop = synth.population(seed=21, nsub=80, nmax=30000)
+    return synth.evolve(pop, nsnap=8, seed=5)
+
+
+@pytest.fixture(scope="module")
+def trained(ctx, evolved):
+    prm = hbtx.workload_params()
+    X, y, _ = learned.collect_training_set(ctx, evolved, prm, train_snaps=range(6))
+    net, loss = learned.train(ctx, X, y, epochs=4)
+    return net, loss
+
+
+def plain_and_prev(ctx, snaps, k, prm):
+    prev = learned.previous_state(snaps[k - 1], ctx.unbind(snaps[k - 1], prm))
+    return prev
+
+
+@pytest.mark.parametrize("k", [6, 7])
+def test_warm_start_matches_plain_and_oracle(ctx, evolved, trained, k):
+    net, _ = trained
+    prm = hbtx.workload_params()
+    pop = evolved[k]
+    prev = plain_and_prev(ctx, evolved, k, prm)
+    warm, _, _, npro = learned.warm_unbind(ctx, pop, prev, net, prm)
+    assert npro > 0
+    plain = ctx.unbind(pop, prm, hbtx.BATCHED | hbtx.SAVE_POTENTIAL)
+    frac, _ = bound_mass_agreement(warm, plain, pop)
+    assert frac <= 1e-3, frac
+    ref = O.unbind_batch(pop, O.default_params(), want_potential=True)
+    frac_o, _ = bound_mass_agreement(warm, ref, pop)
+    assert frac_o <= 1e-3, frac_o
+    err = energy_errors(warm, ref, pop, ref["potential"])
+    assert np.quantile(err, 0.999) < 1e-4
+    assert order_is_reference_layout(warm, pop)
+
+
+@pytest.mark.parametrize("kind", ["all", "none", "random", "inverted"])
+def test_any_proposal_is_verified(ctx, evolved, kind):
+    # the proposal only decides where the iteration starts: a bad one costs time, not accuracy
+    prm = hbtx.workload_params()
+    pop = evolved[6]
+    plain = ctx.unbind(pop, prm)
+    bound = learned.labels(pop, plain)
+    rng = np.random.default_rng(3)
+    prop = {"all": np.ones(pop.n, np.uint8), "none": np.zeros(pop.n, np.uint8),
+            "random": (rng.random(pop.n) < 0.5).astype(np.uint8), "inverted": 1 - bound}[kind]
+    ctx.load(pop)
+    ctx.set_proposal(prop)
+    ctx.run(prm, hbtx.WARMSTART)
+    warm = ctx.fetch()
+    frac, _ = bound_mass_agreement(warm, plain, pop)
+    assert frac <= 1e-3, frac
+    assert np.array_equal(warm["result"]["disrupted"], plain["result"]["disrupted"])
+
+
+def test_exact_proposal_reproduces_plain_exactly(ctx, evolved):
+    # proposing the plain result itself: the warm round's bound set is that set, converged
+    prm = hbtx.workload_params()
+    pop = evolved[7]
+    plain = ctx.unbind(pop, prm)
+    ctx.load(pop)
+    ctx.set_proposal(learned.labels(pop, plain))
+    ctx.run(prm, hbtx.WARMSTART | hbtx.COUNT)
+    st = ctx.stats()
+    warm = ctx.fetch()
+    frac, _ = bound_mass_agreement(warm, plain, pop)
+    assert frac <= 1e-3, frac
+    assert st["rounds"] <= 3
+
+
+def work(ctx, pop, prm, flags):
+    ctx.run(prm, flags | hbtx.COUNT)
+    st = ctx.stats()
+    return st["rounds"], st["tree_lane_visits"] + st["direct_lane_pairs"]
+
+
+def test_trained_network_reduces_work(ctx, evolved, trained):
+    """Ablation: warm start from the trained network vs the plain iteration and vs an
+    untrained (randomly initialised) network on the held-out snapshot 6."""
+    net, _ = trained
+    prm = hbtx.workload_params()
+    pop = evolved[6]
+    prev = plain_and_prev(ctx, evolved, 6, prm)
+    ctx.load(pop)
+    r_plain, w_plain = work(ctx, pop, prm, hbtx.BATCHED)
+    ctx.set_previous(prev)
+    ctx.features(prm, fetch=False)
+    ctx.propose(net, 0.3)
+    r_warm, w_warm = work(ctx, pop, prm, hbtx.WARMSTART)
+    untrained = hbtx.MLP(ctx, 123)
+    ctx.propose(untrained, 0.3)
+    r_un, w_un = work(ctx, pop, prm, hbtx.WARMSTART)
+    print(f"rounds plain {r_plain} warm {r_warm} untrained {r_un}; work plain {w_plain:.3e} warm {w_warm:.3e} "
+          f"untrained {w_un:.3e}")
+    assert r_warm < r_plain
+    assert w_warm < w_plain
+    assert w_warm < w_un
