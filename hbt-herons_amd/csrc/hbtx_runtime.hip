@@ -640,7 +640,9 @@ void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int rou
   const int64_t n = V.n;
   const int gb = blocks_for(n);
   const int gs = blocks_for(V.nsub, 256, 1 << 20);
-  k_partition_flags<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, B.flag64.p);
+  if (warm_round)
+    k_fill_i32<<<gs, 256, 0, st>>>(B.aux32.p, V.nsub, 0);
+  k_partition_flags<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, B.flag64.p, warm_round ? B.aux32.p : nullptr);
   scan64(c, B.flag64.p, B.fscan64.p, n);
   k_partition_scatter<<<gb, 256, 0, st>>>(V.S, V.Q, n, round, B.fscan64.p);
   k_copy_cur<<<gb, 256, 0, st>>>(V.S, V.Q, n, B.fscan64.p);
@@ -649,11 +651,6 @@ void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int rou
     k_fill_i32<<<gs, 256, 0, st>>>(B.aux32.p, V.nsub, 0);
     k_count_unsampled<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, B.aux32.p);
     k_set_unsampled<<<gs, 256, 0, st>>>(V.S, B.aux32.p);
-  }
-  if (warm_round)
-  {
-    k_fill_i32<<<gs, 256, 0, st>>>(B.aux32.p, V.nsub, 0);
-    k_warm_count<<<gb, 256, 0, st>>>(V.S, V.Q, n, B.aux32.p);
   }
   k_round_decide<<<gs, 256, 0, st>>>(V.S, P, B.fscan64.p, warm_round ? B.aux32.p : nullptr);
   k_com_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p);
@@ -1820,6 +1817,19 @@ int hbtx_batch_features(hbtx_ctx *c, const hbtx_params *prm, float *out_features
       HIPCHK(hipMemcpyAsync(out_features, B.feat.p, n * MLP_IN * sizeof(float), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
     }
+  });
+}
+
+int hbtx_batch_get_proposal(hbtx_ctx *c, uint8_t *proposal)
+{
+  if (!c || !proposal)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    Batch &B = c->B;
+    REQUIRE(B.n > 0 && B.has_proposal, HBTX_E_INVALID, "no proposal on the device");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(proposal, B.proposal.p, B.n, hipMemcpyDeviceToHost));
   });
 }
 

@@ -83,9 +83,11 @@ __global__ void k_init_mass_finish(SubArrays S, DevParams P, const double *red, 
   const double *v = red + (int64_t)RED_NV * s;
   S.mbound[s] = (float)v[0];
   S.minit[s] = (float)v[0];
-  if (warm_start && n > 0 && (int64_t)v[1] >= P.min_part)
+  if (warm_start && n > 0 && (int64_t)v[1] >= 2 * P.min_part)
   {
-    // warm round in the proposed set's own centre-of-mass frame
+    // warm round in the proposed set's own centre-of-mass frame; subhaloes whose
+    // proposal is near the disruption limit (MinNumPartOfSub) iterate plainly, since
+    // whether they survive depends on the iteration's path
     S.warm[s] = 1;
     S.npro[s] = (int)v[1];
     if (!P.periodic)

@@ -458,7 +458,10 @@ __global__ void k_items_emit(SubArrays S, const int32_t *order, const int32_t *i
 }
 
 // Partition flags over slots: low 32 bits = bound, high 32 bits = bound tracer.
-__global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, int64_t n, int64_t *flag)
+// In a warm round only proposed particles may stay; bound particles outside the
+// proposal are removed with the unbound ones and counted in warm_np[s].
+__global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, int64_t n, int64_t *flag,
+                                  int32_t *warm_np)
 {
   GRID_STRIDE(q, n)
   {
@@ -469,7 +472,12 @@ __global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, int64_
     {
       const int p = Q.cur[q];
       if (Q.energy[p] < 0.f)
-        f = 1 | ((int64_t)(is_tracer(P, ptype_of(Q, p)) ? 1 : 0) << 32);
+      {
+        if (warm_np && S.warm[s] == 1 && !Q.proposal[p])
+          atomicAdd(warm_np + s, 1);
+        else
+          f = 1 | ((int64_t)(is_tracer(P, ptype_of(Q, p)) ? 1 : 0) << 32);
+      }
     }
     flag[q] = f;
   }
@@ -529,21 +537,27 @@ __global__ void k_round_decide(SubArrays S, DevParams P, const int64_t *fscan, c
   S.bnd_tr[s] = ntr;
   if (S.warm[s] == 1)
   {
-    // Proposal verification.  Accepted when no particle outside the proposed set
-    // is bound to it and at least half of it is bound (then the correction
-    // sequence continues from the proposed set); a few (<= 0.1%) bound outsiders
-    // get one exact round from the bound set; anything else, including too few
-    // bound particles, restarts the plain iteration from every particle.
+    // Proposal verification: the proposed set's exact potential is evaluated at
+    // every particle.  Accepted when at most 0.1% as many particles outside the
+    // proposal as inside it come out bound (those are removed, as the plain
+    // iteration removes the particles it finds unbound on its way), at least half
+    // of the proposal is bound and the subhalo is well above the disruption limit
+    // (twice MinNumPartOfSub and MinNumTracerPartOfSub); the correction sequence
+    // then continues from the proposed set.  Otherwise the plain iteration restarts
+    // from every particle.
     const int np = warm_np ? warm_np[s] : 0;
     const int n = S.off[s + 1] - S.off[s];
     const int npro = S.npro[s];
     S.warm[s] = 2;
-    const bool viable =
-      (int64_t)nb >= P.min_part && (int64_t)ntr >= P.min_tracer && 2 * (int64_t)nb >= (int64_t)npro;
-    if (viable && np == 0)
+    const bool accept = (int64_t)nb >= 2 * P.min_part && (int64_t)ntr >= 2 * P.min_tracer &&
+                        2 * (int64_t)nb >= (int64_t)npro && (int64_t)np * 1000 <= (int64_t)nb;
+    if (accept)
     {
+      // the warm round's frame is the proposal's: at least one more round follows so
+      // that the final energies are taken in the bound set's frame
       const int nrem = npro - nb; // proposed but unbound: the next correction's sources
       S.nrem[s] = nrem;
+      S.nlast[s] = n;
       if (nrem < nb)
       {
         S.corr[s] = 1;
@@ -551,8 +565,6 @@ __global__ void k_round_decide(SubArrays S, DevParams P, const int64_t *fscan, c
           S.oldref[6 * s + k] = S.ref[6 * s + k];
       }
     }
-    else if (viable && (int64_t)np * 1000 <= (int64_t)nb)
-      S.nlast[s] = n;
     else
     {
       S.reset[s] = 1;
@@ -577,17 +589,6 @@ __global__ void k_round_decide(SubArrays S, DevParams P, const int64_t *fscan, c
       for (int k = 0; k < 6; k++)
         S.oldref[6 * s + k] = S.ref[6 * s + k];
     }
-}
-
-// Warm round: bound particles that were not proposed.
-__global__ void k_warm_count(SubArrays S, PartArrays Q, int64_t n, int32_t *cnt)
-{
-  GRID_STRIDE(p, n)
-  {
-    const int s = S.sub_of[p];
-    if (S.mode[s] == MODE_FULL && S.warm[s] == 1 && Q.state[p] == 0 && !Q.proposal[p])
-      atomicAdd(cnt + s, 1);
-  }
 }
 
 // Rejected proposals: back to the starting state of a plain unbinding.

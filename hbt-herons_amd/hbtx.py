@@ -151,6 +151,7 @@ def lib():
         L.hbtx_batch_set_previous.argtypes = [P, P, P, P, P, P, P, P]
         L.hbtx_batch_features.argtypes = [P, C.POINTER(Params), P]
         L.hbtx_batch_propose.argtypes = [P, P, C.c_float, C.POINTER(i64), C.POINTER(C.c_double)]
+        L.hbtx_batch_get_proposal.argtypes = [P, P]
         _lib = L
     return _lib
 
@@ -261,6 +262,11 @@ class Context:
         self._check(lib().hbtx_batch_propose(self._h, mlp._h, threshold, C.byref(npro) if count else None,
                                              C.byref(ms)))
         return (npro.value if count else None), ms.value
+
+    def get_proposal(self):
+        out = np.zeros(self._n, np.uint8)
+        self._check(lib().hbtx_batch_get_proposal(self._h, _p(out)))
+        return out
 
     # --- GravityTree_t API -----------------------------------------------------
     def tree(self, pos, mass, params=None):

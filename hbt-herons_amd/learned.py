@@ -96,7 +96,7 @@ def train(ctx, X, y, seed=0, epochs=4, batch=8192, lr=3e-3):
     return net, loss
 
 
-def warm_unbind(ctx, pop, prev, net, params, threshold=0.3, flags=hbtx.BATCHED):
+def warm_unbind(ctx, pop, prev, net, params, threshold=0.1, flags=hbtx.BATCHED):
     """Features -> proposal -> verified warm-started unbinding of one snapshot.
     Returns (output, run milliseconds, inference milliseconds, proposed count)."""
     ctx.load(pop)

@@ -191,6 +191,8 @@ int hbtx_batch_features(hbtx_ctx *ctx, const hbtx_params *prm, float *out_featur
 /* Inference on the batch features: proposal = P(bound) > threshold, kept on the device
  * for HBTX_WARMSTART; *elapsed_ms times the inference kernel. */
 int hbtx_batch_propose(hbtx_ctx *ctx, hbtx_mlp *mlp, float threshold, int64_t *nproposed, double *elapsed_ms);
+/* Copy of the current proposal (uint8[n]). */
+int hbtx_batch_get_proposal(hbtx_ctx *ctx, uint8_t *proposal);
 
 /* Segmented sort of uint64 keys with int32 payloads (used by the tree build and
  * the final binding-energy order; exported for tests). */

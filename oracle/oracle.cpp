@@ -652,8 +652,14 @@ void core_phase_space(const Cfg &c, int64_t nbound, const Particle *P, const xyz
   sv = std::sqrt(vel2[0] + vel2[1] + vel2[2]);
 }
 
-// Subhalo_t::Unbind, src/subhalo_unbind.cpp:463-808
-void unbind(const Cfg &c, SubState &S, std::vector<int64_t> &final_order, bool want_pot)
+// Subhalo_t::Unbind, src/subhalo_unbind.cpp:463-808.
+// prop (may be null): warm-start proposal, one flag per particle.  The warm start
+// has no reference counterpart; it is restated here with the library's rules (a
+// first round with the proposed set as sources and every particle as target in
+// the proposal's frame, acceptance test, else the plain iteration from every
+// particle) so that the GPU path can be checked round for round.
+void unbind(const Cfg &c, SubState &S, std::vector<int64_t> &final_order, bool want_pot,
+            const uint8_t *prop = nullptr)
 {
   std::vector<Particle> &P = S.P;
   const int64_t N = (int64_t)P.size();
@@ -702,7 +708,117 @@ void unbind(const Cfg &c, SubState &S, std::vector<int64_t> &final_order, bool w
   bool correction = false;
   int iters = 0;
   bool disrupted = false;
-  while (true)
+  // convergence (:687-758): bound part sorted, centre refinement, final permutation
+  auto finalize = [&]() {
+    std::sort(E.begin(), E.begin() + nbound, [](const PE &a, const PE &b) { return a.E < b.E; });
+    if (refine && nbound > mss)
+    {
+      // RefineBindingEnergyOrder (:395-425)
+      int64_t size = std::max(mss, (int64_t)(c.p.bound_fraction_center_refinement * nbound));
+      tree.build(c, es, size);
+      std::vector<PE> ein(size);
+      for (int64_t i = 0; i < size; i++)
+      {
+        const Particle &q = P[E[i].idx];
+        ein[i].idx = i;
+        ein[i].E = tree.binding_energy(q.x, q.v, ref_pos, ref_vel, q.m);
+      }
+      std::sort(ein.begin(), ein.end(), [](const PE &a, const PE &b) { return a.E < b.E; });
+      for (int64_t i = 0; i < size; i++)
+        ein[i] = E[ein[i].idx];
+      for (int64_t i = 0; i < size; i++)
+        E[i] = ein[i];
+    }
+    std::vector<Particle> p2(N);
+    for (int64_t i = 0; i < N; i++)
+    {
+      final_order[i] = E[i].idx;
+      p2[i] = P[E[i].idx];
+      E[i].idx = i;
+    }
+    P.swap(p2);
+    S.mb_pos = P[0].x;
+    S.mb_vel = P[0].v;
+  };
+  // warm start: eligible when the proposal holds at least 2 MinNumPartOfSub particles
+  std::vector<PE> warm_src; // the correction sources after an accepted warm round
+  bool warm_pending = false;
+  bool done = false;
+  if (prop && mss == 0)
+  {
+    int64_t npro = 0;
+    for (int64_t i = 0; i < N; i++)
+      npro += prop[i] ? 1 : 0;
+    warm_pending = npro >= 2 * c.p.min_num_part;
+  }
+  if (warm_pending)
+  {
+    iters++;
+    const float mtotal = mbound;
+    const xyz in_pos = ref_pos, in_vel = ref_vel;
+    std::vector<PE> EP;
+    for (int64_t i = 0; i < N; i++)
+      if (prop[i])
+        EP.push_back(PE{0.f, i});
+    const int64_t npro = (int64_t)EP.size();
+    ESnap esp(EP.data(), npro, P, c);
+    esp.com_frame(ref_pos, ref_vel, npro);
+    tree.build(c, esp, npro);
+    for (int64_t i = 0; i < N; i++)
+    {
+      const Particle &q = P[E[i].idx];
+      E[i].E = tree.binding_energy(q.x, q.v, ref_pos, ref_vel, prop[E[i].idx] ? q.m : 0.f);
+      if (c.p.thermal)
+        E[i].E += q.u;
+    }
+    int64_t np_out = 0;
+    for (int64_t i = 0; i < N; i++)
+      if (E[i].E < 0 && !prop[E[i].idx])
+        np_out++;
+    auto it = std::stable_partition(E.begin(), E.end(), [&](const PE &a) { return a.E < 0 && prop[a.idx]; });
+    const int64_t nb = it - E.begin();
+    int64_t ntr = 0;
+    for (int64_t i = 0; i < nb; i++)
+      ntr += (c.p.dm_only || is_tracer(c, P[E[i].idx])) ? 1 : 0;
+    const bool accept = nb >= 2 * c.p.min_num_part && ntr >= 2 * c.p.min_num_tracer && 2 * nb >= npro &&
+                        np_out * 1000 <= nb;
+    if (accept)
+    {
+      std::sort(E.begin() + nb, E.end(), [](const PE &a, const PE &b) { return a.E < b.E; });
+      for (int64_t i = nb; i < N; i++)
+        if (prop[E[i].idx])
+          warm_src.push_back(E[i]);
+      const int64_t nrem = npro - nb;
+      nbound = nb;
+      nlast = N;
+      if (nrem < nb)
+      {
+        correction = true;
+        old_pos = ref_pos;
+        old_vel = ref_vel;
+      }
+      mbound = es.com_frame(ref_pos, ref_vel, nbound);
+      // convergence against every particle: the warm round ends the iteration only
+      // when nearly all particles are bound
+      if (nbound >= nlast * prec)
+      {
+        finalize();
+        done = true;
+      }
+    }
+    else
+    {
+      for (int64_t i = 0; i < N; i++)
+        E[i] = PE{0.f, i};
+      nbound = N;
+      mbound = mtotal;
+      ref_pos = in_pos;
+      ref_vel = in_vel;
+      warm_pending = false;
+    }
+  }
+  bool after_warm = warm_pending; // the next correction's sources are the proposed, removed particles
+  while (!done)
   {
     iters++;
     if (correction)
@@ -710,14 +826,16 @@ void unbind(const Cfg &c, SubState &S, std::vector<int64_t> &final_order, bool w
       xyz dV;
       relative_velocity(c, old_pos, old_vel, ref_pos, ref_vel, dV);
       float dK = 0.5 * vecdot(dV, dV);
-      ESnap esc(&E[nbound], nlast - nbound, P, c);
-      tree.build(c, esc, nlast - nbound);
+      const int64_t nsrc = after_warm ? (int64_t)warm_src.size() : nlast - nbound;
+      ESnap esc(after_warm ? warm_src.data() : &E[nbound], nsrc, P, c);
+      if (nsrc > 0) // (a plain correction always has sources; after a warm round there may be none)
+        tree.build(c, esc, nsrc);
       for (int64_t i = 0; i < nbound; i++)
       {
         const Particle &q = P[E[i].idx];
         xyz ov;
         relative_velocity(c, q.x, q.v, old_pos, old_vel, ov);
-        E[i].E += vecdot(ov, dV) + dK - tree.potential(q.x, 0.f);
+        E[i].E += vecdot(ov, dV) + dK - (nsrc > 0 ? tree.potential(q.x, 0.f) : 0.0);
       }
       nlast = nbound;
     }
@@ -737,6 +855,7 @@ void unbind(const Cfg &c, SubState &S, std::vector<int64_t> &final_order, bool w
       }
       es.reset();
     }
+    after_warm = false;
     // RemoveUnboundParticles (:35-44)
     auto it = std::stable_partition(E.begin(), E.begin() + nlast, [](const PE &a) { return a.E < 0; });
     nbound = it - E.begin();
@@ -786,35 +905,7 @@ void unbind(const Cfg &c, SubState &S, std::vector<int64_t> &final_order, bool w
     mbound = es.com_frame(ref_pos, ref_vel, nbound);
     if (nbound >= nlast * prec)
     {
-      std::sort(E.begin(), E.begin() + nbound, [](const PE &a, const PE &b) { return a.E < b.E; });
-      if (refine && nbound > mss)
-      {
-        // RefineBindingEnergyOrder (:395-425)
-        int64_t size = std::max(mss, (int64_t)(c.p.bound_fraction_center_refinement * nbound));
-        tree.build(c, es, size);
-        std::vector<PE> ein(size);
-        for (int64_t i = 0; i < size; i++)
-        {
-          const Particle &q = P[E[i].idx];
-          ein[i].idx = i;
-          ein[i].E = tree.binding_energy(q.x, q.v, ref_pos, ref_vel, q.m);
-        }
-        std::sort(ein.begin(), ein.end(), [](const PE &a, const PE &b) { return a.E < b.E; });
-        for (int64_t i = 0; i < size; i++)
-          ein[i] = E[ein[i].idx];
-        for (int64_t i = 0; i < size; i++)
-          E[i] = ein[i];
-      }
-      std::vector<Particle> p2(N);
-      for (int64_t i = 0; i < N; i++)
-      {
-        final_order[i] = E[i].idx;
-        p2[i] = P[E[i].idx];
-        E[i].idx = i;
-      }
-      P.swap(p2);
-      S.mb_pos = P[0].x;
-      S.mb_vel = P[0].v;
+      finalize();
       break;
     }
   }
@@ -874,12 +965,11 @@ double kernel_fp64(double r2, double m, double h, double h_inv)
 }
 } // namespace
 
-extern "C" double oracle_unbind_batch(const ounb_params *prm, int64_t n, const float *pos, const float *vel,
-                                      const float *mass, const float *u, const int32_t *type, const int64_t *id,
-                                      int64_t nsub, const int64_t *offsets, const float *ref_pos,
-                                      const float *ref_vel, const float *mb_pos, const float *mb_vel, int nthreads,
-                                      int64_t *out_order, float *out_energy, float *out_potential,
-                                      ounb_result *out_res)
+static double unbind_batch_impl(const ounb_params *prm, int64_t n, const float *pos, const float *vel,
+                                const float *mass, const float *u, const int32_t *type, const int64_t *id,
+                                int64_t nsub, const int64_t *offsets, const float *ref_pos, const float *ref_vel,
+                                const float *mb_pos, const float *mb_vel, const uint8_t *proposal, int nthreads,
+                                int64_t *out_order, float *out_energy, float *out_potential, ounb_result *out_res)
 {
   Cfg c(*prm);
   std::vector<SubState> subs(nsub);
@@ -915,7 +1005,7 @@ extern "C" double oracle_unbind_batch(const ounb_params *prm, int64_t n, const f
   double t0 = omp_get_wtime();
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
   for (int64_t s = 0; s < nsub; s++)
-    unbind(c, subs[s], orders[s], out_potential != nullptr);
+    unbind(c, subs[s], orders[s], out_potential != nullptr, proposal ? proposal + offsets[s] : nullptr);
   double t1 = omp_get_wtime();
   for (int64_t s = 0; s < nsub; s++)
   {
@@ -965,6 +1055,29 @@ extern "C" double oracle_unbind_batch(const ounb_params *prm, int64_t n, const f
     R.disrupted = S.disrupted;
   }
   return t1 - t0;
+}
+
+extern "C" double oracle_unbind_batch(const ounb_params *prm, int64_t n, const float *pos, const float *vel,
+                                      const float *mass, const float *u, const int32_t *type, const int64_t *id,
+                                      int64_t nsub, const int64_t *offsets, const float *ref_pos,
+                                      const float *ref_vel, const float *mb_pos, const float *mb_vel, int nthreads,
+                                      int64_t *out_order, float *out_energy, float *out_potential,
+                                      ounb_result *out_res)
+{
+  return unbind_batch_impl(prm, n, pos, vel, mass, u, type, id, nsub, offsets, ref_pos, ref_vel, mb_pos, mb_vel,
+                           nullptr, nthreads, out_order, out_energy, out_potential, out_res);
+}
+
+extern "C" double oracle_unbind_batch_warm(const ounb_params *prm, int64_t n, const float *pos, const float *vel,
+                                           const float *mass, const float *u, const int32_t *type,
+                                           const int64_t *id, int64_t nsub, const int64_t *offsets,
+                                           const float *ref_pos, const float *ref_vel, const float *mb_pos,
+                                           const float *mb_vel, const uint8_t *proposal, int nthreads,
+                                           int64_t *out_order, float *out_energy, float *out_potential,
+                                           ounb_result *out_res)
+{
+  return unbind_batch_impl(prm, n, pos, vel, mass, u, type, id, nsub, offsets, ref_pos, ref_vel, mb_pos, mb_vel,
+                           proposal, nthreads, out_order, out_energy, out_potential, out_res);
 }
 
 extern "C" int64_t oracle_tree_potential(const ounb_params *prm, int64_t n_src, const float *src_pos,

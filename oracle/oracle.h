@@ -81,6 +81,14 @@ double oracle_unbind_batch(const ounb_params *prm, int64_t n, const float *pos, 
                            const float *mb_pos, const float *mb_vel, int nthreads, int64_t *out_order,
                            float *out_energy, float *out_potential, ounb_result *out_res);
 
+/* The same with the library's learned warm start restated (no reference
+ * counterpart): proposal uint8[n], 1 = proposed bound.  Requires max_sample_size 0. */
+double oracle_unbind_batch_warm(const ounb_params *prm, int64_t n, const float *pos, const float *vel,
+                                const float *mass, const float *u, const int32_t *type, const int64_t *id,
+                                int64_t nsub, const int64_t *offsets, const float *ref_pos, const float *ref_vel,
+                                const float *mb_pos, const float *mb_vel, const uint8_t *proposal, int nthreads,
+                                int64_t *out_order, float *out_energy, float *out_potential, ounb_result *out_res);
+
 /* GravityTree_t::Build over n_src sources followed by EvaluatePotential at every
  * target (targetMass = tgt_mass[i], or 0 when tgt_mass is NULL).  Double result.
  * Returns the number of tree cells. */

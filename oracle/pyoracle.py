@@ -107,6 +107,9 @@ def lib():
         L.oracle_unbind_batch.restype = C.c_double
         L.oracle_unbind_batch.argtypes = [C.POINTER(Params), C.c_int64, P, P, P, P, P, P, C.c_int64, P, P, P,
                                           P, P, C.c_int, P, P, P, P]
+        L.oracle_unbind_batch_warm.restype = C.c_double
+        L.oracle_unbind_batch_warm.argtypes = [C.POINTER(Params), C.c_int64, P, P, P, P, P, P, C.c_int64, P, P,
+                                               P, P, P, P, C.c_int, P, P, P, P]
         L.oracle_tree_potential.restype = C.c_int64
         L.oracle_tree_potential.argtypes = [C.POINTER(Params), C.c_int64, P, P, C.c_int64, P, P, C.c_int, P]
         L.oracle_direct_potential.restype = None
@@ -127,8 +130,9 @@ def _f32(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
 
 
-def unbind_batch(pop, params=None, nthreads=0, want_potential=False, mb_pos=None, mb_vel=None):
-    """Run the oracle unbinding over a synth.Population. Returns dict of outputs."""
+def unbind_batch(pop, params=None, nthreads=0, want_potential=False, mb_pos=None, mb_vel=None, proposal=None):
+    """Run the oracle unbinding over a synth.Population. Returns dict of outputs.
+    proposal (uint8 per particle) runs the warm-start restatement."""
     prm = params if isinstance(params, Params) else make_params(**(params or {}))
     n = pop.n
     order = np.zeros(n, np.int64)
@@ -136,10 +140,15 @@ def unbind_batch(pop, params=None, nthreads=0, want_potential=False, mb_pos=None
     pot = np.zeros(n, np.float32) if want_potential else None
     res = np.zeros(pop.nsub, RESULT_DTYPE)
     ptype = None if pop.ptype is None else np.ascontiguousarray(pop.ptype, dtype=np.int32)
-    secs = lib().oracle_unbind_batch(C.byref(prm), n, _p(pop.pos), _p(pop.vel), _p(pop.mass), _p(pop.u),
-                                     _p(ptype), _p(pop.pid), pop.nsub, _p(pop.offsets), _p(pop.ref_pos),
-                                     _p(pop.ref_vel), _p(_f32(mb_pos)), _p(_f32(mb_vel)), nthreads, _p(order),
-                                     _p(energy), _p(pot), _p(res))
+    common = (C.byref(prm), n, _p(pop.pos), _p(pop.vel), _p(pop.mass), _p(pop.u), _p(ptype), _p(pop.pid),
+              pop.nsub, _p(pop.offsets), _p(pop.ref_pos), _p(pop.ref_vel), _p(_f32(mb_pos)), _p(_f32(mb_vel)))
+    outs = (_p(order), _p(energy), _p(pot), _p(res))
+    if proposal is None:
+        secs = lib().oracle_unbind_batch(*common, nthreads, *outs)
+    else:
+        pr = np.ascontiguousarray(proposal, dtype=np.uint8)
+        assert len(pr) == n
+        secs = lib().oracle_unbind_batch_warm(*common, _p(pr), nthreads, *outs)
     return dict(order=order, energy=energy, potential=pot, result=res, seconds=secs)
 
 
