@@ -1,20 +1,26 @@
-"""Benchmark: particles unbound per second on one MI355X.
+"""Benchmark: particles unbound per second on one MI355X, batched + learned.
 
 Workload (default): a power-law subhalo population drawn until it holds
 5x10^7 particles (dN/dN_p ~ N_p^-1.9 on [20, 10^6], log-normal concentrations,
-10% interlopers), theta=0.7, eps=1 kpc/h, BoundMassPrecision 0.995, no
-subsampling.  One step = one complete unbinding of the whole population
-(every subhalo iterated to convergence, final ordering and summary outputs),
-with inputs resident in HBM.
+10% interlopers), evolved through 8 ID-preserving snapshots (tidal stripping by
+1.5 v_esc kicks outside r_200 * 0.9^k, 2% jitter).  The 8-64-64-1 classifier is
+trained on the GPU on the plain unbinding's bound flags of snapshots 0-5; the
+timed step is the unbinding of snapshot 6 with the learned warm start:
+per-particle features (from snapshot 5's outputs) + inference + the verified,
+warm-started batched unbinding (every subhalo to convergence, final ordering
+and summary outputs), inputs resident in HBM.  theta=0.7, eps=1 kpc/h,
+BoundMassPrecision 0.995, no subsampling.
 
 Reported beside the headline value:
+  * the same batched unbinding without the warm start, whose result (identical
+    to the plain iteration's) judges the learned one: bound-mass mismatch;
   * the plain GPU loop (one subhalo at a time, host-driven iterations: the
     drop-in Subhalo_t::Unbind called per subhalo) timed on a stratified sample
     (every subhalo above 10^4 particles plus a random share of the rest) and
     extrapolated by sampling weight;
   * the CPU oracle (the reference algorithm restated in C++, OpenMP over
-    subhaloes like RefineParticles) on a bounded random sample;
-  * the roofline of the dominant kernel (tree walk / direct sum).
+    subhaloes like RefineParticles) on a bounded random sample of snapshot 6;
+  * the roofline of the dominant kernel (the tree walk / direct sum).
 """
 import argparse
 import json
@@ -41,6 +47,46 @@ def walk_flops(st):
     return 10.0 * st["tree_lane_visits"] + 6.0 * st["tree_lane_accepts"] + 14.0 * st["direct_lane_pairs"]
 
 
+def timed_steps(ctx, steps, warmup, step):
+    for _ in range(warmup):
+        step()
+    ctx.sync()
+    times = []
+    for k in range(steps):
+        ctx.sync()
+        w0 = time.perf_counter()
+        step()
+        ctx.sync()
+        times.append((time.perf_counter() - w0) * 1e3)
+    return times
+
+
+def plain_gpu_estimate(hbtx, pop, prm, share):
+    counts = pop.counts()
+    rng = np.random.default_rng(123)
+    big = np.where(counts > 10000)[0]
+    small = np.where(counts <= 10000)[0]
+    pick = rng.choice(small, max(1, int(share * len(small))), replace=False) if len(small) else []
+    sample = np.concatenate([big, np.sort(pick)]).astype(np.int64)
+    weight_small = len(small) / max(1, len(pick))
+    c2 = hbtx.Context(0)
+    c2.load(pop.subset(sample))
+    c2.run(prm, hbtx.PLAIN)  # warm-up
+    w0 = time.perf_counter()
+    c2.run(prm, hbtx.PLAIN)
+    c2.sync()
+    t_all = time.perf_counter() - w0
+    c2.load(pop.subset(big))
+    w0 = time.perf_counter()
+    c2.run(prm, hbtx.PLAIN)
+    c2.sync()
+    t_big = time.perf_counter() - w0
+    c2.close()
+    est = t_big + max(0.0, t_all - t_big) * weight_small
+    return est, (f"all {len(big)} subhaloes with >1e4 particles plus {len(pick)} of {len(small)} smaller "
+                 f"ones (weight {weight_small:.1f})")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,6 +94,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--particles", type=float, default=5e7)
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--snapshot", type=int, default=6, help="evaluated snapshot (trained on 0..5)")
+    ap.add_argument("--threshold", type=float, default=0.1, help="proposal: P(bound) > threshold")
+    ap.add_argument("--train-per-snapshot", type=int, default=2_000_000)
+    ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--direct-cutoff", type=int, default=-1,
                     help="sources <= cutoff use the exact sum; -1 = tuned value, 0 = tree everywhere")
     ap.add_argument("--plain-sample", type=float, default=0.01,
@@ -55,45 +105,114 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--state", default=None,
+                    help="npz cache of the trained weights and the previous snapshot's outputs: written "
+                         "after training, read instead of labelling and training when it exists")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="time only the learned step (for kernel profiles); implies --no-plain --no-cpu")
     args = ap.parse_args()
+    if args.learned_only:
+        args.no_plain = args.no_cpu = True
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is described "
-                                   "in DESIGN.md but not built"}))
+        print(json.dumps({"error": "this build runs on exactly one GPU; how the path would shard over "
+                                   "several is described in DESIGN.md, not built"}))
         sys.exit(2)
 
     import hbtx
+    import learned
     import synth
 
     t0 = time.time()
-    pop = synth.population(seed=args.seed, target_particles=int(args.particles))
-    log(f"generated {pop.n} particles in {pop.nsub} subhaloes in {time.time() - t0:.1f}s")
+    pop0 = synth.population(seed=args.seed, target_particles=int(args.particles))
+    log(f"generated {pop0.n} particles in {pop0.nsub} subhaloes in {time.time() - t0:.1f}s")
     prm = hbtx.workload_params(direct_cutoff=args.direct_cutoff)
-
     ctx = hbtx.Context(0)
+
+    # --- labels (plain unbinding) of snapshots 0..5 and the training set --------------
+    t_lab = t_train = 0.0
+    loss = float("nan")
+    if args.state and os.path.exists(args.state):
+        pop = None
+        for k, snap in enumerate(synth.evolve_iter(pop0, nsnap=args.snapshot + 1, seed=4)):
+            pop = snap
+        z = np.load(args.state)
+        prev = {k: z[k] for k in ("rank", "bound", "energy", "com_pos", "com_vel", "mbound", "rscale")}
+        net = hbtx.MLP(ctx, 0)
+        net.set_weights(z["weights"])
+        log(f"loaded trained weights and snapshot {args.snapshot - 1} outputs from {args.state}")
+    else:
+        rng = np.random.default_rng(0)
+        Xs, ys = [], []
+        prev = None
+        pop = None
+        t_lab = time.time()
+        for k, snap in enumerate(synth.evolve_iter(pop0, nsnap=args.snapshot + 1, seed=4)):
+            pop = snap
+            if k == args.snapshot:
+                break
+            if prev is not None:
+                X = learned.snapshot_features(ctx, pop, prev, prm)
+            out = ctx.unbind(pop, prm, hbtx.BATCHED)
+            if prev is not None:
+                take = rng.choice(pop.n, size=min(args.train_per_snapshot, pop.n), replace=False)
+                Xs.append(X[take])
+                ys.append(learned.labels(pop, out)[take])
+                del X
+            prev = learned.previous_state(pop, out)
+            log(f"snapshot {k}: plain labels, {int(out['result']['nbound'].sum())} bound")
+        t_lab = time.time() - t_lab
+        X, y = np.concatenate(Xs), np.concatenate(ys)
+        del Xs, ys
+        t1 = time.time()
+        net, loss = learned.train(ctx, X, y, epochs=args.epochs)
+        ctx.sync()
+        t_train = time.time() - t1
+        log(f"trained on {len(X)} samples ({y.mean():.3f} bound) in {t_train:.1f}s, final loss {loss:.4f}")
+        del X, y
+        if args.state:
+            np.savez(args.state, weights=net.weights()[0], **prev)
+
+    # --- evaluated snapshot: batched without and with the learned warm start -----------
     ctx.load(pop)
-    for _ in range(args.warmup):
-        ctx.run(prm)
-    ctx.sync()
-    times = []
-    for k in range(args.steps):
-        ctx.sync()
-        w0 = time.perf_counter()
-        ms = ctx.run(prm)
-        ctx.sync()
-        w1 = time.perf_counter()
-        times.append((ms, (w1 - w0) * 1e3))
-        log(f"step {k}: {ms:.2f} ms (events) {times[-1][1]:.2f} ms (wall)")
-    walk_ms = ctx.stats()["walk_ms"]
-    out = ctx.fetch()
+    ctx.set_previous(prev)
+    if args.learned_only:
+        ms_plain_b, ref = [float("nan")], None
+    else:
+        ms_plain_b = timed_steps(ctx, args.steps, args.warmup, lambda: ctx.run(prm))
+        ref = ctx.fetch()
+        log(f"batched (no warm start): {np.mean(ms_plain_b):.2f} ms/step")
+
+    def learned_step():
+        ctx.features(prm, fetch=False)
+        ctx.propose(net, args.threshold, count=False)
+        ctx.run(prm, hbtx.WARMSTART)
+
+    ms_learned = timed_steps(ctx, args.steps, args.warmup, learned_step)
+    st_time = ctx.stats()
+    walk_ms = st_time["walk_ms"]
+    rounds = int(st_time["rounds"])
+    warm = ctx.fetch()
+    npro, inf_ms = ctx.propose(net, args.threshold)
+    log(f"batched + learned: {np.mean(ms_learned):.2f} ms/step (walk {walk_ms:.2f} ms, {rounds} rounds, "
+        f"{npro} proposed, inference {inf_ms:.2f} ms)")
+    if args.learned_only:
+        log(json.dumps({"ms_per_step": float(np.mean(ms_learned)), "walk_ms": walk_ms, "rounds": rounds,
+                        "walk_avg_launch_ms": walk_ms / max(1, rounds)}))
+        ctx.close()
+        return
+    mismatch = learned.bound_mass_mismatch(pop, warm, ref)
     # work counters: one extra, untimed run of the counting instantiation (same results, deterministic work)
-    ctx.run(prm, hbtx.COUNT)
+    ctx.run(prm, hbtx.WARMSTART | hbtx.COUNT)
     st = ctx.stats()
+    ctx.run(prm, hbtx.COUNT)
+    st_plain = ctx.stats()
     st["walk_ms"] = walk_ms
-    ms_step = float(np.mean([t[1] for t in times]))
+    ms_step = float(np.mean(ms_learned))
     value = pop.n / (ms_step / 1e3)
-    res = out["result"]
+    batched_value = pop.n / (float(np.mean(ms_plain_b)) / 1e3)
     flops = walk_flops(st)
     achieved = flops / (walk_ms / 1e3) / 1e12 if walk_ms > 0 else 0.0
+    res = warm["result"]
     line = {
         "metric": METRIC,
         "value": value,
@@ -107,75 +226,58 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "power-law NFW subhalo population, batched unbinding",
+        "config": {"workload": "evolved power-law NFW subhalo population (5e7 particles, 8 ID-preserving "
+                               f"snapshots), snapshot {args.snapshot} unbound with the learned warm start",
                    "n_particles": int(pop.n), "n_subhalos": int(pop.nsub), "theta": 0.7,
                    "softening": 0.001, "bound_mass_precision": 0.995, "max_sample_size": 0,
-                   "direct_cutoff": int(args.direct_cutoff), "seed": args.seed,
-                   "rounds": int(st["rounds"]), "bound_particles": int(res["nbound"].sum()),
-                   "disrupted": int(res["disrupted"].sum())},
+                   "direct_cutoff": int(args.direct_cutoff), "seed": args.seed, "snapshot": args.snapshot,
+                   "trained_on_snapshots": f"0-{args.snapshot - 1}", "threshold": args.threshold,
+                   "rounds": rounds, "bound_particles": int(res["nbound"].sum()),
+                   "disrupted": int(res["disrupted"].sum()), "proposed": int(npro)},
         "roofline": {"bound": "valu", "kernel": "k_walk_energy", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": None, "kernel_ms_per_step": walk_ms,
-                     "flop_per_step": flops},
+                     "traffic": None, "kernel_ms_per_step": walk_ms, "launches_per_step": rounds,
+                     "kernel_avg_launch_ms": walk_ms / max(1, rounds), "flop_per_step": flops},
+        "bound_mass_mismatch_vs_plain": mismatch,
+        "batched_without_learned": {"value": batched_value, "unit": "particles/s",
+                                    "ms_per_step": float(np.mean(ms_plain_b)), "rounds": int(st_plain["rounds"]),
+                                    "walk_work": st_plain["tree_lane_visits"] + st_plain["direct_lane_pairs"]},
+        "learned_gain": value / batched_value,
+        "learned": {"inference_ms": inf_ms, "train_seconds": t_train, "label_seconds": t_lab,
+                    "training_samples": 5 * args.train_per_snapshot if not args.state else None,
+                    "final_loss": loss, "walk_work": st["tree_lane_visits"] + st["direct_lane_pairs"],
+                    "mean_iterations": float(res["iterations"].mean()),
+                    "mean_iterations_plain": float(ref["result"]["iterations"].mean())},
         "walk_stats": {k: v for k, v in st.items() if not k.startswith("unused")},
     }
 
     if not args.no_plain:
-        counts = pop.counts()
-        rng = np.random.default_rng(123)
-        big = np.where(counts > 10000)[0]
-        small = np.where(counts <= 10000)[0]
-        pick = rng.choice(small, max(1, int(args.plain_sample * len(small))), replace=False) if len(small) else []
-        sample = np.concatenate([big, np.sort(pick)]).astype(np.int64)
-        weight_small = len(small) / max(1, len(pick))
-        sub = pop.subset(sample)
-        c2 = hbtx.Context(0)
-        c2.load(sub)
-        c2.run(prm, hbtx.PLAIN)  # warm-up
-        w0 = time.perf_counter()
-        c2.run(prm, hbtx.PLAIN)
-        c2.sync()
-        # per-subhalo timing is not exposed; split the sample run by re-running the two strata
-        t_all = time.perf_counter() - w0
-        subb = pop.subset(big)
-        c2.load(subb)
-        w0 = time.perf_counter()
-        c2.run(prm, hbtx.PLAIN)
-        c2.sync()
-        t_big = time.perf_counter() - w0
-        t_small = max(0.0, t_all - t_big)
-        plain_est = t_big + t_small * weight_small
-        c2.close()
-        plain_rate = pop.n / plain_est
-        line["plain_gpu"] = {"value": plain_rate, "unit": "particles/s", "seconds_estimated": plain_est,
-                             "sample": f"all {len(big)} subhaloes with >1e4 particles plus {len(pick)} of "
-                                       f"{len(small)} smaller ones (weight {weight_small:.1f})"}
+        est, sample = plain_gpu_estimate(hbtx, pop, prm, args.plain_sample)
+        plain_rate = pop.n / est
+        line["plain_gpu"] = {"value": plain_rate, "unit": "particles/s", "seconds_estimated": est,
+                             "sample": sample}
         line["speedup_vs_plain"] = value / plain_rate
-        log(f"plain GPU estimate {plain_est:.2f}s -> {plain_rate:.3e} particles/s")
+        log(f"plain GPU estimate {est:.2f}s -> {plain_rate:.3e} particles/s")
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
         cores = min(16, os.cpu_count() or 1)
-        rng = np.random.default_rng(321)
-        order = rng.permutation(pop.nsub)
+        order = np.random.default_rng(321).permutation(pop.nsub)
         # grow the random sample until its CPU time reaches the target
         take = 64
-        secs = 0.0
-        n_done = 0
         while True:
             ids = np.sort(order[:take])
             sub = pop.subset(ids)
-            r = pyoracle.unbind_batch(sub, pyoracle.default_params(bound_mass_precision=0.995),
-                                      nthreads=cores)
+            r = pyoracle.unbind_batch(sub, pyoracle.default_params(bound_mass_precision=0.995), nthreads=cores)
             secs = r["seconds"]
             n_done = sub.n
             if secs >= args.cpu_seconds * 0.5 or take >= pop.nsub:
                 break
             take = min(pop.nsub, int(take * max(2.0, min(8.0, args.cpu_seconds / max(secs, 1e-3)))))
         line["cpu_baseline"] = {"value": n_done / secs, "unit": "particles/s", "cores": cores, "kind": "port",
-                                "sample": f"{take} random subhaloes ({n_done} particles) of the same "
-                                          f"population, {secs:.1f}s of unbinding"}
+                                "sample": f"{take} random subhaloes ({n_done} particles) of snapshot "
+                                          f"{args.snapshot}, plain method, {secs:.1f}s of unbinding"}
         log(f"cpu baseline {n_done / secs:.3e} particles/s on {cores} cores")
     ctx.close()
     print(json.dumps(line), flush=True)

@@ -60,6 +60,13 @@ def labels(pop, out):
     return previous_state(pop, out)["bound"]
 
 
+def bound_mass_mismatch(pop, out, ref):
+    """Mass of the symmetric difference of the bound sets over the bound mass of ref."""
+    a, b = labels(pop, out).astype(bool), labels(pop, ref).astype(bool)
+    m = pop.mass.astype(np.float64)
+    return float(m[a ^ b].sum() / max(m[b].sum(), 1e-300))
+
+
 def snapshot_features(ctx, pop, prev, params):
     ctx.load(pop)
     ctx.set_previous(prev)

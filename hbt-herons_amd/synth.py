@@ -23,6 +23,8 @@ M(<x) ~ ln(1+x) - x/(1+x); velocities are isotropic Gaussians with the local
 isotropic-Jeans dispersion of the truncated profile; the escape speed uses the
 closed-form truncated-NFW potential.
 """
+import os
+
 import numpy as np
 
 G_GRAV = 43.0071
@@ -137,13 +139,18 @@ class Population:
 def mass_weighted_frame(pos, vel, mass, offsets):
     """Mass-weighted mean position/velocity of every subhalo (float64 sums)."""
     nsub = len(offsets) - 1
-    seg = np.repeat(np.arange(nsub), np.diff(offsets))
+    nz = np.diff(offsets) > 0
+    starts = np.asarray(offsets[:-1])[nz]
     m = mass.astype(np.float64)
-    msum = np.bincount(seg, weights=m, minlength=nsub)
-    msum = np.where(msum > 0, msum, 1.0)
-    rp = np.stack([np.bincount(seg, weights=m * pos[:, k], minlength=nsub) for k in range(3)], 1) / msum[:, None]
-    rv = np.stack([np.bincount(seg, weights=m * vel[:, k], minlength=nsub) for k in range(3)], 1) / msum[:, None]
-    return rp.astype(np.float32), rv.astype(np.float32)
+    msum = np.ones(nsub)
+    rp = np.zeros((nsub, 3))
+    rv = np.zeros((nsub, 3))
+    if len(starts):
+        m32 = np.ascontiguousarray(mass, dtype=np.float32)[:, None]
+        msum[nz] = np.add.reduceat(m, starts)
+        rp[nz] = np.add.reduceat(pos * m32, starts, axis=0, dtype=np.float64)
+        rv[nz] = np.add.reduceat(vel * m32, starts, axis=0, dtype=np.float64)
+    return (rp / msum[:, None]).astype(np.float32), (rv / msum[:, None]).astype(np.float32)
 
 
 def _nfw_members(rng, n, c, rs, gm_tot):
@@ -321,8 +328,24 @@ def host_with_subs(seed=1, n_host=2_000_000, nsub=50, nsub_min=500, nsub_max=50_
                       meta=dict(kind="host_with_subs", seed=seed, n_host=n_host, sub_counts=ns))
 
 
-def evolve(pop, nsnap=8, seed=4, shrink=0.9, kick=1.5, jitter=0.02, dt=0.002):
-    """ID-preserving snapshot sequence of a population.
+_NOISE_STREAMS = 16  # fixed, so the noise does not depend on the thread count
+
+
+def _jitter_fill(gens, pool, out, jitter):
+    """out = 1 + jitter * N(0,1), filled by independent streams over fixed chunks."""
+    chunks = np.array_split(out.reshape(-1), _NOISE_STREAMS)
+
+    def fill(k):
+        c = chunks[k]
+        gens[k].standard_normal(out=c, dtype=np.float32)
+        c *= np.float32(jitter)
+        c += np.float32(1.0)
+
+    list(pool.map(fill, range(_NOISE_STREAMS)))
+
+
+def evolve_iter(pop, nsnap=8, seed=4, shrink=0.9, kick=1.5, jitter=0.02, dt=0.002):
+    """ID-preserving snapshot sequence of a population (a generator).
 
     Snapshot k strips particles outside a tidal radius r_200 * shrink**k by a
     velocity kick of ``kick`` x the local escape speed (in a random direction),
@@ -330,34 +353,45 @@ def evolve(pop, nsnap=8, seed=4, shrink=0.9, kick=1.5, jitter=0.02, dt=0.002):
     Gaussian noise to every particle's offset from its subhalo centre and
     velocity.  Particles keep their ids and their slot in the arrays.
     """
+    from concurrent.futures import ThreadPoolExecutor
     rng = np.random.default_rng(seed)
+    gens = [np.random.default_rng(s) for s in np.random.SeedSequence(seed).spawn(_NOISE_STREAMS)]
+    pool = ThreadPoolExecutor(min(_NOISE_STREAMS, os.cpu_count() or 1))
     meta = pop.meta
     nsub = pop.nsub
-    sid = np.repeat(np.arange(nsub), pop.counts())
-    centre = meta["centres"][sid]
-    bulk = meta["bulk"][sid]
-    rs = meta["rs"][sid]
-    conc = meta["conc"][sid]
-    gm = G_GRAV * PARTICLE_MASS * meta["nmem"][sid]
-    r200 = conc * rs
-    dx = pop.pos.astype(np.float64) - centre
-    dv = pop.vel.astype(np.float64) - bulk
+    counts = pop.counts()
+    sid = np.repeat(np.arange(nsub), counts)
+    centre = np.repeat(meta["centres"].astype(np.float32), counts, axis=0)
+    bulk = np.repeat(meta["bulk"].astype(np.float32), counts, axis=0)
+    rs = meta["rs"]
+    conc = meta["conc"]
+    gm = G_GRAV * PARTICLE_MASS * meta["nmem"]
+    r200 = (conc * rs).astype(np.float32)[sid]
+    dx = pop.pos - centre
+    dv = pop.vel - bulk
     kicked = np.zeros(pop.n, bool)
-    snaps = []
+    kdt = np.zeros((pop.n, 1), np.float32)  # dt for kicked particles, else 0
+    noise = np.empty_like(dx)
     for k in range(nsnap):
         if k > 0:
-            dx = dx * (1.0 + jitter * rng.normal(0.0, 1.0, dx.shape))
-            dv = dv * (1.0 + jitter * rng.normal(0.0, 1.0, dv.shape))
-            dx[kicked] += dv[kicked] * dt
-        r = np.sqrt((dx ** 2).sum(1))
-        rt = r200 * shrink ** k
-        newk = (~kicked) & (r > rt)
-        if newk.any():
-            vesc = np.sqrt(2.0 * np.abs(nfw_potential(r[newk] / rs[newk], conc[newk], gm[newk], rs[newk])))
-            dv[newk] += _unit_vectors(rng, int(newk.sum())) * (kick * vesc)[:, None]
-            kicked |= newk
-        p = Population((dx + centre).astype(np.float32), (dv + bulk).astype(np.float32),
-                       pop.mass, pop.offsets, pop.pid, pop.ptype, pop.u,
-                       meta=dict(meta, snapshot=k, kicked=kicked.copy()))
-        snaps.append(p)
-    return snaps
+            _jitter_fill(gens, pool, noise, jitter)
+            dx *= noise
+            _jitter_fill(gens, pool, noise, jitter)
+            dv *= noise
+            np.multiply(dv, kdt, out=noise)
+            dx += noise
+        r = np.sqrt(np.einsum("ij,ij->i", dx, dx))
+        newk = np.where((~kicked) & (r > r200 * np.float32(shrink ** k)))[0]
+        if len(newk):
+            s = sid[newk]
+            vesc = np.sqrt(2.0 * np.abs(nfw_potential(r[newk] / rs[s], conc[s], gm[s], rs[s])))
+            dv[newk] += (_unit_vectors(rng, len(newk)) * (kick * vesc)[:, None]).astype(np.float32)
+            kicked[newk] = True
+            kdt[newk] = np.float32(dt)
+        yield Population(dx + centre, dv + bulk, pop.mass, pop.offsets, pop.pid, pop.ptype, pop.u,
+                         meta=dict(meta, snapshot=k, kicked=kicked.copy()))
+
+
+def evolve(pop, nsnap=8, seed=4, shrink=0.9, kick=1.5, jitter=0.02, dt=0.002):
+    """evolve_iter as a list."""
+    return list(evolve_iter(pop, nsnap, seed, shrink, kick, jitter, dt))
