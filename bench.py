@@ -105,6 +105,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tree", action="store_true", help="skip the host+subhaloes tree-potential line")
     ap.add_argument("--state", default=None,
                     help="npz cache of the trained weights and the previous snapshot's outputs: written "
                          "after training, read instead of labelling and training when it exists")
@@ -112,7 +113,7 @@ def main():
                     help="time only the learned step (for kernel profiles); implies --no-plain --no-cpu")
     args = ap.parse_args()
     if args.learned_only:
-        args.no_plain = args.no_cpu = True
+        args.no_plain = args.no_cpu = args.no_tree = True
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; how the path would shard over "
                                    "several is described in DESIGN.md, not built"}))
@@ -258,6 +259,25 @@ def main():
                              "sample": sample}
         line["speedup_vs_plain"] = value / plain_rate
         log(f"plain GPU estimate {est:.2f}s -> {plain_rate:.3e} particles/s")
+
+    if not args.no_tree:
+        # one tree over a 2e6-particle NFW host with 50 embedded subhaloes, potential at every particle
+        host = synth.host_with_subs(seed=1)
+        tms = []
+        for _ in range(3):
+            w0 = time.perf_counter()
+            tree = ctx.tree(host.pos, host.mass, prm)
+            w1 = time.perf_counter()
+            tree.potential(host.pos, host.mass)
+            w2 = time.perf_counter()
+            tree.close()
+            tms.append(((w1 - w0) * 1e3, (w2 - w1) * 1e3))
+        build_ms, walk_ms_c1 = np.median([t[0] for t in tms]), np.median([t[1] for t in tms])
+        line["tree_potential_host_with_subs"] = {
+            "particles": int(host.n), "build_ms": float(build_ms), "potential_ms": float(walk_ms_c1),
+            "particles_per_s": host.n / ((build_ms + walk_ms_c1) / 1e3),
+            "note": "GravityTree_t Build + EvaluatePotential at every particle, host<->device transfers included"}
+        log(f"host+subs tree: build {build_ms:.1f} ms, potential {walk_ms_c1:.1f} ms for {host.n} particles")
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

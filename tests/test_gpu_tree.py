@@ -123,3 +123,27 @@ def test_binding_energy_external_targets(ctx):
     k = 0.5 * (dv.astype(np.float32) ** 2).sum(1)
     # binding energy = potential + float kinetic term; error relative to |potential|
     assert np.all(np.abs(e - (ref_pot + k)) <= 2e-5 * np.abs(ref_pot))
+
+
+def test_host_with_subhaloes_at_scale(ctx):
+    """One NFW host of 2e6 particles with 50 embedded subhaloes, one tree over all
+    particles, potentials at every particle.  On a 20,000-particle random subsample:
+    (a) the FP32 walk against the oracle's FP64 walk of the same tree (arithmetic,
+    1e-5), (b) against the FP64 direct sum (the theta = 0.7 approximation itself,
+    reported and bounded)."""
+    pop = synth.host_with_subs(seed=1)
+    prm = hbtx.workload_params()
+    t = ctx.tree(pop.pos, pop.mass, prm)
+    phi = t.potential(pop.pos, pop.mass)
+    assert np.all(phi < 0) and np.all(np.isfinite(phi))
+    rng = np.random.default_rng(5)
+    sel = np.sort(rng.choice(pop.n, 20000, replace=False))
+    same, _ = O.tree_potential(pop.pos, pop.mass, pop.pos[sel], pop.mass[sel],
+                               params=dict(theta=0.7))
+    rel_same = np.abs(phi[sel] - same) / np.abs(same)
+    direct = O.direct_potential(pop.pos, pop.mass, pop.pos[sel], pop.mass[sel])
+    rel_direct = np.abs(phi[sel] - direct) / np.abs(direct)
+    print(f"host+subs n={pop.n}: vs same-tree FP64 walk max {rel_same.max():.2e}; vs FP64 direct sum "
+          f"median {np.median(rel_direct):.2e} q99 {np.quantile(rel_direct, 0.99):.2e} max {rel_direct.max():.2e}")
+    assert rel_same.max() < 1e-5
+    assert np.median(rel_direct) < 2e-3 and np.quantile(rel_direct, 0.99) < 1e-2
