@@ -1752,7 +1752,7 @@ int hbtx_mlp_predict(hbtx_ctx *c, hbtx_mlp *M, int64_t n, const float *X, float 
     dX.alloc(n * MLP_IN);
     dl.alloc(n);
     HIPCHK(hipMemcpyAsync(dX.p, X, n * MLP_IN * sizeof(float), hipMemcpyHostToDevice, st));
-    k_mlp_forward<<<blocks_for(n, 256, 1 << 30), 256, 0, st>>>(M->wf.p, dX.p, n, dl.p);
+    k_mlp_forward<<<mlp_forward_blocks(n), 256, 0, st>>>(M->wf.p, M->wb.p, dX.p, n, dl.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(logit, dl.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -1845,7 +1845,7 @@ int hbtx_batch_propose(hbtx_ctx *c, hbtx_mlp *M, float threshold, int64_t *nprop
     const int64_t n = B.n;
     B.logit.alloc(n);
     HIPCHK(hipEventRecord(c->wev0, st));
-    k_mlp_forward<<<blocks_for(n, 256, 1 << 30), 256, 0, st>>>(M->wf.p, B.feat.p, n, B.logit.p);
+    k_mlp_forward<<<mlp_forward_blocks(n), 256, 0, st>>>(M->wf.p, M->wb.p, B.feat.p, n, B.logit.p);
     k_proposal_from_logit<<<blocks_for(n), 256, 0, st>>>(B.logit.p, n, std::log(threshold / (1.f - threshold)),
                                                           B.proposal.p);
     HIPCHK(hipEventRecord(c->wev1, st));

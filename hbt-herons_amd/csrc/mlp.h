@@ -41,37 +41,122 @@ HBTX_HD float bf16_to_f32(uint16_t b)
   return __uint_as_float((uint32_t)b << 16);
 }
 
-// Inference: logit of every sample (one thread per sample).
-__global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ w, const float *__restrict__ X, int64_t n,
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+HBTX_DEV __bf16 bf16_bits(uint16_t b)
+{
+  return __builtin_bit_cast(__bf16, b);
+}
+
+// Inference on the matrix cores (v_mfma_f32_32x32x16_bf16): one wave evaluates 32
+// samples per step, neurons as matrix rows and samples as columns.
+//   layer 1: H1^T[64 x 32] = W1[64 x 8] X^T, with X split into bf16 hi + lo parts
+//            placed in the two K halves (so the product keeps FP32 accuracy);
+//   layer 2: H2^T[64 x 32] = W2[64 x 64] relu(H1^T + b1), H1 again split hi + lo;
+//   layer 3: z = w3 . relu(H2^T + b2) + b3 on the vector units.
+// Lane l holds output column l % 32 and rows 8*(r/4) + 4*(l/32) + r%4 of a 32 x 32
+// accumulator block (r = 0..15); a lane's layer-1 rows are exactly the K entries
+// it supplies to layer 2, with the K order of both operands permuted alike.
+__global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ w, const uint16_t *__restrict__ wb,
+                                                     const float *__restrict__ X, int64_t n,
                                                      float *__restrict__ logit)
 {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n)
-    return;
-  float x[MLP_IN];
+  const int lane = threadIdx.x & 63, h = lane >> 5, c = lane & 31;
+  bf16x8 a1[2];
+  bf16x8 a2[2][4];
+  float b1v[32], b2v[32], w3v[32];
 #pragma unroll
-  for (int i = 0; i < MLP_IN; i++)
-    x[i] = X[p * MLP_IN + i];
-  float h1[MLP_H];
-#pragma unroll
-  for (int j = 0; j < MLP_H; j++)
+  for (int mb = 0; mb < 2; mb++)
   {
-    float a = wload(w, MLP_OFF_B1 + j);
+    const int row = 32 * mb + c;
 #pragma unroll
-    for (int i = 0; i < MLP_IN; i++)
-      a += wload(w, MLP_OFF_W1 + j * MLP_IN + i) * x[i];
-    h1[j] = fmaxf(a, 0.f);
+    for (int s = 0; s < 8; s++)
+      a1[mb][s] = bf16_bits(wb[MLP_OFF_W1 + row * MLP_IN + s]);
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+      for (int s = 0; s < 8; s++)
+        a2[mb][kb][s] = bf16_bits(wb[MLP_OFF_W2 + row * MLP_H + 8 * (2 * kb + s / 4) + 4 * h + (s & 3)]);
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+    {
+      const int i = 32 * mb + 8 * (r / 4) + 4 * h + (r & 3);
+      b1v[16 * mb + r] = w[MLP_OFF_B1 + i];
+      b2v[16 * mb + r] = w[MLP_OFF_B2 + i];
+      w3v[16 * mb + r] = w[MLP_OFF_W3 + i];
+    }
   }
-  float z = wload(w, MLP_OFF_B3);
-  for (int j = 0; j < MLP_H; j++)
+  const float b3 = w[MLP_OFF_B3];
+  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x / HBTX_WAVE) + (threadIdx.x >> 6);
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / HBTX_WAVE);
+  for (int64_t g = wave; g * 32 < n; g += nwaves)
   {
-    float a = wload(w, MLP_OFF_B2 + j);
+    const int64_t j = g * 32 + c;
+    float x[8];
+    if (j < n)
+    {
+      const float4 u0 = reinterpret_cast<const float4 *>(X + j * MLP_IN)[0];
+      const float4 u1 = reinterpret_cast<const float4 *>(X + j * MLP_IN)[1];
+      x[0] = u0.x, x[1] = u0.y, x[2] = u0.z, x[3] = u0.w, x[4] = u1.x, x[5] = u1.y, x[6] = u1.z, x[7] = u1.w;
+    }
+    else
+    {
 #pragma unroll
-    for (int i = 0; i < MLP_H; i++)
-      a += wload(w, MLP_OFF_W2 + j * MLP_H + i) * h1[i];
-    z += wload(w, MLP_OFF_W3 + j) * fmaxf(a, 0.f);
+      for (int s = 0; s < 8; s++)
+        x[s] = 0.f;
+    }
+    bf16x8 bx;
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+    {
+      const __bf16 hi = (__bf16)x[s];
+      bx[s] = h ? (__bf16)(x[s] - (float)hi) : hi;
+    }
+    f32x16 zero;
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+      zero[r] = 0.f;
+    bf16x8 bh[4], bl[4];
+#pragma unroll
+    for (int mb = 0; mb < 2; mb++)
+    {
+      const f32x16 d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[mb], bx, zero, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 16; r++)
+      {
+        const float v = fmaxf(d1[r] + b1v[16 * mb + r], 0.f);
+        const int q = 4 * mb + r / 4, kb = q >> 1, s = 4 * (q & 1) + (r & 3);
+        const __bf16 hi = (__bf16)v;
+        bh[kb][s] = hi;
+        bl[kb][s] = (__bf16)(v - (float)hi);
+      }
+    }
+    float z = 0.f;
+#pragma unroll
+    for (int mb = 0; mb < 2; mb++)
+    {
+      f32x16 d2 = zero;
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++)
+      {
+        d2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2[mb][kb], bh[kb], d2, 0, 0, 0);
+        d2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2[mb][kb], bl[kb], d2, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; r++)
+        z += w3v[16 * mb + r] * fmaxf(d2[r] + b2v[16 * mb + r], 0.f);
+    }
+    z += __shfl_xor(z, 32, HBTX_WAVE);
+    if (h == 0 && j < n)
+      logit[j] = z + b3;
   }
-  logit[p] = z;
+}
+
+inline int mlp_forward_blocks(int64_t n)
+{
+  const int64_t groups = (n + 31) / 32;
+  return (int)std::max<int64_t>(1, std::min<int64_t>((groups + 3) / 4, 4096));
 }
 
 // Forward + backward of one block of MLP_BLOCK samples; writes this block's

@@ -94,11 +94,12 @@ __global__ void k_tree_count(TreeArgs A, int32_t n_all, int32_t *__restrict__ cn
     }
     cnt[j] = c;
   }
-  // deepest cell level: wave maximum, one atomic per wave
+  // deepest cell level: wave maximum, and an atomic only from waves that raise the
+  // current value (one shared word: an atomic from every wave serialises)
 #pragma unroll
   for (int d = 1; d < HBTX_WAVE; d <<= 1)
     depth = max(depth, __shfl_xor(depth, d, HBTX_WAVE));
-  if (lane_id() == 0 && depth > 0)
+  if (lane_id() == 0 && depth > 0 && depth > __hip_atomic_load(maxdepth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
     atomicMax(maxdepth, depth);
 }
 
