@@ -11,6 +11,9 @@
 #define HBTX_KEY_LEVELS 21
 
 #define GRID_STRIDE(i, n) for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+// Loop over the active-slot tiles assigned to this block (tile = int4: subhalo,
+// first slot, end slot, first tile of the subhalo); the count lives on the device.
+#define TILE_LOOP(tiles, ntiles_p) for (int t_ = blockIdx.x, nt_ = *(ntiles_p); t_ < nt_; t_ += gridDim.x)
 
 namespace hbtx
 {

@@ -12,6 +12,7 @@
 #include "points.h"
 #include "subsample.h"
 #include "learn.h"
+#include "tiles.h"
 
 #include <algorithm>
 #include <cmath>
@@ -128,6 +129,10 @@ struct Batch
   DArr<float> mlast, mbound, ref, oldref, mb, upscale, ref_in, mb_in, minit;
   DArr<int32_t> warm, npro, reset_;
   DArr<float> feat, logit;
+  // active-slot tiles of the current round and their prefix sums
+  DArr<int4> stiles;
+  DArr<int32_t> stile_n, stile_off, tsum32, tpre32, sub_first32, sub_total32;
+  DArr<int64_t> tsum64, tpre64, sub_first64, sub_total64;
   // previous snapshot's outputs for the learned warm start (same slot = same particle)
   DArr<int32_t> prev_rank;
   DArr<uint8_t> prev_bound;
@@ -210,19 +215,24 @@ namespace
 // primitives
 // ---------------------------------------------------------------------------
 template <class T>
-void scan_excl(hbtx_ctx *c, const T *in, T *out, int64_t n, DArr<T> &part)
+void scan_excl(hbtx_ctx *c, const T *in, T *out, int64_t n, DArr<T> &part, const int32_t *n_dev = nullptr)
 {
   const int64_t nb = std::max<int64_t>(1, (n + SCAN_TILE - 1) / SCAN_TILE);
   part.alloc(nb);
-  k_scan_reduce<T><<<(unsigned)nb, SCAN_THREADS, 0, c->stream>>>(in, n, part.p);
+  k_scan_reduce<T><<<(unsigned)nb, SCAN_THREADS, 0, c->stream>>>(in, n, n_dev, part.p);
   k_scan_partials<T><<<1, SCAN_THREADS, 0, c->stream>>>(part.p, nb);
-  k_scan_down<T><<<(unsigned)nb, SCAN_THREADS, 0, c->stream>>>(in, out, n, part.p);
+  k_scan_down<T><<<(unsigned)nb, SCAN_THREADS, 0, c->stream>>>(in, out, n, n_dev, part.p);
   HIPCHK(hipGetLastError());
 }
 
 void scan32(hbtx_ctx *c, const int32_t *in, int32_t *out, int64_t n)
 {
   scan_excl<int32_t>(c, in, out, n, c->part32);
+}
+// length min(n_max, *n_dev), known on the device only; out[length] = total
+void scan32_dev(hbtx_ctx *c, const int32_t *in, int32_t *out, int64_t n_max, const int32_t *n_dev)
+{
+  scan_excl<int32_t>(c, in, out, n_max, c->part32, n_dev);
 }
 void scan64(hbtx_ctx *c, const int64_t *in, int64_t *out, int64_t n)
 {
@@ -406,6 +416,20 @@ void batch_reserve(hbtx_ctx *c, Batch &B)
   B.misc.alloc(8);
   B.dres.alloc(ns + 1);
   c->stats.alloc(NSTATS);
+  {
+    const int64_t maxt = n / SLOT_TILE + ns + 2;
+    B.stiles.alloc(maxt);
+    B.stile_n.alloc(ns + 1);
+    B.stile_off.alloc(ns + 1);
+    B.tsum32.alloc(maxt + 1);
+    B.tpre32.alloc(maxt + 1);
+    B.tsum64.alloc(maxt + 1);
+    B.tpre64.alloc(maxt + 1);
+    B.sub_first32.alloc(ns + 1);
+    B.sub_total32.alloc(ns + 1);
+    B.sub_first64.alloc(ns + 1);
+    B.sub_total64.alloc(ns + 1);
+  }
   reserve_sort(c, n, ns);
 }
 
@@ -491,7 +515,7 @@ TreeArgs tree_args(Batch &B, View &V, const DevParams &P)
   T.upscale = V.S.upscale;
   T.cnt = B.cnt.p;
   T.cscan = B.cscan.p;
-  T.s_total = B.fscan.p + V.n;
+  T.s_total = B.tpre32.p + (V.n / SLOT_TILE + V.nsub + 1); // grand total of the source tile scan
   T.ent = B.ent.p;
   T.ent_depth = B.ent_depth.p;
   T.cap = B.ent_cap;
@@ -500,7 +524,43 @@ TreeArgs tree_args(Batch &B, View &V, const DevParams &P)
   return T;
 }
 
-void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P);
+void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, const int32_t *nt);
+
+int64_t slot_tile_cap(const View &V)
+{
+  return V.n / SLOT_TILE + V.nsub + 1; // (tree_args reads the source total at tpre32[this])
+}
+int tile_grid(hbtx_ctx *c, const View &V)
+{
+  return (int)std::max<int64_t>(1, std::min<int64_t>(slot_tile_cap(V), 16 * c->num_cu));
+}
+
+// Tiles over the slot ranges of the view's active subhaloes (mode set by the
+// round's plan); returns the device-side tile count.
+const int32_t *build_slot_tiles(hbtx_ctx *c, Batch &B, View &V)
+{
+  const int gs = blocks_for(V.nsub, 256, 1 << 20);
+  k_slot_ntiles<<<gs, 256, 0, c->stream>>>(V.S, B.stile_n.p);
+  scan32(c, B.stile_n.p, B.stile_off.p, V.nsub);
+  k_slot_tiles<<<gs, 256, 0, c->stream>>>(V.S, B.stile_off.p, B.stiles.p);
+  return B.stile_off.p + V.nsub;
+}
+
+// Exclusive prefix of flag over the active tiles' slots: out[q] (global over the
+// tiles), per subhalo the prefix at its first slot and its total; returns the
+// device pointer to the grand total.
+template <typename T>
+const T *tile_scan(hbtx_ctx *c, View &V, const int4 *tiles, const int32_t *nt, const T *flag, T *out, T *tsum,
+                   T *tpre, T *sub_first, T *sub_total, DArr<T> &part)
+{
+  const int64_t maxt = slot_tile_cap(V);
+  const int tg = tile_grid(c, V);
+  k_tile_sum<T><<<tg, SLOT_THREADS, 0, c->stream>>>(tiles, nt, (int32_t)maxt, flag, tsum);
+  scan_excl<T>(c, tsum, tpre, maxt, part);
+  k_tile_down<T><<<tg, SLOT_THREADS, 0, c->stream>>>(V.S, tiles, nt, flag, tpre, out, sub_first, sub_total);
+  HIPCHK(hipGetLastError());
+  return tpre + maxt;
+}
 
 // Sources of the round -> dense list, then (tree-mode subhaloes) the octree.
 void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int round)
@@ -509,6 +569,7 @@ void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, 
   const int64_t n = V.n;
   const int gb = blocks_for(n);
   const int gs = blocks_for(V.nsub, 256, 1 << 20);
+  const int32_t *nt = build_slot_tiles(c, B, V);
   if (P.mss > 0)
   {
     // candidate ranks in E-list order and the mass upscale factors (:290-391)
@@ -521,24 +582,25 @@ void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, 
     k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red2);
     k_upscale_finish<<<gs, 256, 0, st>>>(V.S, P, V.red2);
   }
-  else
-    // rank is irrelevant without subsampling (every candidate is a source)
-    k_fill_i32<<<gb, 256, 0, st>>>(B.crank.p, n + 1, 0);
-  k_mark_sources<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, round, B.crank.p, B.flag.p);
-  build_sources_tree(c, B, V, P);
+  // (without subsampling the candidate rank is irrelevant: every candidate is a source)
+  k_mark_sources<<<tile_grid(c, V), 256, 0, st>>>(V.S, V.Q, P, B.stiles.p, nt, round,
+                                                  P.mss > 0 ? B.crank.p : nullptr, B.flag.p);
+  build_sources_tree(c, B, V, P, nt);
 }
 
-// Flags in B.flag -> dense source list, tree cube, keys, sort, octree entries.
-void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
+// Flags in B.flag (on the slots of the active tiles) -> dense source list, tree
+// cube, keys, sort, octree entries.
+void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, const int32_t *nt)
 {
   hipStream_t st = c->stream;
   const int64_t n = V.n;
   const int gb = blocks_for(n);
   const int gs = blocks_for(V.nsub, 256, 1 << 20);
-  scan32(c, B.flag.p, B.fscan.p, n);
-  k_scatter_sources<<<gb, 256, 0, st>>>(V.S, n, B.flag.p, B.fscan.p, B.src.p, B.src_sub.p);
-  k_sub_src<<<gs, 256, 0, st>>>(V.S, B.fscan.p);
-  const int32_t *s_total = B.fscan.p + n;
+  const int32_t *s_total = tile_scan<int32_t>(c, V, B.stiles.p, nt, B.flag.p, B.fscan.p, B.tsum32.p, B.tpre32.p,
+                                              B.sub_first32.p, B.sub_total32.p, c->part32);
+  k_scatter_sources<<<tile_grid(c, V), 256, 0, st>>>(V.S, B.stiles.p, nt, B.flag.p, B.fscan.p, B.src.p,
+                                                     B.src_sub.p);
+  k_sub_src<<<gs, 256, 0, st>>>(V.S, B.sub_first32.p);
   k_tree_src_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p);
   const int32_t *nbt = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, n);
   k_bbox_tiles<<<red_grid(c, n, V.nsub), 256, 0, st>>>(V.S, V.Q.posm, B.src.p, c->red_tiles.p, nbt);
@@ -549,8 +611,8 @@ void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   k_targets_from_sources<<<gb, 256, 0, st>>>(V.S, V.Q, B.src.p, B.src_sub.p, s_total);
   TreeArgs T = tree_args(B, V, P);
   k_fill_i32<<<1, 64, 0, st>>>(B.misc.p + 4, 1, 0);
-  k_tree_count<<<blocks_for(n + 1, 256, 1 << 24), 256, 0, st>>>(T, (int32_t)n, B.cnt.p, B.misc.p + 4);
-  scan32(c, B.cnt.p, B.cscan.p, n);
+  k_tree_count<<<blocks_for(n, 256, 8 * 1024), 256, 0, st>>>(T, B.cnt.p, B.misc.p + 4);
+  scan32_dev(c, B.cnt.p, B.cscan.p, n, s_total);
   // exact entry count and depth of this round's trees; grow the entry buffers if needed
   k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, 0x7fffffff, B.misc.p + 2);
   int32_t hm[2] = {0, 0};
@@ -568,7 +630,7 @@ void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
     B.ent_cap = (int32_t)cap;
     T = tree_args(B, V, P);
   }
-  k_tree_emit<<<blocks_for(n, 256, 1 << 24), 256, 0, st>>>(T);
+  k_tree_emit<<<blocks_for(n, 256, 8 * 1024), 256, 0, st>>>(T);
   k_sub_entries<<<gs, 256, 0, st>>>(V.S, B.cscan.p, B.ent_cap);
   k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, B.ent_cap, B.misc.p + 2);
   for (int d = maxdepth; d >= 0; d--)
@@ -640,19 +702,25 @@ void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int rou
   const int64_t n = V.n;
   const int gb = blocks_for(n);
   const int gs = blocks_for(V.nsub, 256, 1 << 20);
+  // the round's active tiles (built from the plan at the start of the round)
+  const int32_t *nt = B.stile_off.p + V.nsub;
+  const int tg = tile_grid(c, V);
   if (warm_round)
     k_fill_i32<<<gs, 256, 0, st>>>(B.aux32.p, V.nsub, 0);
-  k_partition_flags<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, B.flag64.p, warm_round ? B.aux32.p : nullptr);
-  scan64(c, B.flag64.p, B.fscan64.p, n);
-  k_partition_scatter<<<gb, 256, 0, st>>>(V.S, V.Q, n, round, B.fscan64.p);
-  k_copy_cur<<<gb, 256, 0, st>>>(V.S, V.Q, n, B.fscan64.p);
+  k_partition_flags<<<tg, 256, 0, st>>>(V.S, V.Q, P, B.stiles.p, nt, B.flag64.p,
+                                        warm_round ? B.aux32.p : nullptr);
+  tile_scan<int64_t>(c, V, B.stiles.p, nt, B.flag64.p, B.fscan64.p, B.tsum64.p, B.tpre64.p, B.sub_first64.p,
+                     B.sub_total64.p, c->part64);
+  k_partition_scatter<<<tg, 256, 0, st>>>(V.S, V.Q, B.stiles.p, nt, round, B.flag64.p, B.fscan64.p,
+                                          B.sub_first64.p);
+  k_copy_cur<<<tg, 256, 0, st>>>(V.S, V.Q, B.stiles.p, nt, B.sub_total64.p);
   if (P.mss > 0)
   {
     k_fill_i32<<<gs, 256, 0, st>>>(B.aux32.p, V.nsub, 0);
     k_count_unsampled<<<gb, 256, 0, st>>>(V.S, V.Q, P, n, B.aux32.p);
     k_set_unsampled<<<gs, 256, 0, st>>>(V.S, B.aux32.p);
   }
-  k_round_decide<<<gs, 256, 0, st>>>(V.S, P, B.fscan64.p, warm_round ? B.aux32.p : nullptr);
+  k_round_decide<<<gs, 256, 0, st>>>(V.S, P, B.sub_total64.p, warm_round ? B.aux32.p : nullptr);
   k_com_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p);
   if (P.periodic)
   {
@@ -805,9 +873,10 @@ void run_final(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot
   {
     // RefineBindingEnergyOrder (:395-425): tree over the most bound particles, true masses
     k_refine_plan<<<gs, 256, 0, st>>>(V.S, P);
+    const int32_t *nt = build_slot_tiles(c, B, V);
     k_slot_of<<<gb, 256, 0, st>>>(V.S, n, V.fvals, V.Q.cur2);
     k_mark_refine<<<gb, 256, 0, st>>>(V.S, V.Q, n, V.Q.cur2, B.flag.p);
-    build_sources_tree(c, B, V, P);
+    build_sources_tree(c, B, V, P, nt);
     round_walk(c, B, V, P);
     k_refine_keys<<<gb, 256, 0, st>>>(V.S, n, V.fvals, B.eref.p + V.p0, B.keys.p, B.seg_begin.p, B.seg_len.p);
     k_refine_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);

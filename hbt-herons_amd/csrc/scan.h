@@ -49,11 +49,20 @@ HBTX_DEV T block_excl_scan(T v, T *lds_wave, T *total)
 }
 
 // Per-tile totals with coalesced loads (thread t reads elements t, t+256, ...).
+// n_dev (may be null): the length is min(n, *n_dev), read on the device.
 template <typename T>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const T *__restrict__ in, int64_t n,
-                                                             T *__restrict__ partials)
+                                                             const int32_t *n_dev, T *__restrict__ partials)
 {
   __shared__ T lds[SCAN_THREADS / HBTX_WAVE];
+  if (n_dev)
+    n = min(n, (int64_t)*n_dev);
+  if ((int64_t)blockIdx.x * SCAN_TILE >= n)
+  {
+    if (threadIdx.x == 0)
+      partials[blockIdx.x] = 0;
+    return;
+  }
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x;
   T s = 0;
 #pragma unroll
@@ -105,10 +114,17 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_partials(T *__restrict__ 
 // Tile scan: each wave owns 1024 consecutive elements, read in 16 coalesced
 // passes of 64 and scanned in registers; wave totals are combined through LDS.
 template <typename T>
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const T *in, T *out, int64_t n,
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const T *in, T *out, int64_t n, const int32_t *n_dev,
                                                            const T *__restrict__ partials)
 {
   __shared__ T lds[SCAN_THREADS / HBTX_WAVE];
+  if (n_dev)
+    n = min(n, (int64_t)*n_dev);
+  const int64_t bb = (int64_t)blockIdx.x * SCAN_TILE;
+  // the block holding index n (or the last block) writes the grand total at out[n]
+  const bool writes_total = n >= bb && (n < bb + SCAN_TILE || blockIdx.x == gridDim.x - 1);
+  if (bb >= n && !writes_total)
+    return;
   const int w = threadIdx.x / HBTX_WAVE;
   const int lane = lane_id();
   const int64_t wbase = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)w * (HBTX_WAVE * SCAN_ITEMS);
@@ -136,8 +152,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const T *in, T *out,
     if (i < n)
       out[i] = off + v[k];
   }
-  // the last block also writes the grand total at out[n]
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+  if (writes_total && threadIdx.x == 0)
   {
     T tot = partials[blockIdx.x];
     for (int k = 0; k < SCAN_THREADS / HBTX_WAVE; k++)

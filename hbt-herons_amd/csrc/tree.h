@@ -71,26 +71,22 @@ HBTX_DEV void pair_levels(const TreeArgs &A, int j, int b, int e, int &cprev, in
 }
 
 // (also records the deepest cell level in *maxdepth)
-__global__ void k_tree_count(TreeArgs A, int32_t n_all, int32_t *__restrict__ cnt, int32_t *maxdepth)
+__global__ void k_tree_count(TreeArgs A, int32_t *__restrict__ cnt, int32_t *maxdepth)
 {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int S = *A.s_total;
   int depth = 0;
-  if (j <= n_all)
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < S; j += gridDim.x * blockDim.x)
   {
     int c = 0;
-    if (j < S)
+    const int s = A.src_sub[j];
+    if (A.treemode[s])
     {
-      const int s = A.src_sub[j];
-      if (A.treemode[s])
-      {
-        const int b = A.src_off[s], e = b + A.nsrc[s];
-        int cprev, cj;
-        pair_levels(A, j, b, e, cprev, cj);
-        c = max(0, cj - cprev);
-        if (c > 0)
-          depth = cj;
-      }
+      const int b = A.src_off[s], e = b + A.nsrc[s];
+      int cprev, cj;
+      pair_levels(A, j, b, e, cprev, cj);
+      c = max(0, cj - cprev);
+      if (c > 0)
+        depth = max(depth, cj);
     }
     cnt[j] = c;
   }
@@ -103,12 +99,8 @@ __global__ void k_tree_count(TreeArgs A, int32_t n_all, int32_t *__restrict__ cn
     atomicMax(maxdepth, depth);
 }
 
-__global__ void k_tree_emit(TreeArgs A)
+HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
 {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int S = *A.s_total;
-  if (j >= S)
-    return;
   const int p = A.src[j];
   const int s = A.src_sub[j];
   const int P = j + A.cscan[j];
@@ -173,6 +165,13 @@ __global__ void k_tree_emit(TreeArgs A)
     A.ent_depth[pos] = (int8_t)l;
     L = (L >= A.P.res) ? L * 0.5 : L;
   }
+}
+
+__global__ void k_tree_emit(TreeArgs A)
+{
+  const int S = *A.s_total;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < S; j += gridDim.x * blockDim.x)
+    tree_emit_one(A, j);
 }
 
 // Mass and centre of mass of every cell at one depth from its children, in

@@ -161,62 +161,73 @@ __global__ void k_mark_candidates(SubArrays S, PartArrays Q, int64_t n, int32_t 
 
 // Source flags: full mode -> candidates with rank < nsrc; correction -> removed
 // in the previous round.  Also the self mass used to cancel the self term.
-__global__ void k_mark_sources(SubArrays S, PartArrays Q, DevParams P, int64_t n, int round, const int32_t *crank,
-                               int32_t *flag)
+__global__ void k_mark_sources(SubArrays S, PartArrays Q, DevParams P, const int4 *tiles, const int32_t *ntiles_p,
+                               int round, const int32_t *crank, int32_t *flag)
 {
-  GRID_STRIDE(p, n)
+  TILE_LOOP(tiles, ntiles_p)
   {
-    const int s = S.sub_of[p];
+    const int4 tl = tiles[t_];
+    const int s = tl.x;
     const int md = S.mode[s];
-    int f = 0;
-    float sm = 0.f;
-    if (md == MODE_FULL && Q.state[p] == 0 && S.warm[s] == 1)
+    const int warm = S.warm[s];
+    const int nsrc = S.nsrc[s];
+    const int rank0 = crank ? crank[S.off[s]] : 0;
+    for (int p = tl.y + threadIdx.x; p < tl.z; p += blockDim.x)
     {
-      if (Q.proposal[p])
+      int f = 0;
+      float sm = 0.f;
+      const int st = Q.state[p];
+      if (md == MODE_FULL && st == 0 && warm == 1)
       {
-        f = 1;
-        sm = Q.posm[p].w;
+        if (Q.proposal[p])
+        {
+          f = 1;
+          sm = Q.posm[p].w;
+        }
       }
-    }
-    else if (md == MODE_FULL && Q.state[p] == 0)
-    {
-      const int rank = crank[p] - crank[S.off[s]];
-      if (rank < S.nsrc[s])
+      else if (md == MODE_FULL && st == 0)
       {
-        f = 1;
-        const float m = Q.posm[p].w;
-        const float *up = S.upscale + 6 * s;
-        const int t = ptype_of(Q, p);
-        sm = P.dm_only ? m * up[1] : (donot_subsample(P, t) ? m : m * up[t]);
+        // candidate rank in E-list order (only with subsampling; else every candidate is a source)
+        const int rank = crank ? crank[p] - rank0 : 0;
+        if (rank < nsrc)
+        {
+          f = 1;
+          const float m = Q.posm[p].w;
+          const float *up = S.upscale + 6 * s;
+          const int t = ptype_of(Q, p);
+          sm = P.dm_only ? m * up[1] : (donot_subsample(P, t) ? m : m * up[t]);
+        }
       }
+      else if (md == MODE_CORR && st == round - 1)
+        // after a warm round only the proposed particles were sources
+        f = (round == 2 && warm == 2 && Q.proposal) ? (int)Q.proposal[p] : 1;
+      flag[p] = f;
+      Q.selfm[p] = sm;
     }
-    else if (md == MODE_CORR && Q.state[p] == round - 1)
-      // after a warm round only the proposed particles were sources
-      f = (round == 2 && S.warm[s] == 2 && Q.proposal) ? (int)Q.proposal[p] : 1;
-    flag[p] = f;
-    Q.selfm[p] = sm;
   }
 }
 
-__global__ void k_scatter_sources(SubArrays S, int64_t n, const int32_t *flag, const int32_t *fscan, int32_t *src,
-                                  int32_t *src_sub)
+__global__ void k_scatter_sources(SubArrays S, const int4 *tiles, const int32_t *ntiles_p, const int32_t *flag,
+                                  const int32_t *fscan, int32_t *src, int32_t *src_sub)
 {
-  GRID_STRIDE(p, n)
+  TILE_LOOP(tiles, ntiles_p)
   {
-    if (flag[p])
-    {
-      src[fscan[p]] = (int32_t)p;
-      src_sub[fscan[p]] = S.sub_of[p];
-    }
+    const int4 tl = tiles[t_];
+    for (int p = tl.y + threadIdx.x; p < tl.z; p += blockDim.x)
+      if (flag[p])
+      {
+        src[fscan[p]] = (int32_t)p;
+        src_sub[fscan[p]] = tl.x;
+      }
   }
 }
 
-__global__ void k_sub_src(SubArrays S, const int32_t *fscan)
+__global__ void k_sub_src(SubArrays S, const int32_t *sub_first)
 {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S.nsub)
     return;
-  S.src_off[s] = fscan[S.off[s]];
+  S.src_off[s] = (S.mode[s] != MODE_IDLE) ? sub_first[s] : 0;
   const uint32_t big = float_to_ordered(3.0e38f), small = float_to_ordered(-3.0e38f);
   for (int k = 0; k < 3; k++)
   {
@@ -460,44 +471,51 @@ __global__ void k_items_emit(SubArrays S, const int32_t *order, const int32_t *i
 // Partition flags over slots: low 32 bits = bound, high 32 bits = bound tracer.
 // In a warm round only proposed particles may stay; bound particles outside the
 // proposal are removed with the unbound ones and counted in warm_np[s].
-__global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, int64_t n, int64_t *flag,
-                                  int32_t *warm_np)
+__global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, const int4 *tiles, const int32_t *ntiles_p,
+                                  int64_t *flag, int32_t *warm_np)
 {
-  GRID_STRIDE(q, n)
+  TILE_LOOP(tiles, ntiles_p)
   {
-    const int s = S.sub_of[q];
-    const int k = (int)(q - S.off[s]);
-    int64_t f = 0;
-    if (S.mode[s] != MODE_IDLE && k < S.ntgt[s])
+    const int4 tl = tiles[t_];
+    const int s = tl.x;
+    const int end = min(tl.z, S.off[s] + S.ntgt[s]);
+    const bool warm = warm_np && S.warm[s] == 1;
+    for (int q = tl.y + threadIdx.x; q < tl.z; q += blockDim.x)
     {
-      const int p = Q.cur[q];
-      if (Q.energy[p] < 0.f)
+      int64_t f = 0;
+      if (q < end)
       {
-        if (warm_np && S.warm[s] == 1 && !Q.proposal[p])
-          atomicAdd(warm_np + s, 1);
-        else
-          f = 1 | ((int64_t)(is_tracer(P, ptype_of(Q, p)) ? 1 : 0) << 32);
+        const int p = Q.cur[q];
+        if (Q.energy[p] < 0.f)
+        {
+          if (warm && !Q.proposal[p])
+            atomicAdd(warm_np + s, 1);
+          else
+            f = 1 | ((int64_t)(is_tracer(P, ptype_of(Q, p)) ? 1 : 0) << 32);
+        }
       }
+      flag[q] = f;
     }
-    flag[q] = f;
   }
 }
 
 // RemoveUnboundParticles (:35-44): stable compaction of the bound candidates;
 // removed particles are tagged with the round.
-__global__ void k_partition_scatter(SubArrays S, PartArrays Q, int64_t n, int round, const int64_t *fscan)
+__global__ void k_partition_scatter(SubArrays S, PartArrays Q, const int4 *tiles, const int32_t *ntiles_p, int round,
+                                    const int64_t *flag, const int64_t *fscan, const int64_t *sub_first)
 {
-  GRID_STRIDE(q, n)
+  TILE_LOOP(tiles, ntiles_p)
   {
-    const int s = S.sub_of[q];
-    const int k = (int)(q - S.off[s]);
-    if (S.mode[s] != MODE_IDLE && k < S.ntgt[s])
+    const int4 tl = tiles[t_];
+    const int s = tl.x;
+    const int end = min(tl.z, S.off[s] + S.ntgt[s]);
+    const int64_t first = sub_first[s];
+    for (int q = tl.y + threadIdx.x; q < end; q += blockDim.x)
     {
       const int p = Q.cur[q];
-      const int64_t f = fscan[q + 1] - fscan[q];
-      if (f & 0xffffffffll)
+      if (flag[q] & 0xffffffffll)
       {
-        const int r = (int)((fscan[q] - fscan[S.off[s]]) & 0xffffffffll);
+        const int r = (int)((fscan[q] - first) & 0xffffffffll);
         Q.cur2[S.off[s] + r] = p;
       }
       else
@@ -506,28 +524,27 @@ __global__ void k_partition_scatter(SubArrays S, PartArrays Q, int64_t n, int ro
   }
 }
 
-__global__ void k_copy_cur(SubArrays S, PartArrays Q, int64_t n, const int64_t *fscan)
+__global__ void k_copy_cur(SubArrays S, PartArrays Q, const int4 *tiles, const int32_t *ntiles_p,
+                           const int64_t *sub_total)
 {
-  GRID_STRIDE(q, n)
+  TILE_LOOP(tiles, ntiles_p)
   {
-    const int s = S.sub_of[q];
-    const int k = (int)(q - S.off[s]);
-    if (S.mode[s] != MODE_IDLE)
-    {
-      const int nb = (int)((fscan[S.off[s + 1]] - fscan[S.off[s]]) & 0xffffffffll);
-      if (k < nb)
-        Q.cur[q] = Q.cur2[q];
-    }
+    const int4 tl = tiles[t_];
+    const int s = tl.x;
+    const int nb = (int)(sub_total[s] & 0xffffffffll);
+    const int end = min(tl.z, S.off[s] + nb);
+    for (int q = tl.y + threadIdx.x; q < end; q += blockDim.x)
+      Q.cur[q] = Q.cur2[q];
   }
 }
 
 // Disruption test and correction decision (:625-683).
-__global__ void k_round_decide(SubArrays S, DevParams P, const int64_t *fscan, const int32_t *warm_np)
+__global__ void k_round_decide(SubArrays S, DevParams P, const int64_t *sub_total, const int32_t *warm_np)
 {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S.nsub || S.mode[s] == MODE_IDLE)
     return;
-  const int64_t d = fscan[S.off[s + 1]] - fscan[S.off[s]];
+  const int64_t d = sub_total[s];
   const int nb = (int)(d & 0xffffffffll);
   const int ntr = P.dm_only ? nb : (int)(d >> 32);
   const int nlast = S.nlast[s];
