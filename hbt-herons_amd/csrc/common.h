@@ -77,7 +77,7 @@ struct __attribute__((aligned(32))) Entry
   float crit2;
   int32_t skip;
   float len2;
-  int32_t pad;
+  int32_t rskip; // skip in bytes relative to the first entry of the entry's tree (the walk's stride)
 };
 
 // Loads through the constant address space: a wave-uniform index then becomes a
@@ -100,6 +100,27 @@ struct WEntry
   float x, y, z, m, crit2;
   int32_t skip;
 };
+// entry at byte offset off from a tree's first entry: one s_load_dwordx8 with the
+// offset in an SGPR
+struct REntry
+{
+  float x, y, z, m, crit2;
+  int32_t rskip;
+};
+HBTX_DEV REntry const_load_rentry(const Entry *tree, int32_t off)
+{
+  typedef const __attribute__((address_space(4))) char cc;
+  typedef const __attribute__((address_space(4))) Entry ce;
+  const ce *q = (const ce *)((const cc *)(uintptr_t)tree + off);
+  REntry e;
+  e.x = q->x;
+  e.y = q->y;
+  e.z = q->z;
+  e.m = q->m;
+  e.crit2 = q->crit2;
+  e.rskip = q->rskip;
+  return e;
+}
 HBTX_DEV WEntry const_load_entry(const Entry *p, int i)
 {
   typedef const __attribute__((address_space(4))) float cf;

@@ -14,6 +14,8 @@
 #include "learn.h"
 #include "tiles.h"
 
+#define WALK_MAX_CHUNKS 16
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -140,7 +142,8 @@ struct Batch
   bool has_prev = false;
   DArr<double> cube, len0, red, red2;
   DArr<uint32_t> bbox;
-  DArr<int32_t> misc; // [0] active, [1] work counter, [2] overflow, [3] entry total
+  DArr<int32_t> misc; // [0] active, [1] (unused), [2] overflow, [3] entry total, [4] max depth
+  DArr<int32_t> wctr; // walk work counters, 16 ints apart
   DArr<hbtx_sub_result> dres;
   bool ran = false;
 };
@@ -190,6 +193,7 @@ struct hbtx_ctx
   hipStream_t stream = nullptr;
   std::string err;
   int num_cu = 256;
+  int walk_chunks = 8; // item chunks of the walk (XCDs of the device; HBTX_WALK_CHUNKS overrides)
   // scratch
   DArr<int32_t> part32;
   DArr<int64_t> part64;
@@ -414,6 +418,7 @@ void batch_reserve(hbtx_ctx *c, Batch &B)
   B.red2.alloc((int64_t)RED_NV * ns + RED_NV);
   B.bbox.alloc(6 * (int64_t)ns + 6);
   B.misc.alloc(8);
+  B.wctr.alloc(16 * WALK_MAX_CHUNKS);
   B.dres.alloc(ns + 1);
   c->stats.alloc(NSTATS);
   {
@@ -643,7 +648,8 @@ WalkArgs walk_args(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   WalkArgs W;
   W.items = B.items.p;
   W.nitems = B.iscan.p + V.nsub;
-  W.counter = B.misc.p + 1;
+  W.counter = B.wctr.p;
+  W.nchunks = c->walk_chunks;
   W.cur = V.Q.cur;
   W.off = V.S.off;
   W.ntgt = V.S.ntgt;
@@ -673,7 +679,7 @@ void round_walk(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   scan32(c, B.icnt.p, B.iscan.p, V.nsub);
   k_items_emit<<<blocks_for((int64_t)V.nsub * HBTX_WAVE, 256, 1 << 24), 256, 0, st>>>(V.S, V.order, B.iscan.p,
                                                                                        B.items.p);
-  k_fill_i32<<<1, 64, 0, st>>>(B.misc.p + 1, 1, 0);
+  k_fill_i32<<<1, 256, 0, st>>>(B.wctr.p, 16 * WALK_MAX_CHUNKS, 0);
   WalkArgs W = walk_args(c, B, V, P);
   const int64_t maxitems = V.n / HBTX_WAVE + V.nsub + 1;
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxitems, 32 * c->num_cu));
@@ -1317,6 +1323,8 @@ int hbtx_create(hbtx_ctx **out, int device)
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     c->num_cu = std::max(1, prop.multiProcessorCount);
+    if (const char *e = getenv("HBTX_WALK_CHUNKS"))
+      c->walk_chunks = std::min(WALK_MAX_CHUNKS, std::max(1, atoi(e)));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));

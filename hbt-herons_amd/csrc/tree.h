@@ -106,18 +106,19 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
   const int P = j + A.cscan[j];
   const float4 x = A.posm[p];
   const float m = source_mass(A, p, s);
+  const int b = A.src_off[s], e = b + A.nsrc[s];
+  const int t0 = b + A.cscan[b]; // first entry of this subhalo's tree
   if (!A.treemode[s])
   {
     if (P < A.cap)
     {
-      A.ent[P] = Entry{x.x, x.y, x.z, m, 0.f, P + 1, 0.f, 0};
+      A.ent[P] = Entry{x.x, x.y, x.z, m, 0.f, P + 1, 0.f, (P + 1 - t0) * (int)sizeof(Entry)};
       A.ent_depth[P] = -1;
     }
     else
       *A.overflow = 1;
     return;
   }
-  const int b = A.src_off[s], e = b + A.nsrc[s];
   int cprev, cj;
   pair_levels(A, j, b, e, cprev, cj);
   const int nn = max(0, cj - cprev);
@@ -127,7 +128,7 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
     *A.overflow = 1;
     return;
   }
-  A.ent[q] = Entry{x.x, x.y, x.z, m, 0.f, q + 1, 0.f, 0};
+  A.ent[q] = Entry{x.x, x.y, x.z, m, 0.f, q + 1, 0.f, (q + 1 - t0) * (int)sizeof(Entry)};
   A.ent_depth[q] = -1;
   if (nn == 0)
     return;
@@ -161,7 +162,7 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
     const float len2 = lf * lf;
     // open if len^2 > r^2 theta^2 (src/gravity_tree.cpp:139)  <=>  r^2 < len^2 / theta^2
     const float crit2 = (A.P.theta2 > 0.f) ? len2 / A.P.theta2 : __int_as_float(0x7f800000);
-    A.ent[pos] = Entry{0.f, 0.f, 0.f, 0.f, crit2, skip, len2, 0};
+    A.ent[pos] = Entry{0.f, 0.f, 0.f, 0.f, crit2, skip, len2, (skip - t0) * (int)sizeof(Entry)};
     A.ent_depth[pos] = (int8_t)l;
     L = (L >= A.P.res) ? L * 0.5 : L;
   }

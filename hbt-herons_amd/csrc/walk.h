@@ -39,7 +39,8 @@ struct WalkArgs
 {
   const int2 *items;        // (subhalo, first target) per work item
   const int32_t *nitems;    // number of items (device)
-  int32_t *counter;         // work-stealing counter (zeroed before launch)
+  int32_t *counter;         // work-stealing counters, one per item chunk, 16 ints apart (zeroed before launch)
+  int32_t nchunks;          // item list split into this many contiguous chunks (1 = one shared counter)
   const int32_t *cur;       // slot-layout target order
   const int32_t *off;       // per subhalo slot offset
   const int32_t *ntgt;
@@ -88,20 +89,25 @@ struct WalkCounts
 
 // Walk the entries [eb, ee) for this lane's target, accumulating the pair terms
 // (-m/r outside h, the spline inside): plain FP32 sums over blocks of 16 steps,
-// Kahan-compensated across blocks.
+// Kahan-compensated across blocks.  The wave position is a byte offset from the
+// tree's first entry (an SGPR, the scalar load's offset); entries carry their
+// skip in the same units.
 template <bool PERIODIC, bool STATS>
 HBTX_DEV void tree_walk(const Entry *ent, int eb, int ee, bool valid, float px, float py, float pz,
                         const DevParams &P, Kahan &acc, WalkCounts &wc)
 {
-  int i = eb;
-  int myskip = valid ? eb : 0x7fffffff;
-  while (i < ee)
+  const Entry *tree = ent + eb;
+  const int32_t span = (ee - eb) * (int32_t)sizeof(Entry);
+  int32_t off = 0;
+  int32_t myskip = valid ? 0 : 0x7fffffff;
+  for (;;)
   {
     float blk = 0.f;
+#pragma unroll
     for (int k = 0; k < 16; k++)
     {
-      const WEntry e = const_load_entry(ent, i);
-      const bool active = i >= myskip;
+      const REntry e = const_load_rentry(tree, off);
+      const bool active = off >= myskip;
       const float r2 = sep2<PERIODIC>(e.x, e.y, e.z, px, py, pz, P);
       const bool open = r2 < e.crit2;
       const bool accept = active && !open;
@@ -115,7 +121,7 @@ HBTX_DEV void tree_walk(const Entry *ent, int eb, int ee, bool valid, float px, 
         term = (r2 < P.h2) ? tn : term;
       }
       blk += accept ? term : 0.f;
-      myskip = accept ? e.skip : myskip;
+      myskip = accept ? e.rskip : myskip;
       const bool anyopen = wave_mask(active && open) != 0ull;
       if (STATS)
       {
@@ -123,10 +129,14 @@ HBTX_DEV void tree_walk(const Entry *ent, int eb, int ee, bool valid, float px, 
         wc.visits += __popcll(wave_mask(active));
         wc.accepts += __popcll(wave_mask(accept));
       }
-      // entries of a subtree follow it, so skip > i; max() keeps the walk finite regardless
-      i = uniform(anyopen ? i + 1 : max(e.skip, i + 1));
-      if (i >= ee)
-        break;
+      // entries of a subtree follow it, so rskip > off; max() keeps the walk finite regardless
+      const int32_t next = off + (int32_t)sizeof(Entry);
+      off = uniform(anyopen ? next : max(e.rskip, next));
+      if (off >= span)
+      {
+        acc.add(blk);
+        return;
+      }
     }
     acc.add(blk);
   }
@@ -195,13 +205,34 @@ __global__ __launch_bounds__(64) void k_walk_energy(WalkArgs A)
   const int lane = lane_id();
   unsigned long long st_steps = 0, st_visits = 0, st_accepts = 0, st_pairs = 0, st_targets = 0, st_items = 0,
                      st_near = 0;
+  // XCD-aware work distribution: workgroups are dispatched round-robin over the
+  // XCDs, so workgroup b runs on XCD b % 8.  The item list (targets in key order,
+  // largest subhalo first) is cut into nchunks contiguous chunks; a wave drains the
+  // chunk of its XCD first, so neighbouring targets - which walk the same part of
+  // the tree - share that XCD's L2, then helps with the other chunks.
+  const int nit = *A.nitems;
+  const int nch = A.nchunks;
+  int chunk = blockIdx.x % nch;
+  int tried = 0;
   for (;;)
   {
-    int it = 0;
-    if (lane == 0)
-      it = atomicAdd(A.counter, 1);
-    it = uniform(__shfl(it, 0, HBTX_WAVE));
-    if (it >= *A.nitems)
+    int it = -1;
+    while (tried < nch)
+    {
+      const int cb = (int)((int64_t)nit * chunk / nch), ce = (int)((int64_t)nit * (chunk + 1) / nch);
+      int got = 0;
+      if (lane == 0)
+        got = atomicAdd(A.counter + 16 * chunk, 1);
+      got = uniform(__shfl(got, 0, HBTX_WAVE));
+      if (cb + got < ce)
+      {
+        it = cb + got;
+        break;
+      }
+      tried++;
+      chunk = (chunk + 1 == nch) ? 0 : chunk + 1;
+    }
+    if (it < 0)
       break;
     const int2 item = A.items[it];
     const int s = uniform(item.x);
