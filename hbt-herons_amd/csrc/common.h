@@ -75,9 +75,9 @@ struct __attribute__((aligned(32))) Entry
 {
   float x, y, z, m;
   float crit2;
-  int32_t skip;
-  float len2;
   int32_t rskip; // skip in bytes relative to the first entry of the entry's tree (the walk's stride)
+  float len2;
+  int32_t skip;
 };
 
 // Loads through the constant address space: a wave-uniform index then becomes a
@@ -107,18 +107,19 @@ struct REntry
   float x, y, z, m, crit2;
   int32_t rskip;
 };
-HBTX_DEV REntry const_load_rentry(const Entry *tree, int32_t off)
+HBTX_DEV REntry const_load_rentry(const Entry *tree, uint32_t off)
 {
+  typedef int v8i __attribute__((ext_vector_type(8)));
   typedef const __attribute__((address_space(4))) char cc;
-  typedef const __attribute__((address_space(4))) Entry ce;
-  const ce *q = (const ce *)((const cc *)(uintptr_t)tree + off);
+  typedef const __attribute__((address_space(4))) v8i cv;
+  const v8i raw = *(const cv *)((const cc *)(uintptr_t)tree + off); // all fields in one load
   REntry e;
-  e.x = q->x;
-  e.y = q->y;
-  e.z = q->z;
-  e.m = q->m;
-  e.crit2 = q->crit2;
-  e.rskip = q->rskip;
+  e.x = __int_as_float(raw[0]);
+  e.y = __int_as_float(raw[1]);
+  e.z = __int_as_float(raw[2]);
+  e.m = __int_as_float(raw[3]);
+  e.crit2 = __int_as_float(raw[4]);
+  e.rskip = raw[5];
   return e;
 }
 HBTX_DEV WEntry const_load_entry(const Entry *p, int i)

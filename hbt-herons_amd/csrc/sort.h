@@ -147,6 +147,12 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_offsets(const int32_t *_
   }
 }
 
+// One pass of the tiled LSD radix sort.  Wave w ranks elements [1024 w, 1024 w +
+// 1024) of the tile in order (ballot match of the digit bits, wave-private digit
+// counters in LDS), the tile is then reordered by digit in LDS, and threads write
+// it out in that order, so consecutive threads store to consecutive addresses of
+// each digit's run.
+#define SCAT_ITEMS (SORT_TILE / SORT_THREADS)
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *__restrict__ kin,
                                                                const int32_t *__restrict__ vin,
                                                                uint64_t *__restrict__ kout,
@@ -155,30 +161,35 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
                                                                const int32_t *__restrict__ ntiles_p, int shift,
                                                                const int32_t *__restrict__ ghist)
 {
-  __shared__ int32_t run[RADIX_BINS];
+  __shared__ uint64_t sk[SORT_TILE];
+  __shared__ int32_t sv[SORT_TILE];
   __shared__ int32_t wcnt[SORT_THREADS / HBTX_WAVE][RADIX_BINS];
+  __shared__ int32_t dbase[RADIX_BINS];
+  __shared__ int32_t scan_lds[SORT_THREADS / HBTX_WAVE];
   const int nt = *ntiles_p;
   const int tid = threadIdx.x;
   const int w = tid / HBTX_WAVE;
   const int lane = lane_id();
+  const int per_wave = SCAT_ITEMS * HBTX_WAVE;
   for (int t = blockIdx.x; t < nt; t += gridDim.x)
   {
-    run[tid] = 0;
+    const int4 tl = tiles[t];
+    const int n = tl.z - tl.y;
 #pragma unroll
     for (int k = 0; k < SORT_THREADS / HBTX_WAVE; k++)
       wcnt[k][tid] = 0;
     __syncthreads();
-    const int4 tl = tiles[t];
-    const int32_t *off = ghist + (int64_t)t * RADIX_BINS;
-    for (int r = 0; r < SORT_TILE / SORT_THREADS; r++)
+    uint64_t key[SCAT_ITEMS];
+    int32_t val[SCAT_ITEMS];
+    int32_t rank[SCAT_ITEMS];
+#pragma unroll
+    for (int r = 0; r < SCAT_ITEMS; r++)
     {
-      if (tl.y + r * SORT_THREADS >= tl.z)
-        break; // uniform across the block
-      const int i = tl.y + r * SORT_THREADS + tid;
-      const bool valid = i < tl.z;
-      const uint64_t k = valid ? kin[i] : 0ull;
-      const int32_t v = valid ? vin[i] : 0;
-      const int d = (int)((k >> shift) & (RADIX_BINS - 1));
+      const int li = w * per_wave + r * HBTX_WAVE + lane;
+      const bool valid = li < n;
+      key[r] = valid ? kin[tl.y + li] : 0ull;
+      val[r] = valid ? vin[tl.y + li] : 0;
+      const int d = (int)((key[r] >> shift) & (RADIX_BINS - 1));
       uint64_t m = ballot(valid);
 #pragma unroll
       for (int bit = 0; bit < RADIX_BITS; bit++)
@@ -187,30 +198,51 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
         const uint64_t bb = ballot(valid && on);
         m &= on ? bb : ~bb;
       }
-      const int lrank = __popcll(m & lanemask_lt());
+      const int base = valid ? wcnt[w][d] : 0;
+      rank[r] = base + __popcll(m & lanemask_lt());
+      // the highest lane of each digit group advances the wave's counter
       if (valid && (m >> lane) == 1ull)
-        wcnt[w][d] = __popcll(m);
-      __syncthreads();
-      if (valid)
-      {
-        int rank = run[d] + lrank;
-        for (int w2 = 0; w2 < w; w2++)
-          rank += wcnt[w2][d];
-        const int dst = off[d] + rank;
-        kout[dst] = k;
-        vout[dst] = v;
-      }
-      __syncthreads();
-      int acc = 0;
-#pragma unroll
-      for (int k2 = 0; k2 < SORT_THREADS / HBTX_WAVE; k2++)
-      {
-        acc += wcnt[k2][tid];
-        wcnt[k2][tid] = 0;
-      }
-      run[tid] += acc;
-      __syncthreads();
+        wcnt[w][d] = base + __popcll(m);
     }
+    __syncthreads();
+    {
+      // digit tid: offsets of the waves' runs and the first slot of the digit in the tile
+      int c[SORT_THREADS / HBTX_WAVE];
+      int tot = 0;
+#pragma unroll
+      for (int k = 0; k < SORT_THREADS / HBTX_WAVE; k++)
+      {
+        c[k] = wcnt[k][tid];
+        wcnt[k][tid] = tot;
+        tot += c[k];
+      }
+      int all;
+      dbase[tid] = block_excl_scan(tot, scan_lds, &all);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SCAT_ITEMS; r++)
+    {
+      const int li = w * per_wave + r * HBTX_WAVE + lane;
+      if (li < n)
+      {
+        const int d = (int)((key[r] >> shift) & (RADIX_BINS - 1));
+        const int pos = dbase[d] + wcnt[w][d] + rank[r];
+        sk[pos] = key[r];
+        sv[pos] = val[r];
+      }
+    }
+    __syncthreads();
+    const int32_t *off = ghist + (int64_t)t * RADIX_BINS;
+    for (int i = tid; i < n; i += SORT_THREADS)
+    {
+      const uint64_t k = sk[i];
+      const int d = (int)((k >> shift) & (RADIX_BINS - 1));
+      const int dst = off[d] + (i - dbase[d]);
+      kout[dst] = k;
+      vout[dst] = sv[i];
+    }
+    __syncthreads();
   }
 }
 

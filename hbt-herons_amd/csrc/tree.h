@@ -112,7 +112,7 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
   {
     if (P < A.cap)
     {
-      A.ent[P] = Entry{x.x, x.y, x.z, m, 0.f, P + 1, 0.f, (P + 1 - t0) * (int)sizeof(Entry)};
+      A.ent[P] = Entry{x.x, x.y, x.z, m, 0.f, (P + 1 - t0) * (int)sizeof(Entry), 0.f, P + 1};
       A.ent_depth[P] = -1;
     }
     else
@@ -128,7 +128,7 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
     *A.overflow = 1;
     return;
   }
-  A.ent[q] = Entry{x.x, x.y, x.z, m, 0.f, q + 1, 0.f, (q + 1 - t0) * (int)sizeof(Entry)};
+  A.ent[q] = Entry{x.x, x.y, x.z, m, 0.f, (q + 1 - t0) * (int)sizeof(Entry), 0.f, q + 1};
   A.ent_depth[q] = -1;
   if (nn == 0)
     return;
@@ -162,7 +162,7 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
     const float len2 = lf * lf;
     // open if len^2 > r^2 theta^2 (src/gravity_tree.cpp:139)  <=>  r^2 < len^2 / theta^2
     const float crit2 = (A.P.theta2 > 0.f) ? len2 / A.P.theta2 : __int_as_float(0x7f800000);
-    A.ent[pos] = Entry{0.f, 0.f, 0.f, 0.f, crit2, skip, len2, (skip - t0) * (int)sizeof(Entry)};
+    A.ent[pos] = Entry{0.f, 0.f, 0.f, 0.f, crit2, (skip - t0) * (int)sizeof(Entry), len2, skip};
     A.ent_depth[pos] = (int8_t)l;
     L = (L >= A.P.res) ? L * 0.5 : L;
   }

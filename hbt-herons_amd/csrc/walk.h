@@ -100,17 +100,21 @@ HBTX_DEV void tree_walk(const Entry *ent, int eb, int ee, bool valid, float px, 
   const int32_t span = (ee - eb) * (int32_t)sizeof(Entry);
   int32_t off = 0;
   int32_t myskip = valid ? 0 : 0x7fffffff;
+  // (offsets stay below 2^31: a subhalo's tree holds fewer than 2^26 entries)
   for (;;)
   {
     float blk = 0.f;
 #pragma unroll
     for (int k = 0; k < 16; k++)
     {
-      const REntry e = const_load_rentry(tree, off);
+      const REntry e = const_load_rentry(tree, (uint32_t)off);
       const bool active = off >= myskip;
       const float r2 = sep2<PERIODIC>(e.x, e.y, e.z, px, py, pz, P);
       const bool open = r2 < e.crit2;
       const bool accept = active && !open;
+      // lane masks straight from the compares (no round trip through a VGPR)
+      const bool anyopen = (__builtin_amdgcn_sicmp(off, myskip, 39 /* signed >= */) &
+                            __builtin_amdgcn_fcmpf(r2, e.crit2, 4 /* ordered < */)) != 0ull;
       const float rinv = __builtin_amdgcn_rsqf(r2);
       float term = -e.m * rinv;
       if (wave_mask(accept && r2 < P.h2))
@@ -122,7 +126,6 @@ HBTX_DEV void tree_walk(const Entry *ent, int eb, int ee, bool valid, float px, 
       }
       blk += accept ? term : 0.f;
       myskip = accept ? e.rskip : myskip;
-      const bool anyopen = wave_mask(active && open) != 0ull;
       if (STATS)
       {
         wc.steps++;
