@@ -233,6 +233,7 @@ def main():
                    "softening": 0.001, "bound_mass_precision": 0.995, "max_sample_size": 0,
                    "direct_cutoff": int(args.direct_cutoff), "seed": args.seed, "snapshot": args.snapshot,
                    "trained_on_snapshots": f"0-{args.snapshot - 1}", "threshold": args.threshold,
+                   "tuning": ctx.tuning(),
                    "rounds": rounds, "bound_particles": int(res["nbound"].sum()),
                    "disrupted": int(res["disrupted"].sum()), "proposed": int(npro)},
         "roofline": {"bound": "valu", "kernel": "k_walk_energy", "achieved": achieved,

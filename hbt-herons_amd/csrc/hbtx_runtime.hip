@@ -18,6 +18,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <dlfcn.h>
+#include <fstream>
+#include <sstream>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -193,7 +196,9 @@ struct hbtx_ctx
   hipStream_t stream = nullptr;
   std::string err;
   int num_cu = 256;
-  int walk_chunks = 8; // item chunks of the walk (XCDs of the device; HBTX_WALK_CHUNKS overrides)
+  int walk_chunks = 8;          // item chunks of the walk (one per XCD)
+  int64_t direct_cutoff = 1024; // sources at or below use the exact sum when params ask for the tuned value
+  std::string tune_source = "built-in defaults";
   // scratch
   DArr<int32_t> part32;
   DArr<int64_t> part64;
@@ -348,12 +353,42 @@ DevParams make_dev_params(const hbtx_params &p)
 }
 
 // default direct/tree cutoff when the caller asks for the tuned value
-int64_t tuned_direct_cutoff()
+// ---------------------------------------------------------------------------
+// Tuning: the direct/tree cutoff and the walk's item chunking.  Defaults are
+// replaced by the autotuner's file (HBTX_TUNE_FILE, else hbtx_tune.json next to
+// this library; written by hbt-herons_amd/autotune.py on the target GPU), then
+// by the environment (HBTX_DIRECT_CUTOFF, HBTX_WALK_CHUNKS), then by
+// hbtx_set_tuning.
+// ---------------------------------------------------------------------------
+bool json_int(const std::string &txt, const char *name, int64_t &out)
 {
-  const char *e = getenv("HBTX_DIRECT_CUTOFF");
-  if (e)
-    return atoll(e);
-  return 1024;
+  const std::string key = std::string("\"") + name + "\"";
+  const size_t k = txt.find(key);
+  if (k == std::string::npos)
+    return false;
+  const size_t c = txt.find(':', k + key.size());
+  if (c == std::string::npos)
+    return false;
+  char *end = nullptr;
+  const long long v = strtoll(txt.c_str() + c + 1, &end, 10);
+  if (end == txt.c_str() + c + 1)
+    return false;
+  out = v;
+  return true;
+}
+
+std::string tune_file_path()
+{
+  if (const char *e = getenv("HBTX_TUNE_FILE"))
+    return e;
+  Dl_info info;
+  if (dladdr((void *)&tune_file_path, &info) && info.dli_fname)
+  {
+    std::string lib = info.dli_fname;
+    const size_t slash = lib.find_last_of('/');
+    return (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/hbtx_tune.json";
+  }
+  return "hbtx_tune.json";
 }
 
 void batch_reserve(hbtx_ctx *c, Batch &B)
@@ -1064,7 +1099,7 @@ void run_batch(hbtx_ctx *c, const hbtx_params &prm, uint32_t flags, double *elap
   REQUIRE(B.n > 0 || B.nsub > 0, HBTX_E_INVALID, "no batch loaded");
   hbtx_params p = prm;
   if (p.direct_cutoff < 0)
-    p.direct_cutoff = tuned_direct_cutoff();
+    p.direct_cutoff = c->direct_cutoff;
   REQUIRE(p.max_sample_size >= 0, HBTX_E_INVALID, "MaxSampleSizeOfPotentialEstimate must be >= 0");
   const bool warm = (flags & HBTX_WARMSTART) != 0;
   REQUIRE(!warm || B.has_proposal, HBTX_E_INVALID, "warm start needs a proposal (hbtx_batch_set_proposal / propose)");
@@ -1305,6 +1340,38 @@ void hbtx_default_params(hbtx_params *p)
   p->direct_cutoff = -1;
 }
 
+} // extern "C"
+
+void load_tuning(hbtx_ctx *c)
+{
+  const std::string path = tune_file_path();
+  std::ifstream f(path);
+  if (f)
+  {
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string txt = ss.str();
+    int64_t v;
+    if (json_int(txt, "direct_cutoff", v) && v >= 0)
+      c->direct_cutoff = v;
+    if (json_int(txt, "walk_chunks", v) && v >= 1)
+      c->walk_chunks = (int)std::min<int64_t>(WALK_MAX_CHUNKS, v);
+    c->tune_source = path;
+  }
+  if (const char *e = getenv("HBTX_DIRECT_CUTOFF"))
+  {
+    c->direct_cutoff = std::max<int64_t>(0, atoll(e));
+    c->tune_source = "environment";
+  }
+  if (const char *e = getenv("HBTX_WALK_CHUNKS"))
+  {
+    c->walk_chunks = std::min(WALK_MAX_CHUNKS, std::max(1, atoi(e)));
+    c->tune_source = "environment";
+  }
+}
+
+extern "C" {
+
 int hbtx_create(hbtx_ctx **out, int device)
 {
   if (!out)
@@ -1323,8 +1390,7 @@ int hbtx_create(hbtx_ctx **out, int device)
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     c->num_cu = std::max(1, prop.multiProcessorCount);
-    if (const char *e = getenv("HBTX_WALK_CHUNKS"))
-      c->walk_chunks = std::min(WALK_MAX_CHUNKS, std::max(1, atoi(e)));
+    load_tuning(c);
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
@@ -1943,6 +2009,31 @@ int hbtx_batch_propose(hbtx_ctx *c, hbtx_mlp *M, float threshold, int64_t *nprop
       *nproposed = cnt;
     }
   });
+}
+
+} // extern "C"
+
+extern "C" {
+
+int hbtx_get_tuning(hbtx_ctx *c, int64_t *direct_cutoff, int32_t *walk_chunks)
+{
+  if (!c)
+    return HBTX_E_INVALID;
+  if (direct_cutoff)
+    *direct_cutoff = c->direct_cutoff;
+  if (walk_chunks)
+    *walk_chunks = c->walk_chunks;
+  return HBTX_OK;
+}
+
+int hbtx_set_tuning(hbtx_ctx *c, int64_t direct_cutoff, int32_t walk_chunks)
+{
+  if (!c || direct_cutoff < 0 || walk_chunks < 1 || walk_chunks > WALK_MAX_CHUNKS)
+    return HBTX_E_INVALID;
+  c->direct_cutoff = direct_cutoff;
+  c->walk_chunks = walk_chunks;
+  c->tune_source = "hbtx_set_tuning";
+  return HBTX_OK;
 }
 
 } // extern "C"

@@ -152,6 +152,8 @@ def lib():
         L.hbtx_batch_features.argtypes = [P, C.POINTER(Params), P]
         L.hbtx_batch_propose.argtypes = [P, P, C.c_float, C.POINTER(i64), C.POINTER(C.c_double)]
         L.hbtx_batch_get_proposal.argtypes = [P, P]
+        L.hbtx_get_tuning.argtypes = [P, C.POINTER(i64), C.POINTER(C.c_int32)]
+        L.hbtx_set_tuning.argtypes = [P, i64, C.c_int32]
         _lib = L
     return _lib
 
@@ -236,6 +238,16 @@ class Context:
                                             _p(pop.ref_pos), _p(pop.ref_vel), _p(_f32(mb_pos)), _p(_f32(mb_vel)),
                                             flags, _p(order), _p(energy), _p(pot), _p(res)))
         return dict(order=order, energy=energy, potential=pot, result=res)
+
+    # --- tuning ------------------------------------------------------------------
+    def tuning(self):
+        dc = C.c_int64(0)
+        wc = C.c_int32(0)
+        self._check(lib().hbtx_get_tuning(self._h, C.byref(dc), C.byref(wc)))
+        return dict(direct_cutoff=dc.value, walk_chunks=wc.value)
+
+    def set_tuning(self, direct_cutoff, walk_chunks):
+        self._check(lib().hbtx_set_tuning(self._h, int(direct_cutoff), int(walk_chunks)))
 
     # --- learned warm start ----------------------------------------------------
     def set_previous(self, prev):

@@ -194,6 +194,15 @@ int hbtx_batch_propose(hbtx_ctx *ctx, hbtx_mlp *mlp, float threshold, int64_t *n
 /* Copy of the current proposal (uint8[n]). */
 int hbtx_batch_get_proposal(hbtx_ctx *ctx, uint8_t *proposal);
 
+/* Tuning of the batched path (no counterpart in the reference): the source count
+ * at or below which a subhalo takes the exact O(N^2) sum (used when
+ * hbtx_params.direct_cutoff is -1) and the number of contiguous item chunks of the
+ * walk's work distribution (one per XCD).  Loaded at hbtx_create from the
+ * autotuner's file (HBTX_TUNE_FILE, else hbtx_tune.json beside the library), then
+ * HBTX_DIRECT_CUTOFF / HBTX_WALK_CHUNKS. */
+int hbtx_get_tuning(hbtx_ctx *ctx, int64_t *direct_cutoff, int32_t *walk_chunks);
+int hbtx_set_tuning(hbtx_ctx *ctx, int64_t direct_cutoff, int32_t walk_chunks);
+
 /* Segmented sort of uint64 keys with int32 payloads (used by the tree build and
  * the final binding-energy order; exported for tests). */
 int hbtx_segmented_sort(hbtx_ctx *ctx, int64_t n, uint64_t *keys, int32_t *vals, int64_t nseg,
