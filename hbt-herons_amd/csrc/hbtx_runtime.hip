@@ -198,6 +198,7 @@ struct hbtx_ctx
   int num_cu = 256;
   int walk_chunks = 8;          // item chunks of the walk (one per XCD)
   int64_t direct_cutoff = 1024; // sources at or below use the exact sum when params ask for the tuned value
+  int walk_waves = 32;          // resident walk waves per CU (grid of the persistent walk)
   std::string tune_source = "built-in defaults";
   // scratch
   DArr<int32_t> part32;
@@ -717,7 +718,7 @@ void round_walk(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   k_fill_i32<<<1, 256, 0, st>>>(B.wctr.p, 16 * WALK_MAX_CHUNKS, 0);
   WalkArgs W = walk_args(c, B, V, P);
   const int64_t maxitems = V.n / HBTX_WAVE + V.nsub + 1;
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxitems, 32 * c->num_cu));
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxitems, (int64_t)c->walk_waves * c->num_cu));
   HIPCHK(hipEventRecord(c->wev0, st));
   if (P.periodic)
   {
@@ -1356,6 +1357,8 @@ void load_tuning(hbtx_ctx *c)
       c->direct_cutoff = v;
     if (json_int(txt, "walk_chunks", v) && v >= 1)
       c->walk_chunks = (int)std::min<int64_t>(WALK_MAX_CHUNKS, v);
+    if (json_int(txt, "walk_waves", v) && v >= 1)
+      c->walk_waves = (int)std::min<int64_t>(64, v);
     c->tune_source = path;
   }
   if (const char *e = getenv("HBTX_DIRECT_CUTOFF"))
@@ -1366,6 +1369,11 @@ void load_tuning(hbtx_ctx *c)
   if (const char *e = getenv("HBTX_WALK_CHUNKS"))
   {
     c->walk_chunks = std::min(WALK_MAX_CHUNKS, std::max(1, atoi(e)));
+    c->tune_source = "environment";
+  }
+  if (const char *e = getenv("HBTX_WALK_WAVES"))
+  {
+    c->walk_waves = std::min(64, std::max(1, atoi(e)));
     c->tune_source = "environment";
   }
 }
