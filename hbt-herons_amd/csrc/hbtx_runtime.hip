@@ -147,6 +147,7 @@ struct Batch
   DArr<uint32_t> bbox;
   DArr<int32_t> misc; // [0] active, [1] (unused), [2] overflow, [3] entry total, [4] max depth
   DArr<int32_t> wctr; // walk work counters, 16 ints apart
+  DArr<int32_t> cells, depth_cnt; // tree cells bucketed by depth; counts, offsets, cursors
   DArr<hbtx_sub_result> dres;
   bool ran = false;
 };
@@ -285,8 +286,8 @@ void seg_sort(hbtx_ctx *c, uint64_t *keys, int32_t *vals, int32_t nseg, const in
     uint64_t *kout = (pass & 1) ? keys : c->kalt.p;
     int32_t *vout = (pass & 1) ? vals : c->valt.p;
     k_radix_hist<<<grid, SORT_THREADS, 0, c->stream>>>(kin, c->sort_tiles.p, ntiles_p, shift, c->ghist.p);
-    k_radix_offsets<<<std::max(1, std::min(nseg, 4096)), SORT_THREADS, 0, c->stream>>>(
-      seg_begin, seg_len, c->sort_toff.p, c->sort_ntile.p, nseg, c->ghist.p);
+    k_radix_offsets<<<grid, SORT_THREADS, 0, c->stream>>>(seg_begin, c->sort_tiles.p, ntiles_p, c->sort_toff.p,
+                                                          c->sort_ntile.p, c->ghist.p);
     k_radix_scatter<<<grid, SORT_THREADS, 0, c->stream>>>(kin, vin, kout, vout, c->sort_tiles.p, ntiles_p, shift,
                                                           c->ghist.p);
   }
@@ -674,8 +675,17 @@ void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, cons
   k_tree_emit<<<blocks_for(n, 256, 8 * 1024), 256, 0, st>>>(T);
   k_sub_entries<<<gs, 256, 0, st>>>(V.S, B.cscan.p, B.ent_cap);
   k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, B.ent_cap, B.misc.p + 2);
+  // moments bottom-up, one pass per depth over that depth's cells
+  B.cells.alloc(std::max<int64_t>(B.ent_cap, 1));
+  B.depth_cnt.alloc(3 * DEPTH_BINS + 1);
+  int32_t *dcnt = B.depth_cnt.p, *doff = dcnt + DEPTH_BINS, *dcur = doff + DEPTH_BINS + 1;
+  const int eg = blocks_for(B.ent_cap, 256, 8192);
+  k_fill_i32<<<1, 64, 0, st>>>(dcnt, DEPTH_BINS, 0);
+  k_depth_count<<<eg, 256, 0, st>>>(B.ent_depth.p, B.misc.p + 3, dcnt);
+  k_depth_offsets<<<1, 64, 0, st>>>(dcnt, doff, dcur);
+  k_depth_scatter<<<eg, 256, 0, st>>>(B.ent_depth.p, B.misc.p + 3, doff, dcur, B.cells.p);
   for (int d = maxdepth; d >= 0; d--)
-    k_tree_moments<<<blocks_for(2 * n / 16 + 1, 256, 8192), 256, 0, st>>>(B.ent.p, B.ent_depth.p, B.misc.p + 3, d);
+    k_tree_moments<<<blocks_for(n / 4 + 1, 256, 8192), 256, 0, st>>>(B.ent.p, B.cells.p, doff, B.misc.p + 3, d);
   HIPCHK(hipGetLastError());
 }
 

@@ -117,32 +117,55 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint64_t *__r
   }
 }
 
-// Per large segment: turn tile histograms into global scatter bases.
+// Per large segment: turn tile histograms into global scatter bases.  Driven by
+// the radix tiles (a block takes the segments whose first tile it meets), so
+// launches with no large segment cost nothing; each digit's column of tile counts
+// is read 8 tiles at a time.
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_offsets(const int32_t *__restrict__ seg_begin,
-                                                               const int32_t *__restrict__ seg_len,
+                                                               const int4 *__restrict__ tiles,
+                                                               const int32_t *__restrict__ ntiles_p,
                                                                const int32_t *__restrict__ tile_off,
-                                                               const int32_t *__restrict__ ntile, int32_t nseg,
+                                                               const int32_t *__restrict__ ntile,
                                                                int32_t *__restrict__ ghist)
 {
   __shared__ int32_t lds[SORT_THREADS / HBTX_WAVE];
   const int d = threadIdx.x;
-  for (int s = blockIdx.x; s < nseg; s += gridDim.x)
+  const int nt_all = *ntiles_p;
+  for (int t = blockIdx.x; t < nt_all; t += gridDim.x)
   {
-    if (seg_len[s] <= SORT_SMALL_MAX)
-      continue;
+    const int4 tl = tiles[t];
+    const int s = tl.x;
+    if (tl.y != seg_begin[s])
+      continue; // not the segment's first tile
     const int t0 = tile_off[s], nt = ntile[s];
     int run = 0;
-    for (int t = t0; t < t0 + nt; t++)
+    for (int u = t0; u < t0 + nt; u += 8)
     {
-      const int64_t q = (int64_t)t * RADIX_BINS + d;
-      const int c = ghist[q];
-      ghist[q] = run;
-      run += c;
+      int c[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        c[k] = (u + k < t0 + nt) ? ghist[(int64_t)(u + k) * RADIX_BINS + d] : 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+      {
+        if (u + k < t0 + nt)
+          ghist[(int64_t)(u + k) * RADIX_BINS + d] = run;
+        run += c[k];
+      }
     }
     int tot;
     const int base = block_excl_scan(run, lds, &tot) + seg_begin[s];
-    for (int t = t0; t < t0 + nt; t++)
-      ghist[(int64_t)t * RADIX_BINS + d] += base;
+    for (int u = t0; u < t0 + nt; u += 8)
+    {
+      int c[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        c[k] = (u + k < t0 + nt) ? ghist[(int64_t)(u + k) * RADIX_BINS + d] : 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (u + k < t0 + nt)
+          ghist[(int64_t)(u + k) * RADIX_BINS + d] = c[k] + base;
+    }
     __syncthreads();
   }
 }

@@ -175,50 +175,103 @@ __global__ void k_tree_emit(TreeArgs A)
     tree_emit_one(A, j);
 }
 
+// Cells bucketed by depth, so that each depth's moment pass touches only its own
+// cells.  Pass 1 counts cells per depth (block histograms in LDS); after the
+// offsets, pass 2 writes every cell's index into its depth's bucket (a block
+// reserves one range per depth, threads take places through LDS counters).
+#define DEPTH_BINS 32
+__global__ __launch_bounds__(256) void k_depth_count(const int8_t *__restrict__ ent_depth,
+                                                    const int32_t *__restrict__ e_total_p, int32_t *__restrict__ cnt)
+{
+  __shared__ int32_t h[DEPTH_BINS];
+  if (threadIdx.x < DEPTH_BINS)
+    h[threadIdx.x] = 0;
+  __syncthreads();
+  const int E = *e_total_p;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < E; q += gridDim.x * blockDim.x)
+  {
+    const int d = ent_depth[q];
+    if (d >= 0)
+      atomicAdd(&h[d], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < DEPTH_BINS && h[threadIdx.x])
+    atomicAdd(cnt + threadIdx.x, h[threadIdx.x]);
+}
+
+// off[d] = first slot of depth d (exclusive prefix of cnt), cursor zeroed; one thread.
+__global__ void k_depth_offsets(const int32_t *__restrict__ cnt, int32_t *__restrict__ off,
+                                int32_t *__restrict__ cursor)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0)
+    return;
+  int run = 0;
+  for (int d = 0; d < DEPTH_BINS; d++)
+  {
+    off[d] = run;
+    run += cnt[d];
+    cursor[d] = 0;
+  }
+  off[DEPTH_BINS] = run;
+}
+
+__global__ __launch_bounds__(256) void k_depth_scatter(const int8_t *__restrict__ ent_depth,
+                                                      const int32_t *__restrict__ e_total_p,
+                                                      const int32_t *__restrict__ off, int32_t *__restrict__ cursor,
+                                                      int32_t *__restrict__ cells)
+{
+  __shared__ int32_t h[DEPTH_BINS], base[DEPTH_BINS];
+  const int E = *e_total_p;
+  const int stride = gridDim.x * blockDim.x;
+  // every block runs the same number of iterations (the barriers are uniform)
+  for (int q0 = blockIdx.x * blockDim.x; q0 < E; q0 += stride)
+  {
+    if (threadIdx.x < DEPTH_BINS)
+      h[threadIdx.x] = 0;
+    __syncthreads();
+    const int q = q0 + threadIdx.x;
+    const int d = (q < E) ? ent_depth[q] : -1;
+    int slot = 0;
+    if (d >= 0)
+      slot = atomicAdd(&h[d], 1);
+    __syncthreads();
+    if (threadIdx.x < DEPTH_BINS)
+      base[threadIdx.x] = h[threadIdx.x] ? atomicAdd(cursor + threadIdx.x, h[threadIdx.x]) : 0;
+    __syncthreads();
+    if (d >= 0)
+      cells[off[d] + base[d] + slot] = q;
+    __syncthreads();
+  }
+}
+
 // Mass and centre of mass of every cell at one depth from its children, in
 // child (sub-index) order with double accumulation of the float child values,
 // as UpdateInternalNodes/ProcessNode do (src/gravity_tree.cpp:25-80).
-__global__ void k_tree_moments(Entry *__restrict__ ent, const int8_t *__restrict__ ent_depth,
-                               const int32_t *__restrict__ e_total_p, int depth)
+__global__ void k_tree_moments(Entry *__restrict__ ent, const int32_t *__restrict__ cells,
+                               const int32_t *__restrict__ off, const int32_t *__restrict__ e_total_p, int depth)
 {
   const int E = *e_total_p;
-  // 16 depth bytes per thread and load; ent_depth is allocated in multiples of 16
-  for (int q0 = (blockIdx.x * blockDim.x + threadIdx.x) * 16; q0 < E; q0 += gridDim.x * blockDim.x * 16)
+  const int b = off[depth], e = off[depth + 1];
+  for (int k = b + blockIdx.x * blockDim.x + threadIdx.x; k < e; k += gridDim.x * blockDim.x)
   {
-    const int4 dv = *(const int4 *)(ent_depth + q0);
-    const int dw[4] = {dv.x, dv.y, dv.z, dv.w};
-    const uint32_t pat = 0x01010101u * (uint32_t)(uint8_t)depth;
-    bool any = false;
-    for (int k = 0; k < 4; k++)
-    {
-      const uint32_t x = (uint32_t)dw[k] ^ pat; // a zero byte marks a cell of this depth
-      any |= ((x - 0x01010101u) & ~x & 0x80808080u) != 0u;
-    }
-    if (!any)
-      continue;
-    for (int k = 0; k < 16; k++)
-    {
-    const int q = q0 + k;
-    if (q >= E || ent_depth[q] != depth)
-      continue;
+    const int q = cells[k];
     const int end = min(ent[q].skip, E);
     double m = 0., cx = 0., cy = 0., cz = 0.;
     int c = q + 1;
     while (c < end)
     {
-      const Entry e = ent[c];
-      const double cm = e.m;
+      const Entry en = ent[c];
+      const double cm = en.m;
       m += cm;
-      cx += (double)e.x * cm;
-      cy += (double)e.y * cm;
-      cz += (double)e.z * cm;
-      c = max(e.skip, c + 1);
+      cx += (double)en.x * cm;
+      cy += (double)en.y * cm;
+      cz += (double)en.z * cm;
+      c = max(en.skip, c + 1);
     }
     ent[q].x = (float)(cx / m);
     ent[q].y = (float)(cy / m);
     ent[q].z = (float)(cz / m);
     ent[q].m = (float)m;
-    }
   }
 }
 
