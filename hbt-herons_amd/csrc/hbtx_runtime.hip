@@ -999,7 +999,7 @@ int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, 
       // warm round: every target in key order of the proposal's tree cube
       k_warm_keys<<<gb, 256, 0, st>>>(V.S, V.Q, V.n, B.keys.p, B.seg_begin.p, B.seg_len.p);
       k_warm_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
-      seg_sort(c, B.keys.p, V.Q.cur, V.nsub, B.seg_begin.p, B.seg_len.p, V.n, 63);
+      seg_sort(c, B.keys.p, V.Q.cur, V.nsub, B.seg_begin.p, B.seg_len.p, V.n, 30);
     }
     round_walk(c, B, V, P);
     round_partition(c, B, V, P, round, warm && round == 1);

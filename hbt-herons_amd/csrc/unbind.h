@@ -638,7 +638,8 @@ __global__ void k_warm_reset_subs(SubArrays S, const float *ref_in)
   S.reset[s] = 0;
 }
 
-// Warm-round walk order: every target sorted by its key in the proposal's tree cube.
+// Warm-round walk order: every target sorted by its key in the proposal's tree cube
+// (top 10 levels).
 __global__ void k_warm_keys(SubArrays S, PartArrays Q, int64_t n, uint64_t *keys, int32_t *seg_begin, int32_t *seg_len)
 {
   GRID_STRIDE(q, n)
@@ -648,7 +649,9 @@ __global__ void k_warm_keys(SubArrays S, PartArrays Q, int64_t n, uint64_t *keys
     {
       const float4 x = Q.posm[Q.cur[q]];
       const double *c = S.cube + 4 * s;
-      keys[q] = octree_key(x.x, x.y, x.z, c[0], c[1], c[2], c[3]);
+      // the top 10 levels are enough to group targets for the walk (cells below
+      // 1/1024 of the cube hold particles closer than the softening)
+      keys[q] = octree_key(x.x, x.y, x.z, c[0], c[1], c[2], c[3]) >> (3 * (HBTX_KEY_LEVELS - 10));
     }
   }
 }
