@@ -37,6 +37,20 @@ METRIC = "particles unbound/s: batched+learned vs plain GPU at ≥99.9% bound-ma
 FP32_PEAK_TFLOPS = 157.3
 
 
+def walk_traffic():
+    """HBM bytes per walk launch on this workload, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (each scaled by a copy kernel's reading)."""
+    path = os.path.join(ROOT, "profiles", "pmc_k_walk_energy_evolved_5e7.md")
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return float(ln.split(":")[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -109,6 +123,8 @@ def main():
     ap.add_argument("--state", default=None,
                     help="npz cache of the trained weights and the previous snapshot's outputs: written "
                          "after training, read instead of labelling and training when it exists")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every round's kernels directly instead of replaying the captured round graphs")
     ap.add_argument("--learned-only", action="store_true",
                     help="time only the learned step (for kernel profiles); implies --no-plain --no-cpu")
     args = ap.parse_args()
@@ -174,19 +190,20 @@ def main():
             np.savez(args.state, weights=net.weights()[0], **prev)
 
     # --- evaluated snapshot: batched without and with the learned warm start -----------
+    gflag = hbtx.NO_GRAPH if args.no_graph else 0
     ctx.load(pop)
     ctx.set_previous(prev)
     if args.learned_only:
         ms_plain_b, ref = [float("nan")], None
     else:
-        ms_plain_b = timed_steps(ctx, args.steps, args.warmup, lambda: ctx.run(prm))
+        ms_plain_b = timed_steps(ctx, args.steps, args.warmup, lambda: ctx.run(prm, gflag))
         ref = ctx.fetch()
         log(f"batched (no warm start): {np.mean(ms_plain_b):.2f} ms/step")
 
     def learned_step():
         ctx.features(prm, fetch=False)
         ctx.propose(net, args.threshold, count=False)
-        ctx.run(prm, hbtx.WARMSTART)
+        ctx.run(prm, hbtx.WARMSTART | gflag)
 
     ms_learned = timed_steps(ctx, args.steps, args.warmup, learned_step)
     st_time = ctx.stats()
@@ -214,6 +231,7 @@ def main():
     flops = walk_flops(st)
     achieved = flops / (walk_ms / 1e3) / 1e12 if walk_ms > 0 else 0.0
     res = warm["result"]
+    traffic, traffic_src = walk_traffic()
     line = {
         "metric": METRIC,
         "value": value,
@@ -233,12 +251,13 @@ def main():
                    "softening": 0.001, "bound_mass_precision": 0.995, "max_sample_size": 0,
                    "direct_cutoff": int(args.direct_cutoff), "seed": args.seed, "snapshot": args.snapshot,
                    "trained_on_snapshots": f"0-{args.snapshot - 1}", "threshold": args.threshold,
-                   "tuning": ctx.tuning(),
+                   "tuning": ctx.tuning(), "round_graphs": not args.no_graph,
                    "rounds": rounds, "bound_particles": int(res["nbound"].sum()),
                    "disrupted": int(res["disrupted"].sum()), "proposed": int(npro)},
         "roofline": {"bound": "valu", "kernel": "k_walk_energy", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": None, "kernel_ms_per_step": walk_ms, "launches_per_step": rounds,
+                     "traffic": traffic, "traffic_source": traffic_src, "kernel_ms_per_step": walk_ms,
+                     "launches_per_step": rounds,
                      "kernel_avg_launch_ms": walk_ms / max(1, rounds), "flop_per_step": flops},
         "bound_mass_mismatch_vs_plain": mismatch,
         "batched_without_learned": {"value": batched_value, "unit": "particles/s",

@@ -17,6 +17,7 @@
 #define WALK_MAX_CHUNKS 16
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <dlfcn.h>
 #include <fstream>
@@ -55,6 +56,11 @@ struct Error : public std::runtime_error
       throw Error(code, msg);                                                                                        \
   } while (0)
 
+// Advanced by every device allocation and release: kernels captured in a graph
+// hold buffer addresses, so a graph is valid only for the generation it was
+// captured in.
+std::atomic<uint64_t> g_alloc_gen{0};
+
 template <class T>
 struct DArr
 {
@@ -65,6 +71,7 @@ struct DArr
     if (p && count <= n)
       return;
     release();
+    g_alloc_gen++;
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
     hipError_t e = hipMalloc((void **)&p, bytes);
     if (e != hipSuccess)
@@ -78,7 +85,10 @@ struct DArr
   void release()
   {
     if (p)
+    {
       (void)hipFree(p);
+      g_alloc_gen++;
+    }
     p = nullptr;
     n = 0;
   }
@@ -126,6 +136,7 @@ struct Batch
   DArr<Entry> ent; // ent_cap + 1 entries (the walk may prefetch one past the end)
   DArr<int8_t> ent_depth;
   int32_t ent_cap = 0;
+  int64_t ent_bound = 0; // entries of the largest trees the batch's subhaloes can make
   DArr<int2> items;
   DArr<int32_t> icnt, iscan;
   // subhaloes
@@ -191,6 +202,22 @@ struct hbtx_mlp
   int64_t step = 0;
 };
 
+// What a captured round graph depends on besides buffer addresses.
+struct GraphKey
+{
+  uint64_t gen = ~0ull;
+  int64_t p0 = -1, n = -1;
+  int32_t s0 = -1, nsub = -1;
+  const void *ptype = nullptr, *proposal = nullptr;
+  int32_t ent_cap = -1;
+  DevParams P;
+  bool operator==(const GraphKey &o) const
+  {
+    return gen == o.gen && p0 == o.p0 && n == o.n && s0 == o.s0 && nsub == o.nsub && ptype == o.ptype &&
+           proposal == o.proposal && ent_cap == o.ent_cap && std::memcmp(&P, &o.P, sizeof(P)) == 0;
+  }
+};
+
 struct hbtx_ctx
 {
   int device = 0;
@@ -216,6 +243,11 @@ struct hbtx_ctx
   bool count_work = false;
   double stage_ms[8] = {0};
   int rounds = 0;
+  // round graphs: head (plan, sources, octree, walk items) and tail (partition,
+  // decision, frame), replayed around the directly launched walk
+  hipGraphExec_t g_head = nullptr, g_tail = nullptr;
+  GraphKey g_key;
+  int graph_rounds = 0, graph_captures = 0;
   Batch B;
 };
 
@@ -393,6 +425,23 @@ std::string tune_file_path()
   return "hbtx_tune.json";
 }
 
+// Most entries an octree over n sources can have: n particles, and at each of
+// the key's depths 0..HBTX_KEY_LEVELS at most min(8^depth, n/2) cells (a cell
+// holds >= 2 sources, and the cells of one depth are disjoint).  Every source
+// set of a subhalo is a subset of its particles.
+int64_t entry_bound(int64_t n)
+{
+  int64_t e = n, w = 1;
+  if (n < 2)
+    return e;
+  for (int l = 0; l <= HBTX_KEY_LEVELS; l++)
+  {
+    e += std::min(w, n / 2);
+    w = std::min<int64_t>(w * 8, (int64_t)1 << 40);
+  }
+  return e;
+}
+
 void batch_reserve(hbtx_ctx *c, Batch &B)
 {
   const int64_t n = B.n;
@@ -566,7 +615,24 @@ TreeArgs tree_args(Batch &B, View &V, const DevParams &P)
   return T;
 }
 
-void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, const int32_t *nt);
+void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, const int32_t *nt, bool graph);
+// device round counter (1 in the first round), advanced by k_round_begin
+int32_t *round_ptr(Batch &B)
+{
+  return B.misc.p + 5;
+}
+
+// entry buffers for at least `need` entries (slack: a quarter more, for the next rounds)
+void grow_entries(Batch &B, int64_t need, bool slack = true)
+{
+  const int64_t cap = std::min<int64_t>(slack ? need + need / 4 + 64 : need, (int64_t)0x7ffffff0);
+  REQUIRE(cap >= need, HBTX_E_CAPACITY, "tree entry count overflows int32");
+  B.ent.release();
+  B.ent_depth.release();
+  B.ent.alloc(cap + 1);
+  B.ent_depth.alloc((cap + 16) & ~(int64_t)15);
+  B.ent_cap = (int32_t)cap;
+}
 
 int64_t slot_tile_cap(const View &V)
 {
@@ -605,7 +671,7 @@ const T *tile_scan(hbtx_ctx *c, View &V, const int4 *tiles, const int32_t *nt, c
 }
 
 // Sources of the round -> dense list, then (tree-mode subhaloes) the octree.
-void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int round)
+void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool graph)
 {
   hipStream_t st = c->stream;
   const int64_t n = V.n;
@@ -625,14 +691,14 @@ void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, 
     k_upscale_finish<<<gs, 256, 0, st>>>(V.S, P, V.red2);
   }
   // (without subsampling the candidate rank is irrelevant: every candidate is a source)
-  k_mark_sources<<<tile_grid(c, V), 256, 0, st>>>(V.S, V.Q, P, B.stiles.p, nt, round,
+  k_mark_sources<<<tile_grid(c, V), 256, 0, st>>>(V.S, V.Q, P, B.stiles.p, nt, round_ptr(B),
                                                   P.mss > 0 ? B.crank.p : nullptr, B.flag.p);
-  build_sources_tree(c, B, V, P, nt);
+  build_sources_tree(c, B, V, P, nt, graph);
 }
 
 // Flags in B.flag (on the slots of the active tiles) -> dense source list, tree
 // cube, keys, sort, octree entries.
-void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, const int32_t *nt)
+void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, const int32_t *nt, bool graph)
 {
   hipStream_t st = c->stream;
   const int64_t n = V.n;
@@ -655,23 +721,25 @@ void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, cons
   k_fill_i32<<<1, 64, 0, st>>>(B.misc.p + 4, 1, 0);
   k_tree_count<<<blocks_for(n, 256, 8 * 1024), 256, 0, st>>>(T, B.cnt.p, B.misc.p + 4);
   scan32_dev(c, B.cnt.p, B.cscan.p, n, s_total);
-  // exact entry count and depth of this round's trees; grow the entry buffers if needed
-  k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, 0x7fffffff, B.misc.p + 2);
-  int32_t hm[2] = {0, 0};
-  HIPCHK(hipMemcpyAsync(hm, B.misc.p + 3, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  const int32_t etot = hm[0], maxdepth = std::min(hm[1], HBTX_KEY_LEVELS);
-  REQUIRE(etot >= 0, HBTX_E_CAPACITY, "tree entry count overflows int32");
-  if (etot > B.ent_cap)
+  int maxdepth = HBTX_KEY_LEVELS;
+  if (!graph)
   {
-    const int64_t cap = std::min<int64_t>((int64_t)etot + etot / 4 + 64, (int64_t)0x7ffffff0);
-    B.ent.release();
-    B.ent_depth.release();
-    B.ent.alloc(cap + 1);
-    B.ent_depth.alloc((cap + 16) & ~(int64_t)15);
-    B.ent_cap = (int32_t)cap;
-    T = tree_args(B, V, P);
+    // exact entry count and depth of this round's trees; grow the entry buffers if needed
+    k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, 0x7fffffff, B.misc.p + 2);
+    int32_t hm[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(hm, B.misc.p + 3, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const int32_t etot = hm[0];
+    maxdepth = std::min(hm[1], HBTX_KEY_LEVELS);
+    REQUIRE(etot >= 0, HBTX_E_CAPACITY, "tree entry count overflows int32");
+    if (etot > B.ent_cap)
+    {
+      grow_entries(B, etot);
+      T = tree_args(B, V, P);
+    }
   }
+  // (a captured round cannot wait for the count: its buffers are sized for the
+  // largest trees the batch can make, and the moments pass covers every depth)
   k_tree_emit<<<blocks_for(n, 256, 8 * 1024), 256, 0, st>>>(T);
   k_sub_entries<<<gs, 256, 0, st>>>(V.S, B.cscan.p, B.ent_cap);
   k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, B.ent_cap, B.misc.p + 2);
@@ -717,7 +785,8 @@ WalkArgs walk_args(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   return W;
 }
 
-void round_walk(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
+// the walk's work items (largest subhaloes first) and its cleared chunk counters
+void round_items(hbtx_ctx *c, Batch &B, View &V)
 {
   hipStream_t st = c->stream;
   const int gs = blocks_for(V.nsub, 256, 1 << 20);
@@ -726,6 +795,13 @@ void round_walk(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   k_items_emit<<<blocks_for((int64_t)V.nsub * HBTX_WAVE, 256, 1 << 24), 256, 0, st>>>(V.S, V.order, B.iscan.p,
                                                                                        B.items.p);
   k_fill_i32<<<1, 256, 0, st>>>(B.wctr.p, 16 * WALK_MAX_CHUNKS, 0);
+  HIPCHK(hipGetLastError());
+}
+
+// the persistent walk over the round's items, timed by the walk events
+void round_walk_launch(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
+{
+  hipStream_t st = c->stream;
   WalkArgs W = walk_args(c, B, V, P);
   const int64_t maxitems = V.n / HBTX_WAVE + V.nsub + 1;
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxitems, (int64_t)c->walk_waves * c->num_cu));
@@ -748,7 +824,13 @@ void round_walk(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   HIPCHK(hipGetLastError());
 }
 
-void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int round, bool warm_round)
+void round_walk(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
+{
+  round_items(c, B, V);
+  round_walk_launch(c, B, V, P);
+}
+
+void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool warm_round)
 {
   hipStream_t st = c->stream;
   const int64_t n = V.n;
@@ -763,7 +845,7 @@ void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int rou
                                         warm_round ? B.aux32.p : nullptr);
   tile_scan<int64_t>(c, V, B.stiles.p, nt, B.flag64.p, B.fscan64.p, B.tsum64.p, B.tpre64.p, B.sub_first64.p,
                      B.sub_total64.p, c->part64);
-  k_partition_scatter<<<tg, 256, 0, st>>>(V.S, V.Q, B.stiles.p, nt, round, B.flag64.p, B.fscan64.p,
+  k_partition_scatter<<<tg, 256, 0, st>>>(V.S, V.Q, B.stiles.p, nt, round_ptr(B), B.flag64.p, B.fscan64.p,
                                           B.sub_first64.p);
   k_copy_cur<<<tg, 256, 0, st>>>(V.S, V.Q, B.stiles.p, nt, B.sub_total64.p);
   if (P.mss > 0)
@@ -928,7 +1010,7 @@ void run_final(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot
     const int32_t *nt = build_slot_tiles(c, B, V);
     k_slot_of<<<gb, 256, 0, st>>>(V.S, n, V.fvals, V.Q.cur2);
     k_mark_refine<<<gb, 256, 0, st>>>(V.S, V.Q, n, V.Q.cur2, B.flag.p);
-    build_sources_tree(c, B, V, P, nt);
+    build_sources_tree(c, B, V, P, nt, false);
     round_walk(c, B, V, P);
     k_refine_keys<<<gb, 256, 0, st>>>(V.S, n, V.fvals, B.eref.p + V.p0, B.keys.p, B.seg_begin.p, B.seg_len.p);
     k_refine_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
@@ -966,7 +1048,9 @@ void k_final_ranges_all(hbtx_ctx *c, Batch &B, View &V)
 }
 
 // Iterate one view to convergence. Returns rounds used.
-int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, bool warm)
+// Per-view state at the start of the unbinding: particles, frames, the initial
+// Mbound (:537-541) and the leading do-not-subsample run (:522-524).
+void init_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool warm)
 {
   hipStream_t st = c->stream;
   const int gb = blocks_for(V.n);
@@ -974,36 +1058,146 @@ int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, 
   k_init_particles<<<gb, 256, 0, st>>>(V.S, V.Q, V.n);
   k_init_subs<<<gs, 256, 0, st>>>(V.S, B.ref_in.p + 6 * V.s0, B.mb_in.p + 6 * V.s0, nullptr);
   k_fill_i32<<<1, 64, 0, st>>>(B.misc.p, 8, 0);
-  {
-    // initial Mbound (:537-541) and the leading do-not-subsample run (:522-524)
-    k_final_ranges_all(c, B, V);
-    const int32_t *nt = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, V.n);
-    k_fill_i32<<<gs, 256, 0, st>>>(B.aux32.p, V.nsub, 0x7fffffff);
-    k_init_mass_tiles<<<red_grid(c, V.n, V.nsub), 256, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, nt, c->red_partial.p,
-                                                                B.aux32.p);
-    k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
-    k_init_mass_finish<<<gs, 256, 0, st>>>(V.S, P, V.red, B.aux32.p, warm ? 1 : 0);
-  }
+  k_final_ranges_all(c, B, V);
+  const int32_t *nt = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, V.n);
+  k_fill_i32<<<gs, 256, 0, st>>>(B.aux32.p, V.nsub, 0x7fffffff);
+  k_init_mass_tiles<<<red_grid(c, V.n, V.nsub), 256, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, nt, c->red_partial.p,
+                                                              B.aux32.p);
+  k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
+  k_init_mass_finish<<<gs, 256, 0, st>>>(V.S, P, V.red, B.aux32.p, warm ? 1 : 0);
   HIPCHK(hipGetLastError());
+}
+
+// First part of a round, up to the walk: plan, sources, octrees, work items.
+// graph: no host synchronisation (the part is being captured).
+void round_head(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool graph, bool warm_round)
+{
+  hipStream_t st = c->stream;
+  const int gs = blocks_for(V.nsub, 256, 1 << 20);
+  k_round_begin<<<1, 64, 0, st>>>(V.S.active_count, round_ptr(B));
+  k_plan<<<gs, 256, 0, st>>>(V.S, P, round_ptr(B));
+  round_sources_and_tree(c, B, V, P, graph);
+  if (warm_round)
+  {
+    // warm round: every target in key order of the proposal's tree cube
+    k_warm_keys<<<blocks_for(V.n), 256, 0, st>>>(V.S, V.Q, V.n, B.keys.p, B.seg_begin.p, B.seg_len.p);
+    k_warm_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
+    seg_sort(c, B.keys.p, V.Q.cur, V.nsub, B.seg_begin.p, B.seg_len.p, V.n, 30);
+  }
+  round_items(c, B, V);
+}
+
+void drop_graphs(hbtx_ctx *c)
+{
+  if (c->g_head)
+    (void)hipGraphExecDestroy(c->g_head);
+  if (c->g_tail)
+    (void)hipGraphExecDestroy(c->g_tail);
+  c->g_head = c->g_tail = nullptr;
+  c->g_key = GraphKey();
+}
+
+template <class F>
+hipGraphExec_t capture(hbtx_ctx *c, F &&record)
+{
+  hipStream_t st = c->stream;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  try
+  {
+    record();
+  }
+  catch (...)
+  {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(st, &g);
+    if (g)
+      (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    throw;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamEndCapture(st, &g));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIPCHK(e);
+  return ex;
+}
+
+// Round graphs for this view, captured again when a buffer moved or the view or
+// the parameters changed.  Capture only records: every buffer the round touches
+// already has its size from the first (direct) round.  Returns false (rounds
+// then run directly) if the capture could not be made.
+bool ensure_round_graphs(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
+{
+  GraphKey k;
+  k.p0 = V.p0;
+  k.n = V.n;
+  k.s0 = V.s0;
+  k.nsub = V.nsub;
+  k.ptype = V.Q.ptype;
+  k.proposal = V.Q.proposal;
+  k.ent_cap = B.ent_cap;
+  k.P = P;
+  k.gen = g_alloc_gen.load();
+  if (c->g_head && c->g_tail && k == c->g_key)
+    return true;
+  drop_graphs(c);
+  try
+  {
+    c->g_head = capture(c, [&] { round_head(c, B, V, P, true, false); });
+    c->g_tail = capture(c, [&] { round_partition(c, B, V, P, false); });
+  }
+  catch (const Error &)
+  {
+    drop_graphs(c);
+    return false;
+  }
+  if (g_alloc_gen.load() != k.gen)
+  {
+    drop_graphs(c);
+    return false;
+  }
+  c->g_key = k;
+  c->graph_captures++;
+  return true;
+}
+
+// Iterate one view to convergence. Returns rounds used.  graphs: rounds after
+// the first replay the captured head and tail around the walk.
+int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, bool warm, bool graphs)
+{
+  hipStream_t st = c->stream;
+  if (graphs)
+  {
+    // replayed rounds need entry buffers for the largest trees the view can make
+    if (B.ent_bound > 0x7ffffff0)
+      graphs = false;
+    else if (B.ent_cap < B.ent_bound)
+      grow_entries(B, B.ent_bound, false);
+  }
+  init_view(c, B, V, P, warm);
   int round = 0;
   const int max_rounds = 2000;
   while (true)
   {
     REQUIRE(round < max_rounds, HBTX_E_INVALID, "unbinding did not converge within the round limit");
     round++;
-    k_fill_i32<<<1, 64, 0, st>>>(V.S.active_count, 1, 0);
-    k_plan<<<gs, 256, 0, st>>>(V.S, P, round);
-    round_sources_and_tree(c, B, V, P, round);
-    if (warm && round == 1)
+    const bool replay = graphs && round >= 2 && ensure_round_graphs(c, B, V, P);
+    if (replay)
     {
-      // warm round: every target in key order of the proposal's tree cube
-      k_warm_keys<<<gb, 256, 0, st>>>(V.S, V.Q, V.n, B.keys.p, B.seg_begin.p, B.seg_len.p);
-      k_warm_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
-      seg_sort(c, B.keys.p, V.Q.cur, V.nsub, B.seg_begin.p, B.seg_len.p, V.n, 30);
+      HIPCHK(hipGraphLaunch(c->g_head, st));
+      round_walk_launch(c, B, V, P);
+      HIPCHK(hipGraphLaunch(c->g_tail, st));
+      c->graph_rounds++;
     }
-    round_walk(c, B, V, P);
-    round_partition(c, B, V, P, round, warm && round == 1);
-    int32_t h_misc[4];
+    else
+    {
+      round_head(c, B, V, P, false, warm && round == 1);
+      round_walk_launch(c, B, V, P);
+      round_partition(c, B, V, P, warm && round == 1);
+    }
+    int32_t h_misc[8];
     HIPCHK(hipMemcpyAsync(h_misc, B.misc.p, sizeof(h_misc), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     float wms = 0.f;
@@ -1041,6 +1235,9 @@ void upload_batch(hbtx_ctx *c, Batch &B, int64_t n, const float *pos, const floa
     B.h_count[s] = (int32_t)(offsets[s + 1] - offsets[s]);
     B.max_count = std::max(B.max_count, B.h_count[s]);
   }
+  B.ent_bound = 0;
+  for (int64_t s = 0; s < nsub; s++)
+    B.ent_bound += entry_bound(B.h_count[s]);
   if (id)
     B.h_id.assign(id, id + n);
   else
@@ -1120,6 +1317,8 @@ void run_batch(hbtx_ctx *c, const hbtx_params &prm, uint32_t flags, double *elap
   HIPCHK(hipMemsetAsync(c->stats.p, 0, NSTATS * sizeof(unsigned long long), st));
   c->walk_ms = 0.0;
   c->rounds = 0;
+  c->graph_rounds = 0;
+  c->graph_captures = 0;
   HIPCHK(hipEventRecord(c->ev0, st));
   const bool want_pot = (flags & HBTX_SAVE_POTENTIAL) != 0;
   c->count_work = (flags & HBTX_COUNT) != 0;
@@ -1128,13 +1327,13 @@ void run_batch(hbtx_ctx *c, const hbtx_params &prm, uint32_t flags, double *elap
     for (int32_t s = 0; s < B.nsub; s++)
     {
       View V = make_view(B, B.h_off[s], B.h_count[s], s, 1, false);
-      c->rounds += run_view(c, B, V, P, want_pot, warm);
+      c->rounds += run_view(c, B, V, P, want_pot, warm, false);
     }
   }
   else
   {
     View V = make_view(B, 0, B.n, 0, B.nsub, true);
-    c->rounds = run_view(c, B, V, P, want_pot, warm);
+    c->rounds = run_view(c, B, V, P, want_pot, warm, (flags & HBTX_NO_GRAPH) == 0);
   }
   HIPCHK(hipEventRecord(c->ev1, st));
   HIPCHK(hipEventSynchronize(c->ev1));
@@ -1200,9 +1399,10 @@ void build_tree(hbtx_ctx *c, const hbtx_params &prm, int64_t n, const float *pos
   hipStream_t st = c->stream;
   k_init_particles<<<blocks_for(n), 256, 0, st>>>(V.S, V.Q, n);
   k_init_subs<<<1, 64, 0, st>>>(V.S, B.ref_in.p, B.mb_in.p, nullptr);
-  k_fill_i32<<<1, 64, 0, st>>>(B.misc.p, 4, 0);
-  k_plan<<<1, 64, 0, st>>>(V.S, P, 1);
-  round_sources_and_tree(c, B, V, P, 1);
+  k_fill_i32<<<1, 64, 0, st>>>(B.misc.p, 8, 0);
+  k_round_begin<<<1, 64, 0, st>>>(V.S.active_count, round_ptr(B));
+  k_plan<<<1, 64, 0, st>>>(V.S, P, round_ptr(B));
+  round_sources_and_tree(c, B, V, P, false);
   int32_t h_misc[4];
   HIPCHK(hipMemcpyAsync(h_misc, B.misc.p, sizeof(h_misc), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -1441,6 +1641,7 @@ void hbtx_destroy(hbtx_ctx *c)
     (void)hipEventDestroy(c->wev0);
   if (c->wev1)
     (void)hipEventDestroy(c->wev1);
+  drop_graphs(c);
   hipStream_t st = c->stream;
   delete c;
   if (st)
@@ -1504,7 +1705,7 @@ int hbtx_batch_fetch(hbtx_ctx *c, int64_t *out_order, float *out_energy, float *
 static const char *kStatNames[NSTATS] = {"tree_wave_steps", "tree_lane_visits", "tree_lane_accepts",
                                          "direct_lane_pairs", "targets", "work_items", "near_interactions",
                                          "unused7", "unused8", "unused9", "unused10", "unused11", "walk_ms",
-                                         "rounds", "unused14", "unused15"};
+                                         "rounds", "graph_rounds", "graph_captures"};
 
 int hbtx_batch_stats(hbtx_ctx *c, double *stats, int nstats)
 {
@@ -1519,6 +1720,10 @@ int hbtx_batch_stats(hbtx_ctx *c, double *stats, int nstats)
       stats[12] = c->walk_ms;
     if (nstats > 13)
       stats[13] = c->rounds;
+    if (nstats > 14)
+      stats[14] = c->graph_rounds;
+    if (nstats > 15)
+      stats[15] = c->graph_captures;
   });
 }
 

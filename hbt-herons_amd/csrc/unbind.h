@@ -101,9 +101,20 @@ __global__ void k_init_subs(SubArrays S, const float *ref_in, const float *mb_in
   }
 }
 
-// Mode of the round (:563-600); subsampling keeps MaxSampleSize+Nunsample sources.
-__global__ void k_plan(SubArrays S, DevParams P, int round)
+// Start of a round: clear the active count, advance the device round counter.
+__global__ void k_round_begin(int32_t *active, int32_t *round_p)
 {
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+  {
+    *active = 0;
+    *round_p += 1;
+  }
+}
+
+// Mode of the round (:563-600); subsampling keeps MaxSampleSize+Nunsample sources.
+__global__ void k_plan(SubArrays S, DevParams P, const int32_t *round_p)
+{
+  const int round = *round_p;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S.nsub)
     return;
@@ -162,8 +173,9 @@ __global__ void k_mark_candidates(SubArrays S, PartArrays Q, int64_t n, int32_t 
 // Source flags: full mode -> candidates with rank < nsrc; correction -> removed
 // in the previous round.  Also the self mass used to cancel the self term.
 __global__ void k_mark_sources(SubArrays S, PartArrays Q, DevParams P, const int4 *tiles, const int32_t *ntiles_p,
-                               int round, const int32_t *crank, int32_t *flag)
+                               const int32_t *round_p, const int32_t *crank, int32_t *flag)
 {
+  const int round = *round_p;
   TILE_LOOP(tiles, ntiles_p)
   {
     const int4 tl = tiles[t_];
@@ -434,6 +446,8 @@ __global__ void k_sub_entries(SubArrays S, const int32_t *cscan, int32_t cap)
   S.ent_end[s] = min(e + cscan[e], cap);
 }
 
+// Entries of the round's trees: e_total = min(E, cap), overflow raised when E
+// exceeds the capacity.
 __global__ void k_entry_total(const int32_t *s_total, const int32_t *cscan, int32_t *e_total, int32_t cap,
                               int32_t *overflow)
 {
@@ -501,9 +515,11 @@ __global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, const 
 
 // RemoveUnboundParticles (:35-44): stable compaction of the bound candidates;
 // removed particles are tagged with the round.
-__global__ void k_partition_scatter(SubArrays S, PartArrays Q, const int4 *tiles, const int32_t *ntiles_p, int round,
-                                    const int64_t *flag, const int64_t *fscan, const int64_t *sub_first)
+__global__ void k_partition_scatter(SubArrays S, PartArrays Q, const int4 *tiles, const int32_t *ntiles_p,
+                                    const int32_t *round_p, const int64_t *flag, const int64_t *fscan,
+                                    const int64_t *sub_first)
 {
+  const int round = *round_p;
   TILE_LOOP(tiles, ntiles_p)
   {
     const int4 tl = tiles[t_];

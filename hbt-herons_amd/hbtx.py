@@ -74,7 +74,7 @@ RESULT_DTYPE = np.dtype([
 
 STAT_NAMES = ["tree_wave_steps", "tree_lane_visits", "tree_lane_accepts", "direct_lane_pairs", "targets",
               "work_items", "near_interactions", "unused7", "unused8", "unused9", "unused10", "unused11",
-              "walk_ms", "rounds", "unused14", "unused15"]
+              "walk_ms", "rounds", "graph_rounds", "graph_captures"]
 
 
 def workload_params(**kw):

@@ -35,7 +35,9 @@ extern "C" {
 #define HBTX_PLAIN 1u            /* one subhalo at a time, host-driven iteration (Subhalo_t::Unbind loop) */
 #define HBTX_WARMSTART 2u        /* start from the proposed bound set (hbtx_batch_set_proposal) */
 #define HBTX_SAVE_POTENTIAL 4u   /* fill out_potential (ParticlePotentialEnergies) */
-#define HBTX_NO_GRAPH 8u         /* launch kernels directly instead of replaying the captured round graph */
+#define HBTX_NO_GRAPH 8u         /* batched rounds after the first replay two captured HIP graphs (plan, sources,
+                                    octrees, work items | partition, decision, frames) around the persistent
+                                    walk launch; this flag launches every kernel directly (same results) */
 #define HBTX_COUNT 16u           /* count walk work (entries visited, interactions) for hbtx_batch_stats;
                                     results are identical, timing is not representative */
 
@@ -130,7 +132,8 @@ int hbtx_batch_set_proposal(hbtx_ctx *ctx, const uint8_t *proposal);
 int hbtx_batch_run(hbtx_ctx *ctx, const hbtx_params *prm, uint32_t flags, double *elapsed_ms);
 int hbtx_batch_fetch(hbtx_ctx *ctx, int64_t *out_order, float *out_energy, float *out_potential,
                      hbtx_sub_result *out_res);
-/* Kernel-level counters of the last run (work counts need HBTX_COUNT): stats[0..15], see hbtx_stat_name. */
+/* Kernel-level counters of the last run (work counts need HBTX_COUNT): stats[0..15], see hbtx_stat_name;
+ * 12 walk ms, 13 rounds, 14 rounds replayed from graphs, 15 graph captures. */
 int hbtx_batch_stats(hbtx_ctx *ctx, double *stats, int nstats);
 const char *hbtx_stat_name(int i);
 
