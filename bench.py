@@ -110,6 +110,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--snapshot", type=int, default=6, help="evaluated snapshot (trained on 0..5)")
     ap.add_argument("--threshold", type=float, default=0.1, help="proposal: P(bound) > threshold")
+    ap.add_argument("--warm-min", type=int, default=1000,
+                    help="subhaloes of fewer particles run the plain iteration (no proposal): a flipped "
+                         "particle there would exceed the per-subhalo mismatch tolerance on its own")
     ap.add_argument("--train-per-snapshot", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--direct-cutoff", type=int, default=-1,
@@ -202,7 +205,7 @@ def main():
 
     def learned_step():
         ctx.features(prm, fetch=False)
-        ctx.propose(net, args.threshold, count=False)
+        ctx.propose(net, args.threshold, count=False, min_particles=args.warm_min)
         ctx.run(prm, hbtx.WARMSTART | gflag)
 
     ms_learned = timed_steps(ctx, args.steps, args.warmup, learned_step)
@@ -210,7 +213,7 @@ def main():
     walk_ms = st_time["walk_ms"]
     rounds = int(st_time["rounds"])
     warm = ctx.fetch()
-    npro, inf_ms = ctx.propose(net, args.threshold)
+    npro, inf_ms = ctx.propose(net, args.threshold, min_particles=args.warm_min)
     log(f"batched + learned: {np.mean(ms_learned):.2f} ms/step (walk {walk_ms:.2f} ms, {rounds} rounds, "
         f"{npro} proposed, inference {inf_ms:.2f} ms)")
     if args.learned_only:
@@ -219,6 +222,25 @@ def main():
         ctx.close()
         return
     mismatch = learned.bound_mass_mismatch(pop, warm, ref)
+    mism = learned.mismatch_summary(pop, warm, ref)
+    # the same comparison at BoundMassPrecision 1.0 (iterate until no particle leaves)
+    prm1 = hbtx.workload_params(direct_cutoff=args.direct_cutoff, bound_mass_precision=1.0)
+    ms_plain1 = timed_steps(ctx, args.steps, args.warmup, lambda: ctx.run(prm1, gflag))
+    ref1 = ctx.fetch()
+
+    def learned_step1():
+        ctx.features(prm1, fetch=False)
+        ctx.propose(net, args.threshold, count=False, min_particles=args.warm_min)
+        ctx.run(prm1, hbtx.WARMSTART | gflag)
+
+    ms_learned1 = timed_steps(ctx, args.steps, args.warmup, learned_step1)
+    warm1 = ctx.fetch()
+    prec1 = {"batched_ms_per_step": float(np.mean(ms_plain1)), "learned_ms_per_step": float(np.mean(ms_learned1)),
+             "learned_gain": float(np.mean(ms_plain1) / np.mean(ms_learned1)),
+             "value": pop.n / (float(np.mean(ms_learned1)) / 1e3),
+             "mismatch": learned.mismatch_summary(pop, warm1, ref1)}
+    log(f"BoundMassPrecision 1.0: batched {prec1['batched_ms_per_step']:.2f} ms, learned "
+        f"{prec1['learned_ms_per_step']:.2f} ms, mismatch {prec1['mismatch']['summed']:.2e}")
     # work counters: one extra, untimed run of the counting instantiation (same results, deterministic work)
     ctx.run(prm, hbtx.WARMSTART | hbtx.COUNT)
     st = ctx.stats()
@@ -251,6 +273,7 @@ def main():
                    "softening": 0.001, "bound_mass_precision": 0.995, "max_sample_size": 0,
                    "direct_cutoff": int(args.direct_cutoff), "seed": args.seed, "snapshot": args.snapshot,
                    "trained_on_snapshots": f"0-{args.snapshot - 1}", "threshold": args.threshold,
+                   "warm_min_particles": args.warm_min,
                    "tuning": ctx.tuning(), "round_graphs": not args.no_graph,
                    "rounds": rounds, "bound_particles": int(res["nbound"].sum()),
                    "disrupted": int(res["disrupted"].sum()), "proposed": int(npro)},
@@ -260,6 +283,8 @@ def main():
                      "launches_per_step": rounds,
                      "kernel_avg_launch_ms": walk_ms / max(1, rounds), "flop_per_step": flops},
         "bound_mass_mismatch_vs_plain": mismatch,
+        "mismatch_detail": mism,
+        "bound_mass_precision_1": prec1,
         "batched_without_learned": {"value": batched_value, "unit": "particles/s",
                                     "ms_per_step": float(np.mean(ms_plain_b)), "rounds": int(st_plain["rounds"]),
                                     "walk_work": st_plain["tree_lane_visits"] + st_plain["direct_lane_pairs"]},

@@ -239,6 +239,7 @@ struct hbtx_ctx
   DArr<double> red_partial;
   DArr<unsigned long long> stats;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, wev0 = nullptr, wev1 = nullptr;
+  int64_t walk_items_hint = INT64_MAX; // work items of the previous round of the running view
   double walk_ms = 0.0;
   bool count_work = false;
   double stage_ms[8] = {0};
@@ -290,7 +291,7 @@ void reserve_sort(hbtx_ctx *c, int64_t n, int32_t nseg)
   c->sort_toff.alloc(nseg + 1);
   const int64_t maxt = n / (SORT_SMALL_MAX) + 2;
   c->sort_tiles.alloc(maxt);
-  c->ghist.alloc(maxt * RADIX_BINS);
+  c->ghist.alloc(maxt * RADIX_MAX_BINS);
 }
 
 // Sort (keys, vals) within each segment [seg_begin, seg_begin+seg_len); keys use
@@ -308,19 +309,33 @@ void seg_sort(hbtx_ctx *c, uint64_t *keys, int32_t *vals, int32_t nseg, const in
   const int32_t *ntiles_p = c->sort_toff.p + nseg;
   const int64_t maxt = n_cap / SORT_SMALL_MAX + 2;
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxt, 4 * c->num_cu));
-  int passes = (key_bits + RADIX_BITS - 1) / RADIX_BITS;
-  passes += passes & 1;
+  // an even number of passes leaves the result in place; 9-bit digits where they save passes
+  auto even_passes = [&](int rb) {
+    const int p = (key_bits + rb - 1) / rb;
+    return p + (p & 1);
+  };
+  const int rb = (even_passes(9) < even_passes(8)) ? 9 : 8;
+  const int passes = even_passes(rb);
   for (int pass = 0; pass < passes; pass++)
   {
-    const int shift = pass * RADIX_BITS;
+    const int shift = pass * rb;
     const uint64_t *kin = (pass & 1) ? c->kalt.p : keys;
     const int32_t *vin = (pass & 1) ? c->valt.p : vals;
     uint64_t *kout = (pass & 1) ? keys : c->kalt.p;
     int32_t *vout = (pass & 1) ? vals : c->valt.p;
-    k_radix_hist<<<grid, SORT_THREADS, 0, c->stream>>>(kin, c->sort_tiles.p, ntiles_p, shift, c->ghist.p);
-    k_radix_offsets<<<grid, SORT_THREADS, 0, c->stream>>>(seg_begin, c->sort_tiles.p, ntiles_p, c->sort_toff.p,
+    if (rb == 9)
+    {
+      k_radix_hist<9><<<grid, SORT_THREADS, 0, c->stream>>>(kin, c->sort_tiles.p, ntiles_p, shift, c->ghist.p);
+      k_radix_offsets<9><<<grid, SORT_THREADS, 0, c->stream>>>(seg_begin, c->sort_tiles.p, ntiles_p,
+                                                               c->sort_toff.p, c->sort_ntile.p, c->ghist.p);
+      k_radix_scatter<9><<<grid, SORT_THREADS, 0, c->stream>>>(kin, vin, kout, vout, c->sort_tiles.p, ntiles_p,
+                                                               shift, c->ghist.p);
+      continue;
+    }
+    k_radix_hist<8><<<grid, SORT_THREADS, 0, c->stream>>>(kin, c->sort_tiles.p, ntiles_p, shift, c->ghist.p);
+    k_radix_offsets<8><<<grid, SORT_THREADS, 0, c->stream>>>(seg_begin, c->sort_tiles.p, ntiles_p, c->sort_toff.p,
                                                           c->sort_ntile.p, c->ghist.p);
-    k_radix_scatter<<<grid, SORT_THREADS, 0, c->stream>>>(kin, vin, kout, vout, c->sort_tiles.p, ntiles_p, shift,
+    k_radix_scatter<8><<<grid, SORT_THREADS, 0, c->stream>>>(kin, vin, kout, vout, c->sort_tiles.p, ntiles_p, shift,
                                                           c->ghist.p);
   }
   HIPCHK(hipGetLastError());
@@ -795,6 +810,7 @@ void round_items(hbtx_ctx *c, Batch &B, View &V)
   k_items_emit<<<blocks_for((int64_t)V.nsub * HBTX_WAVE, 256, 1 << 24), 256, 0, st>>>(V.S, V.order, B.iscan.p,
                                                                                        B.items.p);
   k_fill_i32<<<1, 256, 0, st>>>(B.wctr.p, 16 * WALK_MAX_CHUNKS, 0);
+  k_copy_i32<<<1, 64, 0, st>>>(B.iscan.p + V.nsub, B.misc.p + 7); // item count, read after the round
   HIPCHK(hipGetLastError());
 }
 
@@ -803,7 +819,9 @@ void round_walk_launch(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
 {
   hipStream_t st = c->stream;
   WalkArgs W = walk_args(c, B, V, P);
-  const int64_t maxitems = V.n / HBTX_WAVE + V.nsub + 1;
+  // (the persistent walk needs no more waves than items; a round has no more
+  // items than the round before it, as every subhalo's target count only shrinks)
+  const int64_t maxitems = std::min<int64_t>(V.n / HBTX_WAVE + V.nsub + 1, c->walk_items_hint);
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxitems, (int64_t)c->walk_waves * c->num_cu));
   HIPCHK(hipEventRecord(c->wev0, st));
   if (P.periodic)
@@ -994,15 +1012,18 @@ __global__ void k_final_sub(SubArrays S, PartArrays Q, DevParams P, const int32_
   out[s] = R;
 }
 
-void run_final(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot)
+void run_final(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, int rounds)
 {
   hipStream_t st = c->stream;
   const int64_t n = V.n;
   const int gb = blocks_for(n);
   const int gs = blocks_for(V.nsub, 256, 1 << 20);
-  k_final_keys<<<gb, 256, 0, st>>>(V.S, V.Q, n, B.keys.p, V.fvals);
+  k_final_keys<<<gb, 256, 0, st>>>(V.S, V.Q, n, rounds, B.keys.p, V.fvals);
   k_final_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
-  seg_sort(c, B.keys.p, V.fvals, V.nsub, B.seg_begin.p, B.seg_len.p, n, 48);
+  int gbits = 0;
+  while ((1 << gbits) <= rounds)
+    gbits++;
+  seg_sort(c, B.keys.p, V.fvals, V.nsub, B.seg_begin.p, B.seg_len.p, n, 32 + gbits);
   if (P.refine)
   {
     // RefineBindingEnergyOrder (:395-425): tree over the most bound particles, true masses
@@ -1011,6 +1032,7 @@ void run_final(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot
     k_slot_of<<<gb, 256, 0, st>>>(V.S, n, V.fvals, V.Q.cur2);
     k_mark_refine<<<gb, 256, 0, st>>>(V.S, V.Q, n, V.Q.cur2, B.flag.p);
     build_sources_tree(c, B, V, P, nt, false);
+    c->walk_items_hint = INT64_MAX; // (other targets than the rounds')
     round_walk(c, B, V, P);
     k_refine_keys<<<gb, 256, 0, st>>>(V.S, n, V.fvals, B.eref.p + V.p0, B.keys.p, B.seg_begin.p, B.seg_len.p);
     k_refine_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
@@ -1177,6 +1199,7 @@ int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, 
       grow_entries(B, B.ent_bound, false);
   }
   init_view(c, B, V, P, warm);
+  c->walk_items_hint = INT64_MAX;
   int round = 0;
   const int max_rounds = 2000;
   while (true)
@@ -1203,11 +1226,12 @@ int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, 
     float wms = 0.f;
     HIPCHK(hipEventElapsedTime(&wms, c->wev0, c->wev1));
     c->walk_ms += wms;
+    c->walk_items_hint = std::max(h_misc[7], 1);
     REQUIRE(h_misc[2] == 0, HBTX_E_CAPACITY, "tree entry capacity exceeded");
     if (h_misc[0] == 0)
       break;
   }
-  run_final(c, B, V, P, want_pot);
+  run_final(c, B, V, P, want_pot, round);
   return round;
 }
 
@@ -1901,7 +1925,14 @@ int hbtx_segmented_sort(hbtx_ctx *c, int64_t n, uint64_t *keys, int32_t *vals, i
     HIPCHK(hipMemcpyAsync(dv.p, vals, n * 4, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(sb.p, hb.data(), nseg * 4, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(sl.p, hl.data(), nseg * 4, hipMemcpyHostToDevice, st));
-    seg_sort(c, dk.p, dv.p, (int32_t)nseg, sb.p, sl.p, n, 64);
+    // digit passes only over the bits the keys use
+    uint64_t any = 0;
+    for (int64_t i = 0; i < n; i++)
+      any |= keys[i];
+    int bits = 1;
+    while (bits < 64 && (any >> bits) != 0)
+      bits++;
+    seg_sort(c, dk.p, dv.p, (int32_t)nseg, sb.p, sl.p, n, bits);
     HIPCHK(hipMemcpyAsync(keys, dk.p, n * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(vals, dv.p, n * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -2199,9 +2230,10 @@ int hbtx_batch_get_proposal(hbtx_ctx *c, uint8_t *proposal)
   });
 }
 
-int hbtx_batch_propose(hbtx_ctx *c, hbtx_mlp *M, float threshold, int64_t *nproposed, double *elapsed_ms)
+int hbtx_batch_propose(hbtx_ctx *c, hbtx_mlp *M, float threshold, int64_t min_particles, int64_t *nproposed,
+                       double *elapsed_ms)
 {
-  if (!c || !M || !(threshold > 0.f && threshold < 1.f))
+  if (!c || !M || !(threshold > 0.f && threshold < 1.f) || min_particles < 0)
     return HBTX_E_INVALID;
   API_GUARD(c, {
     HIPCHK(hipSetDevice(c->device));
@@ -2213,7 +2245,7 @@ int hbtx_batch_propose(hbtx_ctx *c, hbtx_mlp *M, float threshold, int64_t *nprop
     HIPCHK(hipEventRecord(c->wev0, st));
     k_mlp_forward<<<mlp_forward_blocks(n), 256, 0, st>>>(M->wf.p, M->wb.p, B.feat.p, n, B.logit.p);
     k_proposal_from_logit<<<blocks_for(n), 256, 0, st>>>(B.logit.p, n, std::log(threshold / (1.f - threshold)),
-                                                          B.proposal.p);
+                                                          B.sub_of.p, B.off.p, min_particles, B.proposal.p);
     HIPCHK(hipEventRecord(c->wev1, st));
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventSynchronize(c->wev1));

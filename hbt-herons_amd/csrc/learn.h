@@ -63,9 +63,17 @@ __global__ void k_features(SubArrays S, PartArrays Q, DevParams P, PrevArrays V,
   }
 }
 
-__global__ void k_proposal_from_logit(const float *logit, int64_t n, float cut, uint8_t *proposal)
+// Proposed bound flag; subhaloes of fewer than min_particles particles get an
+// empty proposal (they run the plain iteration).
+__global__ void k_proposal_from_logit(const float *logit, int64_t n, float cut, const int32_t *sub_of,
+                                      const int32_t *off, int64_t min_particles, uint8_t *proposal)
 {
-  GRID_STRIDE(p, n) proposal[p] = logit[p] > cut ? 1 : 0;
+  GRID_STRIDE(p, n)
+  {
+    const int s = sub_of[p];
+    const int64_t np = off[s + 1] - off[s];
+    proposal[p] = (np >= min_particles && logit[p] > cut) ? 1 : 0;
+  }
 }
 
 } // namespace hbtx

@@ -1,9 +1,9 @@
 // Segmented sort of (uint64 key, int32 value) pairs.
 //  * segments of <= SORT_SMALL_MAX elements: one workgroup each, bitonic sort in LDS;
-//  * larger segments: tiled LSD radix sort (8-bit digits) where every tile of
-//    SORT_TILE elements ranks its keys stably with wave ballots (match of the
-//    8 digit bits) and LDS per-digit counters, so all large segments advance
-//    together in one launch per pass.
+//  * larger segments: tiled LSD radix sort (8- or 9-bit digits, whichever needs
+//    fewer passes) where every tile of SORT_TILE elements ranks its keys stably
+//    with wave ballots (match of the digit bits) and LDS per-digit counters, so
+//    all large segments advance together in one launch per pass.
 #pragma once
 #include "common.h"
 #include "scan.h"
@@ -13,8 +13,7 @@ namespace hbtx
 #define SORT_THREADS 256
 #define SORT_SMALL_MAX 2048
 #define SORT_TILE 4096
-#define RADIX_BITS 8
-#define RADIX_BINS 256
+#define RADIX_MAX_BINS 512 // (9-bit digits)
 
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_small(uint64_t *__restrict__ keys, int32_t *__restrict__ vals,
                                                             const int32_t *__restrict__ seg_begin,
@@ -96,31 +95,37 @@ __global__ void k_sort_tiles(const int32_t *__restrict__ seg_begin, const int32_
     tiles[t] = make_int4(s, x, min(e, x + SORT_TILE), 0);
 }
 
+template <int RB>
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint64_t *__restrict__ kin,
                                                             const int4 *__restrict__ tiles,
                                                             const int32_t *__restrict__ ntiles_p, int shift,
                                                             int32_t *__restrict__ ghist)
 {
-  __shared__ int32_t hist[RADIX_BINS];
+  constexpr int BINS = 1 << RB;
+  __shared__ int32_t hist[BINS];
   const int nt = *ntiles_p;
   const int tid = threadIdx.x;
   for (int t = blockIdx.x; t < nt; t += gridDim.x)
   {
-    hist[tid] = 0;
+    for (int b = tid; b < BINS; b += SORT_THREADS)
+      hist[b] = 0;
     __syncthreads();
     const int4 tl = tiles[t];
     for (int i = tl.y + tid; i < tl.z; i += SORT_THREADS)
-      atomicAdd(&hist[(int)((kin[i] >> shift) & (RADIX_BINS - 1))], 1);
+      atomicAdd(&hist[(int)((kin[i] >> shift) & (BINS - 1))], 1);
     __syncthreads();
-    ghist[(int64_t)t * RADIX_BINS + tid] = hist[tid];
+    for (int b = tid; b < BINS; b += SORT_THREADS)
+      ghist[(int64_t)t * BINS + b] = hist[b];
     __syncthreads();
   }
 }
 
 // Per large segment: turn tile histograms into global scatter bases.  Driven by
 // the radix tiles (a block takes the segments whose first tile it meets), so
-// launches with no large segment cost nothing; each digit's column of tile counts
-// is read 8 tiles at a time.
+// launches with no large segment cost nothing; thread i owns digits
+// [PER i, PER i + PER), and each digit's column of tile counts is read 8 tiles
+// at a time.
+template <int RB>
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_offsets(const int32_t *__restrict__ seg_begin,
                                                                const int4 *__restrict__ tiles,
                                                                const int32_t *__restrict__ ntiles_p,
@@ -128,8 +133,9 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_offsets(const int32_t *_
                                                                const int32_t *__restrict__ ntile,
                                                                int32_t *__restrict__ ghist)
 {
+  constexpr int BINS = 1 << RB, PER = BINS / SORT_THREADS;
   __shared__ int32_t lds[SORT_THREADS / HBTX_WAVE];
-  const int d = threadIdx.x;
+  const int d0 = threadIdx.x * PER;
   const int nt_all = *ntiles_p;
   for (int t = blockIdx.x; t < nt_all; t += gridDim.x)
   {
@@ -138,33 +144,49 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_offsets(const int32_t *_
     if (tl.y != seg_begin[s])
       continue; // not the segment's first tile
     const int t0 = tile_off[s], nt = ntile[s];
-    int run = 0;
-    for (int u = t0; u < t0 + nt; u += 8)
+    int run[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++)
     {
-      int c[8];
-#pragma unroll
-      for (int k = 0; k < 8; k++)
-        c[k] = (u + k < t0 + nt) ? ghist[(int64_t)(u + k) * RADIX_BINS + d] : 0;
-#pragma unroll
-      for (int k = 0; k < 8; k++)
+      const int d = d0 + j;
+      run[j] = 0;
+      for (int u = t0; u < t0 + nt; u += 8)
       {
-        if (u + k < t0 + nt)
-          ghist[(int64_t)(u + k) * RADIX_BINS + d] = run;
-        run += c[k];
+        int c[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          c[k] = (u + k < t0 + nt) ? ghist[(int64_t)(u + k) * BINS + d] : 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+        {
+          if (u + k < t0 + nt)
+            ghist[(int64_t)(u + k) * BINS + d] = run[j];
+          run[j] += c[k];
+        }
       }
     }
+    int mine = 0;
+#pragma unroll
+    for (int j = 0; j < PER; j++)
+      mine += run[j];
     int tot;
-    const int base = block_excl_scan(run, lds, &tot) + seg_begin[s];
-    for (int u = t0; u < t0 + nt; u += 8)
+    int base = block_excl_scan(mine, lds, &tot) + seg_begin[s];
+#pragma unroll
+    for (int j = 0; j < PER; j++)
     {
-      int c[8];
+      const int d = d0 + j;
+      for (int u = t0; u < t0 + nt; u += 8)
+      {
+        int c[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++)
-        c[k] = (u + k < t0 + nt) ? ghist[(int64_t)(u + k) * RADIX_BINS + d] : 0;
+        for (int k = 0; k < 8; k++)
+          c[k] = (u + k < t0 + nt) ? ghist[(int64_t)(u + k) * BINS + d] : 0;
 #pragma unroll
-      for (int k = 0; k < 8; k++)
-        if (u + k < t0 + nt)
-          ghist[(int64_t)(u + k) * RADIX_BINS + d] = c[k] + base;
+        for (int k = 0; k < 8; k++)
+          if (u + k < t0 + nt)
+            ghist[(int64_t)(u + k) * BINS + d] = c[k] + base;
+      }
+      base += run[j];
     }
     __syncthreads();
   }
@@ -176,6 +198,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_offsets(const int32_t *_
 // it out in that order, so consecutive threads store to consecutive addresses of
 // each digit's run.
 #define SCAT_ITEMS (SORT_TILE / SORT_THREADS)
+template <int RB>
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *__restrict__ kin,
                                                                const int32_t *__restrict__ vin,
                                                                uint64_t *__restrict__ kout,
@@ -184,10 +207,11 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
                                                                const int32_t *__restrict__ ntiles_p, int shift,
                                                                const int32_t *__restrict__ ghist)
 {
+  constexpr int BINS = 1 << RB, PER = BINS / SORT_THREADS;
   __shared__ uint64_t sk[SORT_TILE];
   __shared__ int32_t sv[SORT_TILE];
-  __shared__ int32_t wcnt[SORT_THREADS / HBTX_WAVE][RADIX_BINS];
-  __shared__ int32_t dbase[RADIX_BINS];
+  __shared__ int32_t wcnt[SORT_THREADS / HBTX_WAVE][BINS];
+  __shared__ int32_t dbase[BINS];
   __shared__ int32_t scan_lds[SORT_THREADS / HBTX_WAVE];
   const int nt = *ntiles_p;
   const int tid = threadIdx.x;
@@ -200,7 +224,9 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
     const int n = tl.z - tl.y;
 #pragma unroll
     for (int k = 0; k < SORT_THREADS / HBTX_WAVE; k++)
-      wcnt[k][tid] = 0;
+#pragma unroll
+      for (int j = 0; j < PER; j++)
+        wcnt[k][tid * PER + j] = 0;
     __syncthreads();
     uint64_t key[SCAT_ITEMS];
     int32_t val[SCAT_ITEMS];
@@ -212,10 +238,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
       const bool valid = li < n;
       key[r] = valid ? kin[tl.y + li] : 0ull;
       val[r] = valid ? vin[tl.y + li] : 0;
-      const int d = (int)((key[r] >> shift) & (RADIX_BINS - 1));
+      const int d = (int)((key[r] >> shift) & (BINS - 1));
       uint64_t m = ballot(valid);
 #pragma unroll
-      for (int bit = 0; bit < RADIX_BITS; bit++)
+      for (int bit = 0; bit < RB; bit++)
       {
         const bool on = (d >> bit) & 1;
         const uint64_t bb = ballot(valid && on);
@@ -229,18 +255,31 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
     }
     __syncthreads();
     {
-      // digit tid: offsets of the waves' runs and the first slot of the digit in the tile
-      int c[SORT_THREADS / HBTX_WAVE];
-      int tot = 0;
+      // digits PER tid ..: offsets of the waves' runs and the first slot of the digit in the tile
+      int tot[PER];
+      int mine = 0;
 #pragma unroll
-      for (int k = 0; k < SORT_THREADS / HBTX_WAVE; k++)
+      for (int j = 0; j < PER; j++)
       {
-        c[k] = wcnt[k][tid];
-        wcnt[k][tid] = tot;
-        tot += c[k];
+        const int d = tid * PER + j;
+        tot[j] = 0;
+#pragma unroll
+        for (int k = 0; k < SORT_THREADS / HBTX_WAVE; k++)
+        {
+          const int c = wcnt[k][d];
+          wcnt[k][d] = tot[j];
+          tot[j] += c;
+        }
+        mine += tot[j];
       }
       int all;
-      dbase[tid] = block_excl_scan(tot, scan_lds, &all);
+      int b = block_excl_scan(mine, scan_lds, &all);
+#pragma unroll
+      for (int j = 0; j < PER; j++)
+      {
+        dbase[tid * PER + j] = b;
+        b += tot[j];
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -249,18 +288,18 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
       const int li = w * per_wave + r * HBTX_WAVE + lane;
       if (li < n)
       {
-        const int d = (int)((key[r] >> shift) & (RADIX_BINS - 1));
+        const int d = (int)((key[r] >> shift) & (BINS - 1));
         const int pos = dbase[d] + wcnt[w][d] + rank[r];
         sk[pos] = key[r];
         sv[pos] = val[r];
       }
     }
     __syncthreads();
-    const int32_t *off = ghist + (int64_t)t * RADIX_BINS;
+    const int32_t *off = ghist + (int64_t)t * BINS;
     for (int i = tid; i < n; i += SORT_THREADS)
     {
       const uint64_t k = sk[i];
-      const int d = (int)((k >> shift) & (RADIX_BINS - 1));
+      const int d = (int)((k >> shift) & (BINS - 1));
       const int dst = off[d] + (i - dbase[d]);
       kout[dst] = k;
       vout[dst] = sv[i];

@@ -58,6 +58,11 @@ __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v)
 {
   GRID_STRIDE(i, n) p[i] = v;
 }
+__global__ void k_copy_i32(const int32_t *src, int32_t *dst)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    *dst = *src;
+}
 __global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v)
 {
   GRID_STRIDE(i, n) p[i] = v;
@@ -879,7 +884,9 @@ __global__ void k_round_finish_periodic(SubArrays S, PartArrays Q, DevParams P, 
 
 // Sort key of every particle: bound first by energy, then the unbound batches
 // with the latest round first, each by energy (:673, :710, :744-750).
-__global__ void k_final_keys(SubArrays S, PartArrays Q, int64_t n, uint64_t *keys, int32_t *vals)
+// group above the energy bits: 0 for the bound particles, then the unbound
+// batches latest first (removed in round r of `rounds`: rounds + 1 - r)
+__global__ void k_final_keys(SubArrays S, PartArrays Q, int64_t n, int32_t rounds, uint64_t *keys, int32_t *vals)
 {
   GRID_STRIDE(p, n)
   {
@@ -888,7 +895,7 @@ __global__ void k_final_keys(SubArrays S, PartArrays Q, int64_t n, uint64_t *key
     if (S.status[s] == ST_CONVERGED)
     {
       const int st = Q.state[p];
-      const uint64_t group = (st == 0) ? 0ull : (uint64_t)(0x10000 - st);
+      const uint64_t group = (st == 0) ? 0ull : (uint64_t)(rounds + 1 - st);
       key = (group << 32) | (uint64_t)float_to_ordered(Q.energy[p]);
     }
     else

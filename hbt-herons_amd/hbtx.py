@@ -150,7 +150,7 @@ def lib():
         L.hbtx_mlp_predict.argtypes = [P, P, i64, P, P]
         L.hbtx_batch_set_previous.argtypes = [P, P, P, P, P, P, P, P]
         L.hbtx_batch_features.argtypes = [P, C.POINTER(Params), P]
-        L.hbtx_batch_propose.argtypes = [P, P, C.c_float, C.POINTER(i64), C.POINTER(C.c_double)]
+        L.hbtx_batch_propose.argtypes = [P, P, C.c_float, i64, C.POINTER(i64), C.POINTER(C.c_double)]
         L.hbtx_batch_get_proposal.argtypes = [P, P]
         L.hbtx_get_tuning.argtypes = [P, C.POINTER(i64), C.POINTER(C.c_int32)]
         L.hbtx_set_tuning.argtypes = [P, i64, C.c_int32]
@@ -266,13 +266,14 @@ class Context:
         self._check(lib().hbtx_batch_features(self._h, C.byref(prm), _p(out)))
         return out
 
-    def propose(self, mlp, threshold=0.5, count=True):
-        """Proposal = P(bound) > threshold from the batch features; returns
-        (number proposed or None, inference milliseconds)."""
+    def propose(self, mlp, threshold=0.5, count=True, min_particles=0):
+        """Proposal = P(bound) > threshold from the batch features (none for
+        subhaloes of fewer than min_particles particles); returns (number
+        proposed or None, inference milliseconds)."""
         npro = C.c_int64(0)
         ms = C.c_double(0.0)
-        self._check(lib().hbtx_batch_propose(self._h, mlp._h, threshold, C.byref(npro) if count else None,
-                                             C.byref(ms)))
+        self._check(lib().hbtx_batch_propose(self._h, mlp._h, threshold, int(min_particles),
+                                             C.byref(npro) if count else None, C.byref(ms)))
         return (npro.value if count else None), ms.value
 
     def get_proposal(self):

@@ -67,6 +67,36 @@ def bound_mass_mismatch(pop, out, ref):
     return float(m[a ^ b].sum() / max(m[b].sum(), 1e-300))
 
 
+def mismatch_per_subhalo(pop, out, ref):
+    """Per subhalo: mass of the symmetric difference of the bound sets over the
+    bound mass of ref (0 when both are empty, inf when only ref is empty)."""
+    a, b = labels(pop, out).astype(bool), labels(pop, ref).astype(bool)
+    m = pop.mass.astype(np.float64)
+    off = np.asarray(pop.offsets, dtype=np.int64)
+    sub = np.repeat(np.arange(pop.nsub), np.diff(off))
+    diff = np.bincount(sub, weights=m * (a ^ b), minlength=pop.nsub)
+    mref = np.bincount(sub, weights=m * b, minlength=pop.nsub)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.where(mref > 0, diff / np.where(mref > 0, mref, 1.0), np.where(diff > 0, np.inf, 0.0))
+    return r
+
+
+def mismatch_summary(pop, out, ref, tol=1e-3):
+    """Summed mismatch and the per-subhalo distribution against tol."""
+    per = mismatch_per_subhalo(pop, out, ref)
+    b = labels(pop, ref).astype(bool)
+    off = np.asarray(pop.offsets, dtype=np.int64)
+    sub = np.repeat(np.arange(pop.nsub), np.diff(off))
+    mref = np.bincount(sub, weights=pop.mass.astype(np.float64) * b, minlength=pop.nsub)
+    ok = per <= tol
+    return {"summed": bound_mass_mismatch(pop, out, ref), "tolerance": tol,
+            "subhalos_within": float(ok.mean()),
+            "bound_mass_in_subhalos_within": float(mref[ok].sum() / max(mref.sum(), 1e-300)),
+            "per_subhalo_median": float(np.median(per)),
+            "per_subhalo_p99": float(np.quantile(per[np.isfinite(per)], 0.99)) if np.isfinite(per).any() else None,
+            "subhalos_disagreeing_on_disruption": int(((per == np.inf) | ((mref > 0) & (per >= 1.0))).sum())}
+
+
 def snapshot_features(ctx, pop, prev, params):
     ctx.load(pop)
     ctx.set_previous(prev)
@@ -103,13 +133,13 @@ def train(ctx, X, y, seed=0, epochs=4, batch=8192, lr=3e-3):
     return net, loss
 
 
-def warm_unbind(ctx, pop, prev, net, params, threshold=0.1, flags=hbtx.BATCHED):
+def warm_unbind(ctx, pop, prev, net, params, threshold=0.1, flags=hbtx.BATCHED, min_particles=0):
     """Features -> proposal -> verified warm-started unbinding of one snapshot.
     Returns (output, run milliseconds, inference milliseconds, proposed count)."""
     ctx.load(pop)
     ctx.set_previous(prev)
     ctx.features(params, fetch=False)
-    npro, inf_ms = ctx.propose(net, threshold)
+    npro, inf_ms = ctx.propose(net, threshold, min_particles=min_particles)
     ms = ctx.run(params, flags | hbtx.WARMSTART)
     out = ctx.fetch()
     return out, ms, inf_ms, npro

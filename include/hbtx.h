@@ -192,8 +192,10 @@ int hbtx_batch_set_previous(hbtx_ctx *ctx, const int32_t *prev_rank, const uint8
 /* Per-particle features float[n][8] on the device (out_features may be NULL). */
 int hbtx_batch_features(hbtx_ctx *ctx, const hbtx_params *prm, float *out_features);
 /* Inference on the batch features: proposal = P(bound) > threshold, kept on the device
- * for HBTX_WARMSTART; *elapsed_ms times the inference kernel. */
-int hbtx_batch_propose(hbtx_ctx *ctx, hbtx_mlp *mlp, float threshold, int64_t *nproposed, double *elapsed_ms);
+ * for HBTX_WARMSTART; subhaloes of fewer than min_particles particles get no proposal
+ * (they run the plain iteration); *elapsed_ms times the inference kernels. */
+int hbtx_batch_propose(hbtx_ctx *ctx, hbtx_mlp *mlp, float threshold, int64_t min_particles, int64_t *nproposed,
+                       double *elapsed_ms);
 /* Copy of the current proposal (uint8[n]). */
 int hbtx_batch_get_proposal(hbtx_ctx *ctx, uint8_t *proposal);
 

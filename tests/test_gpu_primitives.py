@@ -30,6 +30,26 @@ def test_segmented_sort_matches_numpy(ctx, sizes):
     _check_sorted(keys, vals, k2, v2, off)
 
 
+@pytest.mark.parametrize("bits", [18, 30, 36, 63])
+def test_segmented_sort_key_widths(ctx, bits):
+    """Narrow keys take fewer digit passes (9-bit digits at 18 and 36 bits)."""
+    rng = np.random.default_rng(bits)
+    sizes = [3000, 9000, 1, 40000, 2048, 2049]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(off[-1])
+    keys = rng.integers(0, 2 ** bits, n, dtype=np.uint64)
+    keys[::5] = keys[::5] & np.uint64(0xff)  # shared low digits
+    keys[1::9] = keys[0:n - 1:9]  # duplicates
+    vals = np.arange(n, dtype=np.int32)
+    k2, v2 = ctx.segmented_sort(keys, vals, off)
+    for s in range(len(sizes)):
+        a, b = off[s], off[s + 1]
+        order = np.argsort(keys[a:b], kind="stable")
+        assert np.array_equal(k2[a:b], keys[a:b][order])
+        if b - a > 2048:  # radix path: stable
+            assert np.array_equal(v2[a:b], vals[a:b][order])
+
+
 def test_radix_sort_is_stable_for_large_segments(ctx):
     rng = np.random.default_rng(7)
     n = 50000
