@@ -9,8 +9,8 @@
 // subtree.  The wave descends (index+1) if any active lane opened the entry,
 // otherwise it jumps to the entry's skip.  Every lane therefore sees exactly the
 // entries its own stackless walk would see, in the same order, while lanes on
-// spatially adjacent targets share the traversal.  Both possible next entries
-// are loaded while the current one is evaluated.
+// spatially adjacent targets share the traversal.  An entry arrives in one
+// scalar load at a wave-uniform byte offset.
 //
 // Direct mode: the entry list holds particles only and every pair is summed.
 //
