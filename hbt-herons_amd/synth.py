@@ -328,6 +328,80 @@ def host_with_subs(seed=1, n_host=2_000_000, nsub=50, nsub_min=500, nsub_max=50_
                       meta=dict(kind="host_with_subs", seed=seed, n_host=n_host, sub_counts=ns))
 
 
+class Hierarchy:
+    """Nested subhaloes for Subhalo_t::RecursiveUnbind (src/subhalo_unbind.cpp:810-840):
+    pop holds every subhalo's own particles (one subhalo per node), children[s]
+    its NestedSubhalos, roots the top of every tree, track / sink the TrackId and
+    SinkTrackId (-1 when not trapped) of every subhalo."""
+
+    def __init__(self, pop, children, roots, track, sink):
+        self.pop = pop
+        self.children = children
+        self.roots = roots
+        self.track = track
+        self.sink = sink
+
+
+def hierarchy(seed=5, nhost=3, n_host=20000, kids=(2, 4), n_kid=(500, 5000), n_grandkid=(100, 1500),
+              p_overlap=0.35):
+    """nhost trees: an NFW host (10% interlopers) with embedded NFW children, each
+    with 0-2 embedded grandchildren and, under the first child of every host, one
+    more level.  A share p_overlap of the grandchildren sit on their parent's
+    centre at its bulk velocity, so the phase-space merger test fires; the last
+    host's first child is already trapped (SinkTrackId set)."""
+    rng = np.random.default_rng(seed)
+    m = PARTICLE_MASS
+    xs, vs, counts, children, roots = [], [], [], [], []
+
+    def add(n, c, centre, bulk, inter=0):
+        r200 = R200_PER_1E4 * np.cbrt(n / 1e4)
+        rs = r200 / c
+        gm = G_GRAV * n * m
+        x, v = _nfw_members(rng, n, c, rs, gm)
+        if inter:
+            xi, vi = _interlopers(rng, inter, c, rs, gm, 3.0 * r200)
+            x, v = np.concatenate([x, xi]), np.concatenate([v, vi])
+        xs.append(x + centre)
+        vs.append(v + bulk)
+        counts.append(len(x))
+        children.append([])
+        return len(counts) - 1, r200, float(np.sqrt(np.mean(v[:n] ** 2)))
+
+    def embed(parent, cen, bulk, r200, sig, nrange, overlap):
+        n = int(np.exp(rng.uniform(np.log(nrange[0]), np.log(nrange[1]))))
+        if overlap:
+            c2, b2 = cen.copy(), bulk.copy()
+        else:
+            c2 = cen + _unit_vectors(rng, 1)[0] * 0.5 * r200 * np.cbrt(rng.uniform(0.2, 1.0))
+            b2 = bulk + rng.normal(0.0, sig, 3)
+        k, r, sg = add(n, 8.0, c2, b2)
+        children[parent].append(k)
+        return k, c2, b2, r, sg
+
+    for h in range(nhost):
+        cen = np.array([10.0 + 10.0 * h, 25.0, 25.0])
+        bulk = rng.normal(0.0, 50.0, 3)
+        root, r200, sig = add(n_host, 10.0, cen, bulk, inter=n_host // 10)
+        roots.append(root)
+        for j in range(int(rng.integers(kids[0], kids[1] + 1))):
+            kid, kc, kb, kr, ks = embed(root, cen, bulk, r200, sig, n_kid, False)
+            for _ in range(int(rng.integers(0, 3))):
+                embed(kid, kc, kb, kr, ks, n_grandkid, rng.uniform() < p_overlap)
+            if j == 0:
+                g, gc, gb, gr, gs = embed(kid, kc, kb, kr, ks, n_grandkid, False)
+                embed(g, gc, gb, gr, gs, (60, 200), False)
+    pos = np.concatenate(xs)
+    vel = np.concatenate(vs)
+    off = np.concatenate([[0], np.cumsum(counts)])
+    n = int(off[-1])
+    pop = Population(pos, vel, np.full(n, m), off, np.arange(n), meta=dict(kind="hierarchy", seed=seed))
+    ns = len(counts)
+    track = np.arange(ns, dtype=np.int64)
+    sink = np.full(ns, -1, np.int64)
+    sink[children[roots[-1]][0]] = roots[-1]
+    return Hierarchy(pop, children, roots, track, sink)
+
+
 _NOISE_STREAMS = 16  # fixed, so the noise does not depend on the thread count
 
 

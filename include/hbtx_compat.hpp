@@ -9,6 +9,12 @@
 //   hbtx::RefineParticles(ctx, prm, subs, snapshot_id)
 //                              the unbinding loop of SubhaloSnapshot_t::RefineParticles,
 //                              src/subhalo_unbind.cpp:855-919 (all subhaloes in one batch)
+//   hbtx::RecursiveUnbind(ctx, prm, Subhalos, roots, snapshot_id, merge_trapped)
+//                              Subhalo_t::RecursiveUnbind   src/subhalo_unbind.cpp:810-840
+//                              with MergeRecursively / MergeTo src/subhalo_merge.cpp:15-91,183-239,
+//                              one batch per level of the hierarchies
+//   hbtx::TruncateSource(sub, relax_factor)
+//                              Subhalo_t::TruncateSource    src/subhalo_unbind.cpp:842-852
 //
 // The templates accept any subhalo type with the reference's member names
 // (Particles with Id / ComovingPosition / PhysicalVelocity / Mass and, when
@@ -20,7 +26,9 @@
 #pragma once
 #include "hbtx.h"
 
+#include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -112,6 +120,7 @@ HBTX_MEMBER_TRAIT(SnapshotOfDeath)
 HBTX_MEMBER_TRAIT(DescendantTrackId)
 HBTX_MEMBER_TRAIT(SinkTrackId)
 HBTX_MEMBER_TRAIT(SnapshotOfSink)
+HBTX_MEMBER_TRAIT(Type)
 #undef HBTX_MEMBER_TRAIT
 
 template <class V>
@@ -353,6 +362,188 @@ void Unbind(Context &ctx, const hbtx_params &prm, SubhaloT &sub, int snapshot_id
 {
   std::vector<SubhaloT *> one{&sub};
   RefineParticles(ctx, prm, one, snapshot_id, HBTX_PLAIN, save_binding_energies, save_potential_energies);
+}
+
+// ---------------------------------------------------------------------------
+// Hierarchical unbinding
+// ---------------------------------------------------------------------------
+
+// PeriodicDistance (src/config_parser.h:220-233) in float.
+template <class V>
+float PeriodicDistance(const V &x, const V &y, const hbtx_params &prm)
+{
+  float d[3];
+  for (int k = 0; k < 3; k++)
+  {
+    d[k] = (float)x[k] - (float)y[k];
+    if (prm.periodic)
+    {
+      const float half = prm.box_size / 2.f;
+      d[k] = (d[k] > half) ? d[k] - prm.box_size : ((d[k] < -half) ? d[k] + prm.box_size : d[k]);
+    }
+  }
+  return std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+}
+
+// Subhalo_t::PhaseSpaceDistance (src/subhalo_merge.cpp:15-20): core position
+// offset over the reference's core position dispersion plus core velocity offset
+// over its velocity dispersion.
+template <class SubhaloT>
+float PhaseSpaceDistance(const SubhaloT &sub, const SubhaloT &ref, const hbtx_params &prm)
+{
+  const float dx = PeriodicDistance(ref.CoreComovingPosition, sub.CoreComovingPosition, prm);
+  float dv2 = 0.f;
+  for (int k = 0; k < 3; k++)
+  {
+    const float d = (float)ref.CorePhysicalVelocity[k] - (float)sub.CorePhysicalVelocity[k];
+    dv2 += d * d;
+  }
+  return dx / ref.CoreComovingSigmaR + std::sqrt(dv2) / ref.CorePhysicalSigmaV;
+}
+
+// Subhalo_t::MergeTo (src/subhalo_merge.cpp:183-239): the bound particles go to
+// the host; the subhalo keeps its most bound tracer as an orphan.
+template <class SubhaloT>
+void MergeTo(SubhaloT &sub, SubhaloT &host)
+{
+  if (sub.Nbound <= 1)
+    return;
+  host.Particles.insert(host.Particles.end(), sub.Particles.begin(), sub.Particles.begin() + sub.Nbound);
+  const auto &tracer = sub.Particles[sub.TracerIndex];
+  sub.MostBoundParticleId = tracer.Id;
+  for (int k = 0; k < 3; k++)
+  {
+    sub.ComovingMostBoundPosition[k] = tracer.ComovingPosition[k];
+    sub.PhysicalMostBoundVelocity[k] = tracer.PhysicalVelocity[k];
+    sub.ComovingAveragePosition[k] = sub.ComovingMostBoundPosition[k];
+    sub.PhysicalAverageVelocity[k] = sub.PhysicalMostBoundVelocity[k];
+  }
+  sub.Nbound = 0;
+  sub.Particles.resize(0);
+  // CountParticles (src/subhalo.cpp:435-455) over no particles
+  sub.Mbound = 0.f;
+  if constexpr (detail::has_NboundType<SubhaloT>::value)
+    for (auto &x : sub.NboundType)
+      x = 0;
+  if constexpr (detail::has_MboundType<SubhaloT>::value)
+    for (auto &x : sub.MboundType)
+      x = 0.f;
+}
+
+// Subhalo_t::MergeRecursively (src/subhalo_merge.cpp:45-91): depth first over the
+// hierarchy below Subhalos[self]; every subhalo that is not already trapped and
+// overlaps the reference in phase space (distance < 2) sinks into it.  Returns
+// whether a resolved subhalo (Nbound > 1) merged.
+template <class SubhaloT>
+bool MergeRecursively(std::vector<SubhaloT> &Subhalos, int64_t self, SubhaloT &ref, int snapshot_id,
+                      bool merge_trapped, const hbtx_params &prm)
+{
+  bool merged = false;
+  if (ref.Nbound <= 1)
+    return merged;
+  SubhaloT &S = Subhalos[self];
+  for (size_t i = 0; i < S.NestedSubhalos.size(); i++)
+    merged = MergeRecursively(Subhalos, (int64_t)S.NestedSubhalos[i], ref, snapshot_id, merge_trapped, prm) | merged;
+  if (S.SinkTrackId == -1 && &S != &ref)
+  {
+    if (PhaseSpaceDistance(S, ref, prm) < 2.f)
+    {
+      // SetMergerInformation (:28-39)
+      S.SnapshotOfSink = snapshot_id;
+      S.SinkTrackId = ref.TrackId;
+      if (S.SnapshotOfDeath == -1)
+        S.SnapshotOfDeath = snapshot_id;
+      merged = (S.Nbound > 1) | merged;
+      if (merge_trapped)
+        MergeTo(S, ref);
+    }
+  }
+  return merged;
+}
+
+// Subhalo_t::TruncateSource (src/subhalo_unbind.cpp:842-852).
+template <class SubhaloT>
+void TruncateSource(SubhaloT &sub, double source_sub_relax_factor)
+{
+  int64_t ns = (sub.Nbound <= 1) ? (int64_t)sub.Nbound : (int64_t)(sub.Nbound * source_sub_relax_factor);
+  if (ns > (int64_t)sub.Particles.size())
+    ns = (int64_t)sub.Particles.size();
+  sub.Particles.resize(ns);
+}
+
+// Subhalo_t::RecursiveUnbind (src/subhalo_unbind.cpp:810-840) for the
+// hierarchies rooted at `roots` (each subhalo's children: its NestedSubhalos),
+// with the same results as calling Subhalos[r].RecursiveUnbind(Subhalos, snap)
+// for every root in turn.  A subhalo depends only on its own hierarchy below it,
+// so the subhaloes are unbound one level at a time, every level in one batch:
+// height 0 (no children) first, then the subhaloes whose children are done.
+// Before a subhalo is unbound its children's unbound particles are appended in
+// NestedSubhalos order (:823); after it, its hierarchy is checked for phase-space
+// mergers, and a subhalo that gained particles by one is unbound again (:833-837).
+// Trees must be disjoint; a subhalo reached twice is an error.
+template <class SubhaloT>
+void RecursiveUnbind(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT> &Subhalos,
+                     const std::vector<int64_t> &roots, int snapshot_id, bool merge_trapped,
+                     uint32_t flags = HBTX_BATCHED, bool save_binding_energies = false,
+                     bool save_potential_energies = false)
+{
+  const int64_t ns = (int64_t)Subhalos.size();
+  std::vector<int32_t> height(ns, -1);
+  std::vector<char> seen(ns, 0);
+  std::vector<std::vector<int64_t>> levels;
+  // iterative post-order walk of every tree
+  std::vector<std::pair<int64_t, size_t>> stack;
+  for (int64_t r : roots)
+  {
+    if (r < 0 || r >= ns || seen[r])
+      throw Error(HBTX_E_INVALID, "RecursiveUnbind: bad or repeated root " + std::to_string(r));
+    seen[r] = 1;
+    stack.push_back({r, 0});
+    while (!stack.empty())
+    {
+      auto &top = stack.back();
+      const auto &kids = Subhalos[top.first].NestedSubhalos;
+      if (top.second < kids.size())
+      {
+        const int64_t c = (int64_t)kids[top.second++];
+        if (c < 0 || c >= ns || seen[c])
+          throw Error(HBTX_E_INVALID, "RecursiveUnbind: subhalo " + std::to_string(c) + " reached twice");
+        seen[c] = 1;
+        stack.push_back({c, 0});
+        continue;
+      }
+      int32_t h = 0;
+      for (auto c : kids)
+        h = std::max(h, height[c] + 1);
+      height[top.first] = h;
+      if ((int32_t)levels.size() <= h)
+        levels.resize(h + 1);
+      levels[h].push_back(top.first);
+      stack.pop_back();
+    }
+  }
+  for (auto &level : levels)
+  {
+    std::vector<SubhaloT *> batch;
+    batch.reserve(level.size());
+    for (int64_t s : level)
+    {
+      SubhaloT &S = Subhalos[s];
+      for (auto c : S.NestedSubhalos)
+      {
+        const SubhaloT &C = Subhalos[c];
+        S.Particles.insert(S.Particles.end(), C.Particles.begin() + C.Nbound, C.Particles.end());
+      }
+      batch.push_back(&S);
+    }
+    RefineParticles(ctx, prm, batch, snapshot_id, flags, save_binding_energies, save_potential_energies);
+    std::vector<SubhaloT *> again;
+    for (int64_t s : level)
+      if (MergeRecursively(Subhalos, s, Subhalos[s], snapshot_id, merge_trapped, prm) && merge_trapped)
+        again.push_back(&Subhalos[s]);
+    if (!again.empty())
+      RefineParticles(ctx, prm, again, snapshot_id, flags, save_binding_energies, save_potential_energies);
+  }
 }
 
 } // namespace hbtx

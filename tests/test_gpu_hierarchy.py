@@ -1,0 +1,102 @@
+"""Hierarchical unbinding (Subhalo_t::RecursiveUnbind, src/subhalo_unbind.cpp:810-840,
+with the phase-space mergers of src/subhalo_merge.cpp:45-91) through the C++
+host layer, one GPU batch per level of the hierarchies.  The driver
+(tests/cpp/hierarchy_test.cpp) first checks the level-batched result against
+the reference's depth-first recursion over single-subhalo GPU unbindings
+(identical subhaloes); here the result is compared with the same recursion over
+the CPU oracle (tests/hierarchy_oracle.py)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import hbtx
+import hierarchy_oracle as HO
+import pyoracle as O
+import synth
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(os.path.dirname(hbtx.LIB_PATH), "hierarchy_test")
+REC = np.dtype([("nbound", np.int64), ("sink", np.int64), ("np", np.int64), ("mbid", np.int64),
+                ("snap_sink", np.int32), ("snap_death", np.int32), ("core", np.float32, 8)])
+
+
+def run_driver(H, tmp_path, merge_trapped):
+    pop = H.pop
+    coff = np.concatenate([[0], np.cumsum([len(c) for c in H.children])]).astype(np.int64)
+    child = np.array([c for cs in H.children for c in cs], np.int64)
+    inp, outp = tmp_path / f"h{int(merge_trapped)}.bin", tmp_path / f"o{int(merge_trapped)}.bin"
+    with open(inp, "wb") as f:
+        np.array([pop.nsub, pop.n, len(H.roots), len(child), int(merge_trapped), 0], np.int64).tofile(f)
+        pop.offsets.astype(np.int64).tofile(f)
+        for a in (pop.pos, pop.vel, pop.mass):
+            np.ascontiguousarray(a, np.float32).tofile(f)
+        pop.pid.astype(np.int64).tofile(f)
+        for a in (pop.ref_pos, pop.ref_vel):
+            np.ascontiguousarray(a, np.float32).tofile(f)
+        for a in (H.track, H.sink, coff, child, np.asarray(H.roots, np.int64)):
+            np.asarray(a, np.int64).tofile(f)
+    r = subprocess.run([BIN, str(inp), str(outp)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    words = r.stdout.split()
+    info = dict(zip(words[::2], map(int, words[1::2])))
+    raw = open(outp, "rb").read()
+    subs = np.frombuffer(raw, REC, count=pop.nsub)
+    ids = np.frombuffer(raw, np.int64, offset=REC.itemsize * pop.nsub)
+    lists = np.split(ids, np.cumsum(subs["np"])[:-1])
+    return info, subs, lists
+
+
+@pytest.mark.parametrize("merge_trapped", [False, True])
+def test_recursive_unbind_matches_oracle_recursion(tmp_path, merge_trapped):
+    assert os.path.exists(BIN), "build the C++ test driver first (make -C hbt-herons_amd)"
+    H = synth.hierarchy(seed=5)
+    info, gpu, lists = run_driver(H, tmp_path, merge_trapped)
+    # the level-batched result equals the depth-first recursion, and the case is not trivial
+    assert info["identical"] == 1 and info["truncate_ok"] == 1
+    assert info["levels"] >= 4 and info["merged"] >= 2
+    ref = HO.run(H, O.default_params(), 7, merge_trapped)
+    # the same mergers (sink track, snapshots) in every subhalo
+    assert np.array_equal(gpu["sink"], [nd.sink for nd in ref])
+    assert np.array_equal(gpu["snap_sink"], [nd.snap_sink for nd in ref])
+    assert np.array_equal(gpu["snap_death"], [nd.snap_death for nd in ref])
+    # bound sets: FP32 walk vs the oracle's FP64 walk, <= 1e-3 of the bound mass (equal masses)
+    diff = tot = 0
+    for s, nd in enumerate(ref):
+        g = set(lists[s][:gpu["nbound"][s]].tolist())
+        r = set(nd.pid[:nd.nbound].tolist())
+        diff += len(g ^ r)
+        tot += len(r)
+        assert abs(int(gpu["nbound"][s]) - nd.nbound) <= max(2, nd.nbound // 200), s
+        # sources gathered from the children: the same particles
+        assert set(lists[s].tolist()) == set(nd.pid.tolist()), s
+    assert diff <= 1e-3 * tot
+    # merged subhaloes hand their bound particles over and keep none
+    if merge_trapped:
+        sunk = [s for s, nd in enumerate(ref) if nd.sink != -1 and H.sink[s] == -1]
+        assert sunk and all(gpu["np"][s] == 0 for s in sunk)
+
+
+def test_repeated_child_is_rejected(tmp_path):
+    """A child listed twice is rejected before anything runs."""
+    assert os.path.exists(BIN)
+    H = synth.hierarchy(seed=6, nhost=1)
+    H.children[H.roots[0]].append(H.children[H.roots[0]][0])
+    coff = np.concatenate([[0], np.cumsum([len(c) for c in H.children])]).astype(np.int64)
+    child = np.array([c for cs in H.children for c in cs], np.int64)
+    pop = H.pop
+    inp, outp = tmp_path / "bad.bin", tmp_path / "bad_out.bin"
+    with open(inp, "wb") as f:
+        np.array([pop.nsub, pop.n, len(H.roots), len(child), 0, 0], np.int64).tofile(f)
+        pop.offsets.astype(np.int64).tofile(f)
+        for a in (pop.pos, pop.vel, pop.mass):
+            np.ascontiguousarray(a, np.float32).tofile(f)
+        pop.pid.astype(np.int64).tofile(f)
+        for a in (pop.ref_pos, pop.ref_vel):
+            np.ascontiguousarray(a, np.float32).tofile(f)
+        for a in (H.track, H.sink, coff, child, np.asarray(H.roots, np.int64)):
+            np.asarray(a, np.int64).tofile(f)
+    r = subprocess.run([BIN, str(inp), str(outp)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 4 and "reached twice" in r.stderr
