@@ -246,6 +246,12 @@ def main():
     st = ctx.stats()
     ctx.run(prm, hbtx.COUNT)
     st_plain = ctx.stats()
+    # post-unbinding properties (CalculateProfileProperties + CalculateShape) of that run
+    ctx.profiles(prm)
+    ctx.sync()
+    t0 = time.perf_counter()
+    prof = ctx.profiles(prm)
+    prof_ms = (time.perf_counter() - t0) * 1e3
     st["walk_ms"] = walk_ms
     ms_step = float(np.mean(ms_learned))
     value = pop.n / (ms_step / 1e3)
@@ -284,6 +290,8 @@ def main():
                      "kernel_avg_launch_ms": walk_ms / max(1, rounds), "flop_per_step": flops},
         "bound_mass_mismatch_vs_plain": mismatch,
         "mismatch_detail": mism,
+        "profiles_and_shape": {"ms": prof_ms, "subhalos_with_r200": int(prof["has_r200"].sum()),
+                               "note": "host wall time of hbtx_batch_profiles, results copied to the host"},
         "bound_mass_precision_1": prec1,
         "batched_without_learned": {"value": batched_value, "unit": "particles/s",
                                     "ms_per_step": float(np.mean(ms_plain_b)), "rounds": int(st_plain["rounds"]),

@@ -13,6 +13,7 @@
 #include "subsample.h"
 #include "learn.h"
 #include "tiles.h"
+#include "profile.h"
 
 #define WALK_MAX_CHUNKS 16
 
@@ -160,6 +161,8 @@ struct Batch
   DArr<int32_t> wctr; // walk work counters, 16 ints apart
   DArr<int32_t> cells, depth_cnt; // tree cells bucketed by depth; counts, offsets, cursors
   DArr<hbtx_sub_result> dres;
+  DArr<float> prof_r, prof_m; // profiles: sorted radii and cumulative masses
+  DArr<hbtx_sub_profile> dprof;
   bool ran = false;
 };
 
@@ -1896,6 +1899,56 @@ int hbtx_direct_potential(hbtx_ctx *c, const hbtx_params *prm, int64_t n_src, co
     HIPCHK(hipSetDevice(c->device));
     build_tree(c, *prm, n_src, src_pos, src_mass, &T, 0);
     tree_points(c, &T, n_tgt, tgt_pos, tgt_mass, out_pot);
+  });
+}
+
+int hbtx_batch_profiles(hbtx_ctx *c, const hbtx_params *prm, hbtx_sub_profile *out)
+{
+  if (!c || !prm || !out)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    Batch &B = c->B;
+    REQUIRE(B.ran, HBTX_E_INVALID, "batch has not been run");
+    hbtx_params p = *prm;
+    if (p.direct_cutoff < 0)
+      p.direct_cutoff = c->direct_cutoff;
+    const DevParams P = make_dev_params(p);
+    hipStream_t st = c->stream;
+    const int64_t n = B.n;
+    const int32_t nsub = B.nsub;
+    B.prof_r.alloc(std::max<int64_t>(n, 1));
+    B.prof_m.alloc(std::max<int64_t>(n, 1));
+    B.dprof.alloc(std::max<int32_t>(nsub, 1));
+    ProfArgs A;
+    A.off = B.off.p;
+    A.ford = B.ford.p;
+    A.posm = B.posm.p;
+    A.res = B.dres.p;
+    A.seg_len = B.seg_len.p;
+    A.keys = B.keys.p;
+    A.vals = B.src.p;
+    A.rc = B.prof_r.p;
+    A.mc = B.prof_m.p;
+    A.P = P;
+    // as the reference types them: G, a, H and the factor 200 are floats (src/snapshot.cpp:268-269)
+    const float G = (float)p.G, a = p.scale_factor;
+    A.rho = (float)((double)(200.f * p.hz * p.hz) / 2.0 / (double)G * (double)a * (double)a * (double)a);
+    A.vel_unit = G / a;
+    const int gs = blocks_for(nsub, 256, 1 << 20);
+    k_prof_segments<<<gs, 256, 0, st>>>(nsub, B.off.p, B.dres.p, B.seg_begin.p, B.seg_len.p);
+    k_prof_keys<<<blocks_for(n), 256, 0, st>>>(A, n, B.sub_of.p, B.keys.p, B.src.p);
+    seg_sort(c, B.keys.p, B.src.p, nsub, B.seg_begin.p, B.seg_len.p, n, 32);
+    const int32_t *nt = red_tiles(c, B.seg_begin.p, B.seg_len.p, nsub, n);
+    A.part = c->red_partial.p;
+    const int rg = red_grid(c, n, nsub);
+    k_prof_tile_sum<<<rg, RED_THREADS, 0, st>>>(A, c->red_tiles.p, nt);
+    k_prof_tile_prefix<<<gs, 256, 0, st>>>(nsub, c->red_toff.p, c->red_ntile.p, c->red_partial.p);
+    k_prof_tile_main<<<rg, RED_THREADS, 0, st>>>(A, c->red_tiles.p, nt);
+    k_prof_finish<<<gs, 256, 0, st>>>(A, nsub, c->red_toff.p, c->red_ntile.p, B.dprof.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, B.dprof.p, nsub * sizeof(hbtx_sub_profile), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
   });
 }
 

@@ -63,6 +63,12 @@ class Params(C.Structure):
     ]
 
 
+# hbtx_sub_profile (CalculateProfileProperties + CalculateShape)
+PROFILE_DTYPE = np.dtype([
+    ("rmax", np.float32), ("vmax", np.float32), ("r_half", np.float32), ("r_enclose", np.float32),
+    ("r200", np.float32), ("m200", np.float32), ("inertia", np.float32, 6), ("inertia_w", np.float32, 6),
+    ("has_r200", np.int32), ("reserved", np.int32)])
+
 RESULT_DTYPE = np.dtype([
     ("nbound", np.int64), ("mbound", np.float32), ("nbound_type", np.int64, 6),
     ("mbound_type", np.float32, 6), ("com_pos", np.float32, 3), ("com_vel", np.float32, 3),
@@ -139,6 +145,7 @@ def lib():
         L.hbtx_tree_free.restype = None
         L.hbtx_direct_potential.argtypes = [P, C.POINTER(Params), i64, P, P, i64, P, P, P]
         L.hbtx_segmented_sort.argtypes = [P, i64, P, P, i64, P]
+        L.hbtx_batch_profiles.argtypes = [P, C.POINTER(Params), P]
         L.hbtx_mlp_create.argtypes = [P, C.c_uint64, C.POINTER(C.c_void_p)]
         L.hbtx_mlp_free.argtypes = [P]
         L.hbtx_mlp_free.restype = None
@@ -294,6 +301,14 @@ class Context:
         out = np.zeros(len(tgt_pos), np.float64)
         self._check(lib().hbtx_direct_potential(self._h, C.byref(prm), len(src_pos), _p(src_pos), _p(src_mass),
                                                 len(tgt_pos), _p(tgt_pos), _p(tgt_mass), _p(out)))
+        return out
+
+    def profiles(self, params=None):
+        """Subhalo_t::CalculateProfileProperties + CalculateShape of the last run's
+        subhaloes (PROFILE_DTYPE per subhalo)."""
+        prm = params if isinstance(params, Params) else make_params(**(params or {}))
+        out = np.zeros(self._nsub, PROFILE_DTYPE)
+        self._check(lib().hbtx_batch_profiles(self._h, C.byref(prm), _p(out)))
         return out
 
     def segmented_sort(self, keys, vals, offsets):

@@ -132,6 +132,26 @@ int hbtx_batch_set_proposal(hbtx_ctx *ctx, const uint8_t *proposal);
 int hbtx_batch_run(hbtx_ctx *ctx, const hbtx_params *prm, uint32_t flags, double *elapsed_ms);
 int hbtx_batch_fetch(hbtx_ctx *ctx, int64_t *out_order, float *out_energy, float *out_potential,
                      hbtx_sub_result *out_res);
+/* Post-unbinding properties of the last run's subhaloes, centred on the most bound
+ * particle: Subhalo_t::CalculateProfileProperties (src/subhalo.cpp:175-243, with
+ * Snapshot_t::SphericalOverdensitySize, src/snapshot.cpp:258-282) and
+ * Subhalo_t::CalculateShape (src/subhalo.cpp:245-309).  prm supplies G,
+ * scale_factor, hz, the softening and the periodic box.  All zero when Nbound <= 1. */
+typedef struct
+{
+  float rmax_comoving;            /* RmaxComoving */
+  float vmax_physical;            /* VmaxPhysical */
+  float r_half_comoving;          /* RHalfComoving */
+  float r_enclose_comoving;       /* REncloseComoving */
+  float bound_r200_crit_comoving; /* BoundR200CritComoving (when has_r200) */
+  float bound_m200_crit;          /* BoundM200Crit (when has_r200) */
+  float inertial_tensor[6];       /* InertialTensor {xx, xy, xz, yy, yz, zz} */
+  float inertial_tensor_weighted[6]; /* InertialTensorWeighted */
+  int32_t has_r200;               /* 0: no radius met the overdensity; the reference keeps the old values */
+  int32_t reserved;
+} hbtx_sub_profile;
+int hbtx_batch_profiles(hbtx_ctx *ctx, const hbtx_params *prm, hbtx_sub_profile *out);
+
 /* Kernel-level counters of the last run (work counts need HBTX_COUNT): stats[0..15], see hbtx_stat_name;
  * 12 walk ms, 13 rounds, 14 rounds replayed from graphs, 15 graph captures. */
 int hbtx_batch_stats(hbtx_ctx *ctx, double *stats, int nstats);

@@ -15,6 +15,9 @@
 //                              one batch per level of the hierarchies
 //   hbtx::TruncateSource(sub, relax_factor)
 //                              Subhalo_t::TruncateSource    src/subhalo_unbind.cpp:842-852
+//   hbtx::CalculateProfileProperties(ctx, prm, subs, snapshot_id, shape)
+//                              Subhalo_t::CalculateProfileProperties / CalculateShape
+//                              src/subhalo.cpp:175-309
 //
 // The templates accept any subhalo type with the reference's member names
 // (Particles with Id / ComovingPosition / PhysicalVelocity / Mass and, when
@@ -121,6 +124,7 @@ HBTX_MEMBER_TRAIT(DescendantTrackId)
 HBTX_MEMBER_TRAIT(SinkTrackId)
 HBTX_MEMBER_TRAIT(SnapshotOfSink)
 HBTX_MEMBER_TRAIT(Type)
+HBTX_MEMBER_TRAIT(InertialTensor)
 #undef HBTX_MEMBER_TRAIT
 
 template <class V>
@@ -362,6 +366,54 @@ void Unbind(Context &ctx, const hbtx_params &prm, SubhaloT &sub, int snapshot_id
 {
   std::vector<SubhaloT *> one{&sub};
   RefineParticles(ctx, prm, one, snapshot_id, HBTX_PLAIN, save_binding_energies, save_potential_energies);
+}
+
+// Subhalo_t::CalculateProfileProperties (src/subhalo.cpp:175-243) and, with
+// `shape`, Subhalo_t::CalculateShape (:245-309) for the subhaloes of the last
+// RefineParticles call on this context (the same list, in the same order),
+// computed on the device from that run's final order.  prm gives G, the scale
+// factor, H(z) and the softening of the epoch.
+template <class SubhaloT>
+void CalculateProfileProperties(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT *> &subs, int snapshot_id,
+                                bool shape = true)
+{
+  std::vector<hbtx_sub_profile> pr(subs.size());
+  ctx.check(hbtx_batch_profiles(ctx.get(), &prm, pr.data()));
+  for (size_t s = 0; s < subs.size(); s++)
+  {
+    SubhaloT &S = *subs[s];
+    const hbtx_sub_profile &P = pr[s];
+    S.RmaxComoving = P.rmax_comoving;
+    S.VmaxPhysical = P.vmax_physical;
+    S.RHalfComoving = P.r_half_comoving;
+    S.REncloseComoving = P.r_enclose_comoving;
+    if (S.Nbound <= 1)
+    {
+      S.BoundR200CritComoving = 0.f;
+      S.BoundM200Crit = 0.f;
+    }
+    else
+    {
+      if (P.has_r200)
+      {
+        S.BoundR200CritComoving = P.bound_r200_crit_comoving;
+        S.BoundM200Crit = P.bound_m200_crit;
+      }
+      if (S.VmaxPhysical >= S.LastMaxVmaxPhysical)
+      {
+        S.SnapshotOfLastMaxVmax = snapshot_id;
+        S.LastMaxVmaxPhysical = S.VmaxPhysical;
+        S.RmaxComovingOfLastMaxVmax = S.RmaxComoving;
+      }
+    }
+    if constexpr (detail::has_InertialTensor<SubhaloT>::value)
+      if (shape)
+        for (int k = 0; k < 6; k++)
+        {
+          S.InertialTensor[k] = P.inertial_tensor[k];
+          S.InertialTensorWeighted[k] = P.inertial_tensor_weighted[k];
+        }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -1228,3 +1228,121 @@ extern "C" void oracle_core_phase_space(const ounb_params *prm, int64_t nbound, 
   *sigma_r = sr;
   *sigma_v = sv;
 }
+
+// ---------------------------------------------------------------------------
+// Subhalo_t::CalculateProfileProperties (src/subhalo.cpp:175-243) with
+// Snapshot_t::SphericalOverdensitySize (src/snapshot.cpp:258-282), and
+// Subhalo_t::CalculateShape (src/subhalo.cpp:245-309).
+// ---------------------------------------------------------------------------
+namespace
+{
+struct RadMassVel
+{
+  float r, m, v;
+};
+
+float periodic_distance(const ounb_params *prm, const float *x, const float *y)
+{
+  float dx[3];
+  for (int k = 0; k < 3; k++)
+  {
+    dx[k] = x[k] - y[k];
+    if (prm->periodic)
+    {
+      const float half = prm->box_size / 2.f;
+      dx[k] = (dx[k] > half) ? dx[k] - prm->box_size : ((dx[k] < -half) ? dx[k] + prm->box_size : dx[k]);
+    }
+  }
+  return std::sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+}
+
+void profile_one(const ounb_params *prm, const float *pos, const float *mass, int64_t nb, float mbound,
+                 ounb_profile *o)
+{
+  std::memset(o, 0, sizeof(*o));
+  if (nb <= 1)
+    return;
+  const float G = (float)prm->G;
+  const float vel_unit = G / prm->scale_factor;
+  const float *cen = pos; // most bound particle (final slot 0)
+  std::vector<RadMassVel> prof(nb);
+  for (int64_t i = 0; i < nb; i++)
+  {
+    prof[i].r = periodic_distance(prm, cen, pos + 3 * i);
+    prof[i].m = mass[i];
+  }
+  std::sort(prof.begin(), prof.end(), [](const RadMassVel &a, const RadMassVel &b) { return a.r < b.r; });
+  double m_cum = 0.;
+  for (auto &p : prof)
+    p.m = (float)(m_cum += p.m);
+  const float eps = (prm->max_phys_softening == -1.f)
+                      ? prm->softening
+                      : std::min(prm->softening, prm->max_phys_softening / prm->scale_factor);
+  for (auto &p : prof)
+  {
+    if (p.r < eps)
+      p.r = eps;
+    p.v = p.m / p.r;
+  }
+  auto best = std::max_element(prof.begin(), prof.end(),
+                               [](const RadMassVel &a, const RadMassVel &b) { return a.v < b.v; });
+  o->rmax_comoving = best->r;
+  o->vmax_physical = std::sqrt(best->v * vel_unit);
+  o->r_half_comoving = prof[nb / 2].r;
+  o->r_enclose_comoving = prof[nb - 1].r;
+  const float rho = (float)((double)(200.f * prm->hz * prm->hz) / 2.0 / (double)G * (double)prm->scale_factor *
+                            (double)prm->scale_factor * (double)prm->scale_factor);
+  for (auto p = prof.rbegin(); p != prof.rend(); ++p)
+    if (p->m > rho * p->r * p->r * p->r)
+    {
+      o->bound_m200_crit = p->m;
+      o->bound_r200_crit_comoving = (float)std::pow(p->m / rho, 1.0 / 3);
+      o->has_r200 = 1;
+      break;
+    }
+  double I[6] = {0, 0, 0, 0, 0, 0}, W[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t i = 1; i < nb; i++)
+  {
+    const float m = mass[i];
+    float d[3];
+    for (int k = 0; k < 3; k++)
+    {
+      d[k] = pos[3 * i + k] - cen[k];
+      if (prm->periodic)
+      {
+        const float half = prm->box_size / 2.f;
+        d[k] = (d[k] > half) ? d[k] - prm->box_size : ((d[k] < -half) ? d[k] + prm->box_size : d[k]);
+      }
+    }
+    const float dx2 = d[0] * d[0], dy2 = d[1] * d[1], dz2 = d[2] * d[2];
+    I[0] += dx2 * m;
+    I[3] += dy2 * m;
+    I[5] += dz2 * m;
+    I[1] += d[0] * d[1] * m;
+    I[2] += d[0] * d[2] * m;
+    I[4] += d[1] * d[2] * m;
+    float dr2 = dx2 + dy2 + dz2;
+    dr2 /= m;
+    W[0] += dx2 / dr2;
+    W[3] += dy2 / dr2;
+    W[5] += dz2 / dr2;
+    W[1] += d[0] * d[1] / dr2;
+    W[2] += d[0] * d[2] / dr2;
+    W[4] += d[1] * d[2] / dr2;
+  }
+  for (int k = 0; k < 6; k++)
+  {
+    o->inertial_tensor[k] = (float)I[k] / mbound;
+    o->inertial_tensor_weighted[k] = (float)W[k] / mbound;
+  }
+}
+} // namespace
+
+extern "C" void oracle_profiles(const ounb_params *prm, int64_t n, const float *pos, const float *mass, int64_t nsub,
+                                const int64_t *offsets, const int64_t *nbound, const float *mbound, ounb_profile *out)
+{
+  (void)n;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t s = 0; s < nsub; s++)
+    profile_one(prm, pos + 3 * offsets[s], mass + offsets[s], nbound[s], mbound[s], out + s);
+}

@@ -115,6 +115,27 @@ void oracle_core_phase_space(const ounb_params *prm, int64_t nbound, const float
                              const float *mass, const int32_t *type, float *core_pos, float *core_vel,
                              float *sigma_r, float *sigma_v);
 
+/* Subhalo_t::CalculateProfileProperties and CalculateShape (src/subhalo.cpp:175-309). */
+typedef struct
+{
+  float rmax_comoving;            /* RmaxComoving */
+  float vmax_physical;            /* VmaxPhysical */
+  float r_half_comoving;          /* RHalfComoving */
+  float r_enclose_comoving;       /* REncloseComoving */
+  float bound_r200_crit_comoving; /* BoundR200CritComoving */
+  float bound_m200_crit;          /* BoundM200Crit */
+  float inertial_tensor[6];       /* InertialTensor {xx, xy, xz, yy, yz, zz} */
+  float inertial_tensor_weighted[6];
+  int32_t has_r200;               /* 0: no radius met the overdensity (the reference leaves the fields as they were) */
+  int32_t reserved;
+} ounb_profile;
+
+/* Per subhalo: its bound particles [offsets[s], offsets[s] + nbound[s]) in final
+ * order (the first is the most bound, the centre, src/subhalo_unbind.cpp:754),
+ * mbound[s] its Mbound. */
+void oracle_profiles(const ounb_params *prm, int64_t n, const float *pos, const float *mass, int64_t nsub,
+                     const int64_t *offsets, const int64_t *nbound, const float *mbound, ounb_profile *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -118,6 +118,8 @@ def lib():
         L.oracle_tree_dfs.argtypes = [C.POINTER(Params), C.c_int64, P, P, C.c_int64, P, P, P, P, P, P, P]
         L.oracle_core_phase_space.restype = None
         L.oracle_core_phase_space.argtypes = [C.POINTER(Params), C.c_int64, P, P, P, P, P, P, P, P]
+        L.oracle_profiles.restype = None
+        L.oracle_profiles.argtypes = [C.POINTER(Params), C.c_int64, P, P, C.c_int64, P, P, P, P]
         _lib = L
     return _lib
 
@@ -209,3 +211,22 @@ def core_phase_space(pos, vel, mass, ptype=None, params=None):
     lib().oracle_core_phase_space(C.byref(prm), len(pos), _p(pos), _p(vel), _p(mass), _p(pt), _p(cp), _p(cv),
                                   C.byref(sr), C.byref(sv))
     return cp, cv, sr.value, sv.value
+
+
+PROFILE_DTYPE = np.dtype([
+    ("rmax", np.float32), ("vmax", np.float32), ("r_half", np.float32), ("r_enclose", np.float32),
+    ("r200", np.float32), ("m200", np.float32), ("inertia", np.float32, 6), ("inertia_w", np.float32, 6),
+    ("has_r200", np.int32), ("reserved", np.int32)])
+
+
+def profiles(pos, mass, offsets, nbound, mbound, params=None):
+    """Subhalo_t::CalculateProfileProperties + CalculateShape of every subhalo's
+    bound set, particles given in final order from offsets[s] (first = centre)."""
+    prm = params if isinstance(params, Params) else make_params(**(params or {}))
+    pos, mass = _f32(pos), _f32(mass)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    nb = np.ascontiguousarray(nbound, dtype=np.int64)
+    mb = _f32(mbound)
+    out = np.zeros(len(nb), PROFILE_DTYPE)
+    lib().oracle_profiles(C.byref(prm), len(mass), _p(pos), _p(mass), len(nb), _p(off), _p(nb), _p(mb), _p(out))
+    return out

@@ -9,7 +9,9 @@
 //           float com[6], int32 snapshot_of_death; then int64 final ids[n];
 //           then Unbind() of subhalo 0: int64 nbound, int64 ids[n0];
 //           then GravityTree over subhalo 0: double pot[k], double be[k] for the
-//           first k = min(16, n0) particles.
+//           first k = min(16, n0) particles; then per subhalo float
+//           {Rmax, Vmax, RHalf, REnclose, Ixx, Izz weighted}, int32
+//           SnapshotOfLastMaxVmax.
 #include "hbtx_compat.hpp"
 
 #include <array>
@@ -39,6 +41,11 @@ struct Subhalo_t
   int64_t SinkTrackId = -1, DescendantTrackId = -1;
   HBTxyz CoreComovingPosition, CorePhysicalVelocity;
   float CoreComovingSigmaR = 0, CorePhysicalSigmaV = 0;
+  float RmaxComoving = 0, VmaxPhysical = 0, RHalfComoving = 0, REncloseComoving = 0;
+  float BoundR200CritComoving = 0, BoundM200Crit = 0;
+  float LastMaxVmaxPhysical = 0, RmaxComovingOfLastMaxVmax = 0;
+  int SnapshotOfLastMaxVmax = -1;
+  float InertialTensor[6] = {0, 0, 0, 0, 0, 0}, InertialTensorWeighted[6] = {0, 0, 0, 0, 0, 0};
   std::vector<Particle_t> Particles;
   std::vector<float> ParticleBindingEnergies;
 };
@@ -127,6 +134,7 @@ int main(int argc, char **argv)
     for (auto &s : subs)
       ptrs.push_back(&s);
     hbtx::RefineParticles(ctx, prm, ptrs, 7, HBTX_BATCHED, true, false);
+    hbtx::CalculateProfileProperties(ctx, prm, ptrs, 7);
     FILE *o = fopen(argv[2], "wb");
     for (auto &S : subs)
     {
@@ -169,6 +177,14 @@ int main(int argc, char **argv)
       double be = tree.BindingEnergy(p.ComovingPosition, p.PhysicalVelocity, fresh[0].ComovingAveragePosition,
                                      fresh[0].PhysicalAverageVelocity, p.Mass);
       wr(o, &be, 1);
+    }
+    // profile and shape of every subhalo of the batch
+    for (auto &S : subs)
+    {
+      const float pr[6] = {S.RmaxComoving, S.VmaxPhysical, S.RHalfComoving, S.REncloseComoving,
+                           S.InertialTensor[0], S.InertialTensorWeighted[5]};
+      wr(o, pr, 6);
+      wr(o, &S.SnapshotOfLastMaxVmax, 1);
     }
     fclose(o);
   }

@@ -64,3 +64,17 @@ def test_cpp_host_layer_matches_c_abi(ctx, tmp_path):
     np.testing.assert_allclose(pot, t.potential(pop.pos[:m], pop.mass[:m]), rtol=1e-6)
     np.testing.assert_allclose(be, t.binding_energy(pop.pos[:m], pop.vel[:m], pop.ref_pos[0], pop.ref_vel[0],
                                                     pop.mass[:m]), rtol=1e-6)
+    # CalculateProfileProperties / CalculateShape written back by the host layer
+    k += 16 * m
+    prec = np.dtype([("f", np.float32, 6), ("snap", np.int32)])
+    pr = np.frombuffer(raw, prec, count=pop.nsub, offset=k)
+    ctx.unbind(pop, prm, hbtx.BATCHED)
+    g = ctx.profiles(prm)
+    np.testing.assert_array_equal(pr["f"][:, 0], g["rmax"])
+    np.testing.assert_array_equal(pr["f"][:, 1], g["vmax"])
+    np.testing.assert_array_equal(pr["f"][:, 2], g["r_half"])
+    np.testing.assert_array_equal(pr["f"][:, 3], g["r_enclose"])
+    np.testing.assert_array_equal(pr["f"][:, 4], g["inertia"][:, 0])
+    np.testing.assert_array_equal(pr["f"][:, 5], g["inertia_w"][:, 5])
+    live = res["nbound"] > 1
+    assert np.all(pr["snap"][live] == 7) and np.all(pr["snap"][~live] == -1)
