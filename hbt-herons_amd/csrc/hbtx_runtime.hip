@@ -364,6 +364,41 @@ int red_grid(hbtx_ctx *c, int64_t n_cap, int32_t nsub)
   return (int)std::max<int64_t>(1, std::min<int64_t>(n_cap / RED_TILE + nsub + 1, 8 * c->num_cu));
 }
 
+DevParams make_dev_params(const hbtx_params &p);
+const int32_t *red_tiles(hbtx_ctx *c, const int32_t *begin, const int32_t *len, int32_t nsub, int64_t n_cap);
+int red_grid(hbtx_ctx *c, int64_t n_cap, int32_t nsub);
+
+// Profiles and shapes of the bound sets described by A (positions, final order,
+// results): sort by radius, tiled cumulative mass and candidates, per-subhalo
+// combination into out (device).
+void run_profiles(hbtx_ctx *c, const hbtx_params &prm, ProfArgs &A, int64_t n, int32_t nsub, const int32_t *sub_of,
+                  int32_t *seg_begin, int32_t *seg_len, uint64_t *keys, int32_t *vals, hbtx_sub_profile *out)
+{
+  hbtx_params p = prm;
+  if (p.direct_cutoff < 0)
+    p.direct_cutoff = c->direct_cutoff;
+  A.P = make_dev_params(p);
+  A.keys = keys;
+  A.vals = vals;
+  // as the reference types them: G, a, H and the factor 200 are floats (src/snapshot.cpp:268-269)
+  const float G = (float)p.G, a = p.scale_factor;
+  A.rho = (float)((double)(200.f * p.hz * p.hz) / 2.0 / (double)G * (double)a * (double)a * (double)a);
+  A.vel_unit = G / a;
+  hipStream_t st = c->stream;
+  const int gs = blocks_for(nsub, 256, 1 << 20);
+  k_prof_segments<<<gs, 256, 0, st>>>(nsub, A.off, A.res, seg_begin, seg_len);
+  k_prof_keys<<<blocks_for(n), 256, 0, st>>>(A, n, sub_of, keys, vals);
+  seg_sort(c, keys, vals, nsub, seg_begin, seg_len, n, 32);
+  const int32_t *nt = red_tiles(c, seg_begin, seg_len, nsub, n);
+  A.part = c->red_partial.p;
+  const int rg = red_grid(c, n, nsub);
+  k_prof_tile_sum<<<rg, RED_THREADS, 0, st>>>(A, c->red_tiles.p, nt);
+  k_prof_tile_prefix<<<gs, 256, 0, st>>>(nsub, c->red_toff.p, c->red_ntile.p, c->red_partial.p);
+  k_prof_tile_main<<<rg, RED_THREADS, 0, st>>>(A, c->red_tiles.p, nt);
+  k_prof_finish<<<gs, 256, 0, st>>>(A, nsub, c->red_toff.p, c->red_ntile.p, out);
+  HIPCHK(hipGetLastError());
+}
+
 DevParams make_dev_params(const hbtx_params &p)
 {
   REQUIRE(p.scale_factor > 0.f, HBTX_E_INVALID, "scale_factor must be positive");
@@ -1910,44 +1945,91 @@ int hbtx_batch_profiles(hbtx_ctx *c, const hbtx_params *prm, hbtx_sub_profile *o
     HIPCHK(hipSetDevice(c->device));
     Batch &B = c->B;
     REQUIRE(B.ran, HBTX_E_INVALID, "batch has not been run");
-    hbtx_params p = *prm;
-    if (p.direct_cutoff < 0)
-      p.direct_cutoff = c->direct_cutoff;
-    const DevParams P = make_dev_params(p);
-    hipStream_t st = c->stream;
-    const int64_t n = B.n;
-    const int32_t nsub = B.nsub;
-    B.prof_r.alloc(std::max<int64_t>(n, 1));
-    B.prof_m.alloc(std::max<int64_t>(n, 1));
-    B.dprof.alloc(std::max<int32_t>(nsub, 1));
+    B.prof_r.alloc(std::max<int64_t>(B.n, 1));
+    B.prof_m.alloc(std::max<int64_t>(B.n, 1));
+    B.dprof.alloc(std::max<int32_t>(B.nsub, 1));
     ProfArgs A;
     A.off = B.off.p;
     A.ford = B.ford.p;
     A.posm = B.posm.p;
     A.res = B.dres.p;
     A.seg_len = B.seg_len.p;
-    A.keys = B.keys.p;
-    A.vals = B.src.p;
     A.rc = B.prof_r.p;
     A.mc = B.prof_m.p;
-    A.P = P;
-    // as the reference types them: G, a, H and the factor 200 are floats (src/snapshot.cpp:268-269)
-    const float G = (float)p.G, a = p.scale_factor;
-    A.rho = (float)((double)(200.f * p.hz * p.hz) / 2.0 / (double)G * (double)a * (double)a * (double)a);
-    A.vel_unit = G / a;
-    const int gs = blocks_for(nsub, 256, 1 << 20);
-    k_prof_segments<<<gs, 256, 0, st>>>(nsub, B.off.p, B.dres.p, B.seg_begin.p, B.seg_len.p);
-    k_prof_keys<<<blocks_for(n), 256, 0, st>>>(A, n, B.sub_of.p, B.keys.p, B.src.p);
-    seg_sort(c, B.keys.p, B.src.p, nsub, B.seg_begin.p, B.seg_len.p, n, 32);
-    const int32_t *nt = red_tiles(c, B.seg_begin.p, B.seg_len.p, nsub, n);
-    A.part = c->red_partial.p;
-    const int rg = red_grid(c, n, nsub);
-    k_prof_tile_sum<<<rg, RED_THREADS, 0, st>>>(A, c->red_tiles.p, nt);
-    k_prof_tile_prefix<<<gs, 256, 0, st>>>(nsub, c->red_toff.p, c->red_ntile.p, c->red_partial.p);
-    k_prof_tile_main<<<rg, RED_THREADS, 0, st>>>(A, c->red_tiles.p, nt);
-    k_prof_finish<<<gs, 256, 0, st>>>(A, nsub, c->red_toff.p, c->red_ntile.p, B.dprof.p);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out, B.dprof.p, nsub * sizeof(hbtx_sub_profile), hipMemcpyDeviceToHost, st));
+    run_profiles(c, *prm, A, B.n, B.nsub, B.sub_of.p, B.seg_begin.p, B.seg_len.p, B.keys.p, B.src.p, B.dprof.p);
+    HIPCHK(hipMemcpyAsync(out, B.dprof.p, B.nsub * sizeof(hbtx_sub_profile), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int hbtx_profiles(hbtx_ctx *c, const hbtx_params *prm, int64_t n, const float *pos, const float *mass, int64_t nsub,
+                  const int64_t *offsets, const int64_t *nbound, const float *mbound, hbtx_sub_profile *out)
+{
+  if (!c || !prm || !out || (n > 0 && (!pos || !mass)) || !offsets || !nbound || !mbound || nsub < 1)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    REQUIRE(n < (int64_t)0x20000000 && nsub < (int64_t)0x7fffffff, HBTX_E_INVALID, "too large");
+    REQUIRE(offsets[0] == 0 && offsets[nsub] == n, HBTX_E_INVALID, "offsets must start at 0 and end at n");
+    hipStream_t st = c->stream;
+    std::vector<float4> hp(std::max<int64_t>(n, 1));
+    std::vector<int32_t> hoff(nsub + 1), hsub(std::max<int64_t>(n, 1)), hord(std::max<int64_t>(n, 1));
+    std::vector<hbtx_sub_result> hres(nsub);
+    for (int64_t i = 0; i < n; i++)
+      hp[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], mass[i]);
+    for (int64_t s = 0; s < nsub; s++)
+    {
+      const int64_t a = offsets[s], e = offsets[s + 1];
+      REQUIRE(a <= e && e <= n && nbound[s] >= 0 && nbound[s] <= e - a, HBTX_E_INVALID,
+              "bound sets must lie inside ascending subhalo ranges");
+      hoff[s] = (int32_t)a;
+      hbtx_sub_result &R = hres[s];
+      std::memset(&R, 0, sizeof(R));
+      R.nbound = nbound[s];
+      R.mbound = mbound[s];
+      if (nbound[s] > 0)
+        for (int k = 0; k < 3; k++)
+          R.mostbound_pos[k] = pos[3 * a + k];
+      for (int64_t i = a; i < e; i++)
+      {
+        hsub[i] = (int32_t)s;
+        hord[i] = (int32_t)(i - a);
+      }
+    }
+    hoff[nsub] = (int32_t)n;
+    DArr<float4> dp;
+    DArr<int32_t> doff, dsub, dord, sb, sl, vals;
+    DArr<uint64_t> keys;
+    DArr<hbtx_sub_result> dres;
+    DArr<float> rc, mc;
+    DArr<hbtx_sub_profile> dout;
+    dp.alloc(hp.size());
+    doff.alloc(nsub + 1);
+    dsub.alloc(hsub.size());
+    dord.alloc(hord.size());
+    sb.alloc(nsub + 1);
+    sl.alloc(nsub + 1);
+    vals.alloc(hsub.size());
+    keys.alloc(hsub.size());
+    dres.alloc(nsub);
+    rc.alloc(hsub.size());
+    mc.alloc(hsub.size());
+    dout.alloc(nsub);
+    HIPCHK(hipMemcpyAsync(dp.p, hp.data(), hp.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(doff.p, hoff.data(), (nsub + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dsub.p, hsub.data(), hsub.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dord.p, hord.data(), hord.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dres.p, hres.data(), nsub * sizeof(hbtx_sub_result), hipMemcpyHostToDevice, st));
+    ProfArgs A;
+    A.off = doff.p;
+    A.ford = dord.p;
+    A.posm = dp.p;
+    A.res = dres.p;
+    A.seg_len = sl.p;
+    A.rc = rc.p;
+    A.mc = mc.p;
+    run_profiles(c, *prm, A, n, (int32_t)nsub, dsub.p, sb.p, sl.p, keys.p, vals.p, dout.p);
+    HIPCHK(hipMemcpyAsync(out, dout.p, nsub * sizeof(hbtx_sub_profile), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
   });
 }

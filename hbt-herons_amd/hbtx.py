@@ -146,6 +146,7 @@ def lib():
         L.hbtx_direct_potential.argtypes = [P, C.POINTER(Params), i64, P, P, i64, P, P, P]
         L.hbtx_segmented_sort.argtypes = [P, i64, P, P, i64, P]
         L.hbtx_batch_profiles.argtypes = [P, C.POINTER(Params), P]
+        L.hbtx_profiles.argtypes = [P, C.POINTER(Params), i64, P, P, i64, P, P, P, P]
         L.hbtx_mlp_create.argtypes = [P, C.c_uint64, C.POINTER(C.c_void_p)]
         L.hbtx_mlp_free.argtypes = [P]
         L.hbtx_mlp_free.restype = None
@@ -309,6 +310,18 @@ class Context:
         prm = params if isinstance(params, Params) else make_params(**(params or {}))
         out = np.zeros(self._nsub, PROFILE_DTYPE)
         self._check(lib().hbtx_batch_profiles(self._h, C.byref(prm), _p(out)))
+        return out
+
+    def profiles_of(self, pos, mass, offsets, nbound, mbound, params=None):
+        """Profiles and shapes of caller-given bound sets (final order, first = centre)."""
+        prm = params if isinstance(params, Params) else make_params(**(params or {}))
+        pos, mass = _f32(pos), _f32(mass)
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        nb = np.ascontiguousarray(nbound, dtype=np.int64)
+        mb = _f32(mbound)
+        out = np.zeros(len(off) - 1, PROFILE_DTYPE)
+        self._check(lib().hbtx_profiles(self._h, C.byref(prm), len(mass), _p(pos), _p(mass), len(off) - 1, _p(off),
+                                        _p(nb), _p(mb), _p(out)))
         return out
 
     def segmented_sort(self, keys, vals, offsets):

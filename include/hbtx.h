@@ -151,6 +151,11 @@ typedef struct
   int32_t reserved;
 } hbtx_sub_profile;
 int hbtx_batch_profiles(hbtx_ctx *ctx, const hbtx_params *prm, hbtx_sub_profile *out);
+/* The same for bound sets given by the caller: subhalo s owns particles
+ * [offsets[s], offsets[s+1]) of pos float[n][3] / mass float[n] in final order,
+ * the first nbound[s] of them bound (the first is the centre), Mbound mbound[s]. */
+int hbtx_profiles(hbtx_ctx *ctx, const hbtx_params *prm, int64_t n, const float *pos, const float *mass, int64_t nsub,
+                  const int64_t *offsets, const int64_t *nbound, const float *mbound, hbtx_sub_profile *out);
 
 /* Kernel-level counters of the last run (work counts need HBTX_COUNT): stats[0..15], see hbtx_stat_name;
  * 12 walk ms, 13 rounds, 14 rounds replayed from graphs, 15 graph captures. */

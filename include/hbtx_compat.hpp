@@ -369,16 +369,36 @@ void Unbind(Context &ctx, const hbtx_params &prm, SubhaloT &sub, int snapshot_id
 }
 
 // Subhalo_t::CalculateProfileProperties (src/subhalo.cpp:175-243) and, with
-// `shape`, Subhalo_t::CalculateShape (:245-309) for the subhaloes of the last
-// RefineParticles call on this context (the same list, in the same order),
-// computed on the device from that run's final order.  prm gives G, the scale
-// factor, H(z) and the softening of the epoch.
+// `shape`, Subhalo_t::CalculateShape (:245-309) for every subhalo of `subs`,
+// from its bound particles Particles[0, Nbound) (after Unbind: final order, the
+// most bound first), in one launch sequence.  prm gives G, the scale factor, H(z)
+// and the softening of the epoch.
 template <class SubhaloT>
 void CalculateProfileProperties(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT *> &subs, int snapshot_id,
                                 bool shape = true)
 {
-  std::vector<hbtx_sub_profile> pr(subs.size());
-  ctx.check(hbtx_batch_profiles(ctx.get(), &prm, pr.data()));
+  const int64_t nsub = (int64_t)subs.size();
+  if (nsub == 0)
+    return;
+  std::vector<int64_t> off(nsub + 1, 0), nb(nsub);
+  std::vector<float> mb(nsub);
+  for (int64_t s = 0; s < nsub; s++)
+  {
+    nb[s] = std::min<int64_t>((int64_t)subs[s]->Nbound, (int64_t)subs[s]->Particles.size());
+    mb[s] = (float)subs[s]->Mbound;
+    off[s + 1] = off[s] + nb[s];
+  }
+  std::vector<float> pos(3 * off[nsub]), mass(off[nsub]);
+  for (int64_t s = 0; s < nsub; s++)
+    for (int64_t i = 0; i < nb[s]; i++)
+    {
+      const auto &p = subs[s]->Particles[i];
+      detail::put3(&pos[3 * (off[s] + i)], p.ComovingPosition);
+      mass[off[s] + i] = (float)p.Mass;
+    }
+  std::vector<hbtx_sub_profile> pr(nsub);
+  ctx.check(hbtx_profiles(ctx.get(), &prm, off[nsub], pos.data(), mass.data(), nsub, off.data(), nb.data(), mb.data(),
+                          pr.data()));
   for (size_t s = 0; s < subs.size(); s++)
   {
     SubhaloT &S = *subs[s];

@@ -60,6 +60,22 @@ def test_profiles_match_oracle(ctx, kind):
                                                               out2["result"]["nbound"], 0))
 
 
+def test_caller_given_bound_sets(ctx):
+    """hbtx_profiles on bound sets handed over by the caller equals the batch path."""
+    pop = synth.population(seed=62, nsub=90, nmax=20000)
+    prm = hbtx.workload_params(hz=70.0)
+    out = ctx.unbind(pop, prm)
+    g = ctx.profiles(prm)
+    res = out["result"]
+    nb = np.where(res["disrupted"] == 0, res["nbound"], 0)
+    idx = np.concatenate([pop.offsets[s] + out["order"][pop.offsets[s]:pop.offsets[s] + nb[s]]
+                          for s in range(pop.nsub)])
+    off = np.concatenate([[0], np.cumsum(nb)])
+    h = ctx.profiles_of(pop.pos[idx], pop.mass[idx], off, nb, res["mbound"], prm)
+    for f in ("rmax", "vmax", "r_half", "r_enclose", "r200", "m200", "has_r200", "inertia", "inertia_w"):
+        np.testing.assert_array_equal(h[f], g[f])
+
+
 def test_profiles_need_a_run():
     c = hbtx.Context(0)
     with pytest.raises(hbtx.HbtxError):
