@@ -10,9 +10,13 @@
 
 namespace hbtx
 {
+#ifndef SORT_THREADS
 #define SORT_THREADS 256
+#endif
 #define SORT_SMALL_MAX 2048
+#ifndef SORT_TILE
 #define SORT_TILE 4096
+#endif
 #define RADIX_MAX_BINS 512 // (9-bit digits)
 
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_small(uint64_t *__restrict__ keys, int32_t *__restrict__ vals,
