@@ -94,49 +94,54 @@ HBTX_DEV int2 const_load(const int2 *p, int i)
   const ci *q = (const ci *)(uintptr_t)(p + i);
   return make_int2(q[0], q[1]);
 }
-// walk view of an entry: position+mass, opening radius^2, skip
-struct WEntry
-{
-  float x, y, z, m, crit2;
-  int32_t skip;
-};
-// entry at byte offset off from a tree's first entry: one s_load_dwordx8 with the
-// offset in an SGPR
-struct REntry
+// Walk records: the walk's own copy of every tree, in the entries' depth-first
+// order but packed into 8-byte units.  A particle takes two units {x, y, z, m};
+// a cell three {x, y, z, m with the sign bit set, crit2, rskip}, rskip in bytes
+// from the tree's first record.  Source j's group (its cells, then the particle)
+// starts at unit 2 j + 3 cscan[j].  A particle's skip is the next record and it
+// is never opened.  Packing the particles into 16 bytes shrinks a tree's walk
+// footprint by almost half against the 32-byte entries.
+struct WRec
 {
   float x, y, z, m, crit2;
   int32_t rskip;
+  bool cell;
 };
-HBTX_DEV REntry const_load_rentry(const Entry *tree, uint32_t off)
+// Two scalar loads at a wave-uniform byte offset (8-byte aligned): the record's
+// first 16 bytes, and the 8 bytes after them (a cell's crit2 and rskip).
+HBTX_DEV WRec const_load_wrec(const uint2 *tree, uint32_t off)
 {
-  typedef int v8i __attribute__((ext_vector_type(8)));
+  typedef int v4i __attribute__((ext_vector_type(4)));
+  typedef int v2i __attribute__((ext_vector_type(2)));
   typedef const __attribute__((address_space(4))) char cc;
-  typedef const __attribute__((address_space(4))) v8i cv;
-  const v8i raw = *(const cv *)((const cc *)(uintptr_t)tree + off); // all fields in one load
-  REntry e;
-  e.x = __int_as_float(raw[0]);
-  e.y = __int_as_float(raw[1]);
-  e.z = __int_as_float(raw[2]);
-  e.m = __int_as_float(raw[3]);
-  e.crit2 = __int_as_float(raw[4]);
-  e.rskip = raw[5];
+  typedef const __attribute__((address_space(4))) v4i c4;
+  typedef const __attribute__((address_space(4))) v2i c2;
+  const cc *base = (const cc *)(uintptr_t)tree + off;
+  const v4i a = *(const c4 *)base;
+  const v2i b = *(const c2 *)(base + 16);
+  WRec e;
+  e.x = __int_as_float(a[0]);
+  e.y = __int_as_float(a[1]);
+  e.z = __int_as_float(a[2]);
+  e.cell = a[3] < 0; // sign bit of m
+  e.m = __int_as_float(a[3] & 0x7fffffff);
+  e.crit2 = e.cell ? __int_as_float(b[0]) : 0.f;
+  e.rskip = e.cell ? b[1] : (int32_t)off + 16;
   return e;
 }
-HBTX_DEV WEntry const_load_entry(const Entry *p, int i)
+// a particle record (x, y, z, m) at an 8-byte aligned unit, through the scalar path
+HBTX_DEV float4 const_load_f4u(const uint2 *p)
 {
-  typedef const __attribute__((address_space(4))) float cf;
-  typedef const __attribute__((address_space(4))) int ci;
-  const cf *q = (const cf *)(uintptr_t)(p + i);
-  const ci *qi = (const ci *)(uintptr_t)(p + i);
-  WEntry e;
-  e.x = q[0];
-  e.y = q[1];
-  e.z = q[2];
-  e.m = q[3];
-  e.crit2 = q[4];
-  e.skip = qi[5];
-  return e;
+  typedef int v4i __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(4))) v4i c4;
+  const v4i a = *(const c4 *)(uintptr_t)p;
+  return make_float4(__int_as_float(a[0]), __int_as_float(a[1]), __int_as_float(a[2]), __int_as_float(a[3]));
 }
+HBTX_HD int32_t wrec_unit(int32_t j, int32_t cscan_j)
+{
+  return 2 * j + 3 * cscan_j;
+}
+
 
 HBTX_DEV uint64_t ballot(bool p)
 {

@@ -16,6 +16,8 @@
 #include "profile.h"
 
 #define WALK_MAX_CHUNKS 16
+// entries whose walk records (at most 3 units each) keep unit offsets in int32
+#define WREC_MAX_ENTRIES ((int64_t)0x7ffffff0 / 3 - 8)
 
 #include <algorithm>
 #include <atomic>
@@ -136,6 +138,8 @@ struct Batch
   DArr<uint64_t> keys;
   DArr<Entry> ent; // ent_cap + 1 entries (the walk may prefetch one past the end)
   DArr<int8_t> ent_depth;
+  DArr<uint2> wt;        // walk records: 3 ent_cap + 4 units of 8 bytes
+  DArr<int32_t> cell_u;  // per entry index: walk-record unit of a cell
   int32_t ent_cap = 0;
   int64_t ent_bound = 0; // entries of the largest trees the batch's subhaloes can make
   DArr<int2> items;
@@ -159,6 +163,7 @@ struct Batch
   DArr<uint32_t> bbox;
   DArr<int32_t> misc; // [0] active, [1] (unused), [2] overflow, [3] entry total, [4] max depth
   DArr<int32_t> wctr; // walk work counters, 16 ints apart
+  DArr<char2> lev;                // per dense source: the level pair of its tree
   DArr<int32_t> cells, depth_cnt; // tree cells bucketed by depth; counts, offsets, cursors
   DArr<hbtx_sub_result> dres;
   DArr<float> prof_r, prof_m; // profiles: sorted radii and cumulative masses
@@ -192,6 +197,8 @@ struct hbtx_tree
   DevParams P;
   DArr<Entry> ent;
   DArr<int8_t> ent_depth;
+  DArr<uint2> wt;   // walk records of the tree
+  int32_t nw = 0;   // their units
   DArr<int32_t> src_of_entry;
   DArr<double> cube;
   int32_t treemode = 1;
@@ -529,9 +536,12 @@ void batch_reserve(hbtx_ctx *c, Batch &B)
   B.flag64.alloc(n + 1);
   B.fscan64.alloc(n + 1);
   B.keys.alloc(n);
-  B.ent_cap = (int32_t)std::min<int64_t>(3 * n + 64, (int64_t)0x7ffffff0);
+  B.lev.alloc(n + 1);
+  B.ent_cap = (int32_t)std::min<int64_t>(3 * n + 64, WREC_MAX_ENTRIES);
   B.ent.alloc((int64_t)B.ent_cap + 1);
   B.ent_depth.alloc(((int64_t)B.ent_cap + 16) & ~(int64_t)15);
+  B.wt.alloc(3 * (int64_t)B.ent_cap + 4);
+  B.cell_u.alloc((int64_t)B.ent_cap + 1);
   B.items.alloc(n / HBTX_WAVE + ns + 1);
   B.icnt.alloc(ns + 1);
   B.iscan.alloc(ns + 1);
@@ -662,6 +672,9 @@ TreeArgs tree_args(Batch &B, View &V, const DevParams &P)
   T.s_total = B.tpre32.p + (V.n / SLOT_TILE + V.nsub + 1); // grand total of the source tile scan
   T.ent = B.ent.p;
   T.ent_depth = B.ent_depth.p;
+  T.wt = B.wt.p;
+  T.cell_u = B.cell_u.p;
+  T.lev = B.lev.p;
   T.cap = B.ent_cap;
   T.overflow = B.misc.p + 2;
   T.P = P;
@@ -678,12 +691,16 @@ int32_t *round_ptr(Batch &B)
 // entry buffers for at least `need` entries (slack: a quarter more, for the next rounds)
 void grow_entries(Batch &B, int64_t need, bool slack = true)
 {
-  const int64_t cap = std::min<int64_t>(slack ? need + need / 4 + 64 : need, (int64_t)0x7ffffff0);
-  REQUIRE(cap >= need, HBTX_E_CAPACITY, "tree entry count overflows int32");
+  const int64_t cap = std::min<int64_t>(slack ? need + need / 4 + 64 : need, WREC_MAX_ENTRIES);
+  REQUIRE(cap >= need, HBTX_E_CAPACITY, "tree entry count exceeds the walk-record range");
   B.ent.release();
   B.ent_depth.release();
   B.ent.alloc(cap + 1);
   B.ent_depth.alloc((cap + 16) & ~(int64_t)15);
+  B.wt.release();
+  B.cell_u.release();
+  B.wt.alloc(3 * cap + 4);
+  B.cell_u.alloc(cap + 1);
   B.ent_cap = (int32_t)cap;
 }
 
@@ -804,9 +821,10 @@ void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, cons
   k_fill_i32<<<1, 64, 0, st>>>(dcnt, DEPTH_BINS, 0);
   k_depth_count<<<eg, 256, 0, st>>>(B.ent_depth.p, B.misc.p + 3, dcnt);
   k_depth_offsets<<<1, 64, 0, st>>>(dcnt, doff, dcur);
-  k_depth_scatter<<<eg, 256, 0, st>>>(B.ent_depth.p, B.misc.p + 3, doff, dcur, B.cells.p);
+  k_depth_scatter<<<blocks_for(B.ent_cap, 256 * DEPTH_ITEMS, 8192), 256, 0, st>>>(B.ent_depth.p, B.misc.p + 3, doff,
+                                                                                   dcur, B.cells.p);
   for (int d = maxdepth; d >= 0; d--)
-    k_tree_moments<<<blocks_for(n / 4 + 1, 256, 8192), 256, 0, st>>>(B.ent.p, B.cells.p, doff, B.misc.p + 3, d);
+    k_tree_moments<<<blocks_for(n / 4 + 1, 256, 8192), 256, 0, st>>>(T, B.cells.p, doff, B.misc.p + 3, d);
   HIPCHK(hipGetLastError());
 }
 
@@ -824,7 +842,7 @@ WalkArgs walk_args(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   W.treemode = V.S.treemode;
   W.ent_begin = V.S.ent_begin;
   W.ent_end = V.S.ent_end;
-  W.ent = B.ent.p;
+  W.wt = B.wt.p;
   W.posm = V.Q.posm;
   W.velu = V.Q.velu;
   W.selfm = V.Q.selfm;
@@ -1231,7 +1249,7 @@ int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, 
   if (graphs)
   {
     // replayed rounds need entry buffers for the largest trees the view can make
-    if (B.ent_bound > 0x7ffffff0)
+    if (B.ent_bound > WREC_MAX_ENTRIES)
       graphs = false;
     else if (B.ent_cap < B.ent_bound)
       grow_entries(B, B.ent_bound, false);
@@ -1314,6 +1332,8 @@ void upload_batch(hbtx_ctx *c, Batch &B, int64_t n, const float *pos, const floa
   std::vector<int32_t> hsub(n);
   for (int64_t i = 0; i < n; i++)
   {
+    // (the walk records mark cells by the sign bit of the mass)
+    REQUIRE(mass[i] >= 0.f, HBTX_E_INVALID, "particle masses must be non-negative");
     hp[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], mass[i]);
     hv[i] = make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], u ? u[i] : 0.f);
   }
@@ -1480,6 +1500,10 @@ void build_tree(hbtx_ctx *c, const hbtx_params &prm, int64_t n, const float *pos
   T->ent_depth.alloc(ne);
   T->src_of_entry.alloc(n);
   T->cube.alloc(4);
+  // walk records: 2 units per particle, 3 per cell, and two spare units
+  T->nw = (int32_t)(2 * n + 3 * ((int64_t)ne - n));
+  T->wt.alloc((int64_t)T->nw + 2);
+  HIPCHK(hipMemcpyAsync(T->wt.p, B.wt.p, ((int64_t)T->nw + 2) * sizeof(uint2), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(T->ent.p, B.ent.p, ((int64_t)ne + 1) * sizeof(Entry), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(T->ent_depth.p, B.ent_depth.p, ne, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(T->src_of_entry.p, B.src.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
@@ -1524,9 +1548,9 @@ void tree_points(hbtx_ctx *c, hbtx_tree *T, int64_t nt, const float *tpos, const
   A.tgt_order = vals.p;
   A.ntgt = (int32_t)nt;
   A.eb = 0;
-  A.ee = T->ne;
+  A.ee = T->nw;
   A.treemode = T->treemode;
-  A.ent = T->ent.p;
+  A.wt = T->wt.p;
   A.phi = phi.p;
   A.phi_d = phid.p;
   A.counter = counter.p;

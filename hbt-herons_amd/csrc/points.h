@@ -11,9 +11,9 @@ struct PointArgs
   const float4 *tgt;       // target position + self mass (targetMass)
   const int32_t *tgt_order;// walk order of targets (spatially sorted) or null
   int32_t ntgt;
-  int32_t eb, ee;          // entry range
+  int32_t eb, ee;          // walk-record unit range
   int32_t treemode;
-  const Entry *ent;
+  const uint2 *wt;         // walk records
   float *phi;              // specific potential, scattered to the target's input index
   double *phi_d;           // optional double output
   int32_t *counter;
@@ -47,12 +47,12 @@ __global__ __launch_bounds__(64) void k_points_potential(PointArgs A)
     if (A.ee > A.eb)
     {
       if (A.treemode)
-        tree_walk<PERIODIC, STATS>(A.ent, A.eb, A.ee, valid, x.x, x.y, x.z, A.P, acc, wc);
+        tree_walk<PERIODIC, STATS>(A.wt, A.eb, A.ee, valid, x.x, x.y, x.z, A.P, acc, wc);
       else
       {
-        direct_sum<PERIODIC>(A.ent, A.eb, A.ee, x.x, x.y, x.z, A.P, acc, wc);
+        direct_sum<PERIODIC>(A.wt, A.eb, A.ee, x.x, x.y, x.z, A.P, acc, wc);
         if (STATS)
-          st_pairs += (unsigned long long)__popcll(wave_mask(valid)) * (unsigned long long)(A.ee - A.eb);
+          st_pairs += (unsigned long long)__popcll(wave_mask(valid)) * (unsigned long long)((A.ee - A.eb) / 2);
       }
     }
     if (STATS)

@@ -15,7 +15,7 @@ namespace hbtx
 #endif
 #define SORT_SMALL_MAX 2048
 #ifndef SORT_TILE
-#define SORT_TILE 4096
+#define SORT_TILE 2048
 #endif
 #define RADIX_MAX_BINS 512 // (9-bit digits)
 

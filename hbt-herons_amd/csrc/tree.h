@@ -37,12 +37,15 @@ struct TreeArgs
   const float4 *posm;        // particle position + mass
   const int8_t *ptype;       // particle type or null
   const float *upscale;      // per subhalo [6] mass upscale factors
+  char2 *lev;                // per dense source: (c_{j-1}, c_j) of its tree, (0, 0) in direct mode
   const int32_t *cnt;        // cells starting at each dense source
   const int32_t *cscan;      // exclusive scan of cnt (size >= S+1)
   const int32_t *s_total;    // number of dense sources
   Entry *ent;
   int8_t *ent_depth;
-  int32_t cap;               // entry capacity
+  uint2 *wt;                 // walk records (units of 8 bytes)
+  int32_t *cell_u;           // per entry index: unit of a cell's walk record
+  int32_t cap;               // entry capacity (walk records: 3 cap units)
   int32_t *overflow;         // set to 1 if entries would exceed cap
   DevParams P;
 };
@@ -79,6 +82,7 @@ __global__ void k_tree_count(TreeArgs A, int32_t *__restrict__ cnt, int32_t *max
   {
     int c = 0;
     const int s = A.src_sub[j];
+    char2 lv = make_char2(0, 0);
     if (A.treemode[s])
     {
       const int b = A.src_off[s], e = b + A.nsrc[s];
@@ -87,7 +91,9 @@ __global__ void k_tree_count(TreeArgs A, int32_t *__restrict__ cnt, int32_t *max
       c = max(0, cj - cprev);
       if (c > 0)
         depth = max(depth, cj);
+      lv = make_char2((signed char)cprev, (signed char)cj);
     }
+    A.lev[j] = lv;
     cnt[j] = c;
   }
   // deepest cell level: wave maximum, and an atomic only from waves that raise the
@@ -108,19 +114,22 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
   const float m = source_mass(A, p, s);
   const int b = A.src_off[s], e = b + A.nsrc[s];
   const int t0 = b + A.cscan[b]; // first entry of this subhalo's tree
+  const int u0 = wrec_unit(b, A.cscan[b]), uj = wrec_unit(j, A.cscan[j]);
   if (!A.treemode[s])
   {
     if (P < A.cap)
     {
       A.ent[P] = Entry{x.x, x.y, x.z, m, 0.f, (P + 1 - t0) * (int)sizeof(Entry), 0.f, P + 1};
       A.ent_depth[P] = -1;
+      A.wt[uj] = make_uint2(__float_as_uint(x.x), __float_as_uint(x.y));
+      A.wt[uj + 1] = make_uint2(__float_as_uint(x.z), __float_as_uint(m));
     }
     else
       *A.overflow = 1;
     return;
   }
-  int cprev, cj;
-  pair_levels(A, j, b, e, cprev, cj);
+  const char2 lv = A.lev[j];
+  const int cprev = lv.x, cj = lv.y;
   const int nn = max(0, cj - cprev);
   const int q = P + nn;
   if (q >= A.cap)
@@ -130,6 +139,9 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
   }
   A.ent[q] = Entry{x.x, x.y, x.z, m, 0.f, (q + 1 - t0) * (int)sizeof(Entry), 0.f, q + 1};
   A.ent_depth[q] = -1;
+  const int up = uj + 3 * nn;
+  A.wt[up] = make_uint2(__float_as_uint(x.x), __float_as_uint(x.y));
+  A.wt[up + 1] = make_uint2(__float_as_uint(x.z), __float_as_uint(m));
   if (nn == 0)
     return;
   // len of the first depth, then the reference halving rule (src/gravity_tree.cpp:38-41)
@@ -164,6 +176,11 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
     const float crit2 = (A.P.theta2 > 0.f) ? len2 / A.P.theta2 : __int_as_float(0x7f800000);
     A.ent[pos] = Entry{0.f, 0.f, 0.f, 0.f, crit2, (skip - t0) * (int)sizeof(Entry), len2, skip};
     A.ent_depth[pos] = (int8_t)l;
+    // walk record: position and mass come with the moments
+    const int uc = uj + 3 * (l - cprev - 1);
+    A.cell_u[pos] = uc;
+    A.wt[uc + 2] = make_uint2(__float_as_uint(crit2),
+                              (uint32_t)(8 * (wrec_unit(bend + 1, A.cscan[bend + 1]) - u0)));
     L = (L >= A.P.res) ? L * 0.5 : L;
   }
 }
@@ -177,9 +194,11 @@ __global__ void k_tree_emit(TreeArgs A)
 
 // Cells bucketed by depth, so that each depth's moment pass touches only its own
 // cells.  Pass 1 counts cells per depth (block histograms in LDS); after the
-// offsets, pass 2 writes every cell's index into its depth's bucket (a block
-// reserves one range per depth, threads take places through LDS counters).
+// offsets, pass 2 writes every cell's index into its depth's bucket: a wave
+// takes one LDS place per distinct depth among its lanes (ballot match), the
+// block reserves one range per depth, and each lane writes at its rank.
 #define DEPTH_BINS 32
+#define DEPTH_ITEMS 4
 __global__ __launch_bounds__(256) void k_depth_count(const int8_t *__restrict__ ent_depth,
                                                     const int32_t *__restrict__ e_total_p, int32_t *__restrict__ cnt)
 {
@@ -222,36 +241,58 @@ __global__ __launch_bounds__(256) void k_depth_scatter(const int8_t *__restrict_
 {
   __shared__ int32_t h[DEPTH_BINS], base[DEPTH_BINS];
   const int E = *e_total_p;
-  const int stride = gridDim.x * blockDim.x;
+  const int lane = lane_id();
+  const int per_block = DEPTH_ITEMS * 256;
   // every block runs the same number of iterations (the barriers are uniform)
-  for (int q0 = blockIdx.x * blockDim.x; q0 < E; q0 += stride)
+  for (int q0 = blockIdx.x * per_block; q0 < E; q0 += gridDim.x * per_block)
   {
     if (threadIdx.x < DEPTH_BINS)
       h[threadIdx.x] = 0;
     __syncthreads();
-    const int q = q0 + threadIdx.x;
-    const int d = (q < E) ? ent_depth[q] : -1;
-    int slot = 0;
-    if (d >= 0)
-      slot = atomicAdd(&h[d], 1);
+    int d[DEPTH_ITEMS], slot[DEPTH_ITEMS];
+#pragma unroll
+    for (int r = 0; r < DEPTH_ITEMS; r++)
+    {
+      const int q = q0 + r * 256 + (int)threadIdx.x;
+      d[r] = (q < E) ? ent_depth[q] : -1;
+      slot[r] = 0;
+      uint64_t todo = ballot(d[r] >= 0);
+      while (todo)
+      {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int d0 = __shfl(d[r], leader, HBTX_WAVE);
+        const uint64_t grp = ballot(d[r] == d0);
+        int b = 0;
+        if (lane == leader)
+          b = atomicAdd(&h[d0], __popcll(grp));
+        b = __shfl(b, leader, HBTX_WAVE);
+        if (d[r] == d0)
+          slot[r] = b + __popcll(grp & lanemask_lt());
+        todo &= ~grp;
+      }
+    }
     __syncthreads();
     if (threadIdx.x < DEPTH_BINS)
       base[threadIdx.x] = h[threadIdx.x] ? atomicAdd(cursor + threadIdx.x, h[threadIdx.x]) : 0;
     __syncthreads();
-    if (d >= 0)
-      cells[off[d] + base[d] + slot] = q;
+#pragma unroll
+    for (int r = 0; r < DEPTH_ITEMS; r++)
+      if (d[r] >= 0)
+        cells[off[d[r]] + base[d[r]] + slot[r]] = q0 + r * 256 + (int)threadIdx.x;
     __syncthreads();
   }
 }
 
 // Mass and centre of mass of every cell at one depth from its children, in
 // child (sub-index) order with double accumulation of the float child values,
-// as UpdateInternalNodes/ProcessNode do (src/gravity_tree.cpp:25-80).
-__global__ void k_tree_moments(Entry *__restrict__ ent, const int32_t *__restrict__ cells,
-                               const int32_t *__restrict__ off, const int32_t *__restrict__ e_total_p, int depth)
+// as UpdateInternalNodes/ProcessNode do (src/gravity_tree.cpp:25-80).  Also
+// fills the cell's walk record.
+__global__ void k_tree_moments(TreeArgs A, const int32_t *__restrict__ cells, const int32_t *__restrict__ off,
+                               const int32_t *__restrict__ e_total_p, int depth)
 {
   const int E = *e_total_p;
   const int b = off[depth], e = off[depth + 1];
+  Entry *__restrict__ ent = A.ent;
   for (int k = b + blockIdx.x * blockDim.x + threadIdx.x; k < e; k += gridDim.x * blockDim.x)
   {
     const int q = cells[k];
@@ -268,10 +309,14 @@ __global__ void k_tree_moments(Entry *__restrict__ ent, const int32_t *__restric
       cz += (double)en.z * cm;
       c = max(en.skip, c + 1);
     }
-    ent[q].x = (float)(cx / m);
-    ent[q].y = (float)(cy / m);
-    ent[q].z = (float)(cz / m);
-    ent[q].m = (float)m;
+    const float fx = (float)(cx / m), fy = (float)(cy / m), fz = (float)(cz / m), fm = (float)m;
+    ent[q].x = fx;
+    ent[q].y = fy;
+    ent[q].z = fz;
+    ent[q].m = fm;
+    const int u = A.cell_u[q];
+    A.wt[u] = make_uint2(__float_as_uint(fx), __float_as_uint(fy));
+    A.wt[u + 1] = make_uint2(__float_as_uint(fz), __float_as_uint(fm) | 0x80000000u); // sign bit: a cell
   }
 }
 

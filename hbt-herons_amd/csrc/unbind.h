@@ -447,8 +447,10 @@ __global__ void k_sub_entries(SubArrays S, const int32_t *cscan, int32_t cap)
     return;
   const int b = S.src_off[s], e = b + S.nsrc[s];
   // clamped so that an overflowing tree (reported to the host) is never read past the buffer
-  S.ent_begin[s] = min(b + cscan[b], cap);
-  S.ent_end[s] = min(e + cscan[e], cap);
+  // walk-record units, clamped to the record buffer (3 cap units): an
+  // overflowing round, reported to the host, never reads past the buffer
+  S.ent_begin[s] = min(wrec_unit(b, cscan[b]), 3 * cap);
+  S.ent_end[s] = min(wrec_unit(e, cscan[e]), 3 * cap);
 }
 
 // Entries of the round's trees: e_total = min(E, cap), overflow raised when E

@@ -1,7 +1,8 @@
 // Potential + binding-energy kernel (one wave = 64 targets of one subhalo).
 //
-// Tree mode: wavefront-cooperative walk of the depth-first entry array.  The
-// wave visits entries in preorder at a wave-uniform index (scalar loads).  Each
+// Tree mode: wavefront-cooperative walk of the depth-first walk records (16-byte
+// particles, 24-byte cells).  The wave visits records in preorder at a
+// wave-uniform byte offset (scalar loads).  Each
 // lane applies the reference's own opening test to its own target
 // (open if len^2 > r^2 theta^2 with r to the cell's centre of mass,
 // src/gravity_tree.cpp:139, stored as r^2 < crit2 = len^2/theta^2) and keeps a
@@ -9,8 +10,7 @@
 // subtree.  The wave descends (index+1) if any active lane opened the entry,
 // otherwise it jumps to the entry's skip.  Every lane therefore sees exactly the
 // entries its own stackless walk would see, in the same order, while lanes on
-// spatially adjacent targets share the traversal.  An entry arrives in one
-// scalar load at a wave-uniform byte offset.
+// spatially adjacent targets share the traversal.
 //
 // Direct mode: the entry list holds particles only and every pair is summed.
 //
@@ -46,9 +46,9 @@ struct WalkArgs
   const int32_t *ntgt;
   const int32_t *mode;
   const int32_t *treemode;
-  const int32_t *ent_begin;
-  const int32_t *ent_end;
-  const Entry *ent;         // entry array, one spare entry past the last used one
+  const int32_t *ent_begin; // per subhalo: first walk-record unit of its tree
+  const int32_t *ent_end;   // per subhalo: end unit
+  const uint2 *wt;          // walk records (8-byte units, two spare units past the end)
   const float4 *posm;
   const float4 *velu;
   const float *selfm;       // target self mass (sources of a full iteration), else 0
@@ -87,27 +87,27 @@ struct WalkCounts
   uint32_t steps, visits, accepts, nnear;
 };
 
-// Walk the entries [eb, ee) for this lane's target, accumulating the pair terms
-// (-m/r outside h, the spline inside): plain FP32 sums over blocks of 16 steps,
-// Kahan-compensated across blocks.  The wave position is a byte offset from the
-// tree's first entry (an SGPR, the scalar load's offset); entries carry their
-// skip in the same units.
+// Walk the records of units [ub, ue) for this lane's target, accumulating the
+// pair terms (-m/r outside h, the spline inside): plain FP32 sums over blocks of
+// 16 steps, Kahan-compensated across blocks.  The wave position is a byte offset
+// from the tree's first record (an SGPR, the scalar loads' offset); cells carry
+// their skip in the same units.
 template <bool PERIODIC, bool STATS>
-HBTX_DEV void tree_walk(const Entry *ent, int eb, int ee, bool valid, float px, float py, float pz,
+HBTX_DEV void tree_walk(const uint2 *wt, int ub, int ue, bool valid, float px, float py, float pz,
                         const DevParams &P, Kahan &acc, WalkCounts &wc)
 {
-  const Entry *tree = ent + eb;
-  const int32_t span = (ee - eb) * (int32_t)sizeof(Entry);
+  const uint2 *tree = wt + ub;
+  const int32_t span = 8 * (ue - ub);
   int32_t off = 0;
   int32_t myskip = valid ? 0 : 0x7fffffff;
-  // (offsets stay below 2^31: a subhalo's tree holds fewer than 2^26 entries)
+  // (offsets stay below 2^31: a subhalo's tree holds fewer than 2^26 records)
   for (;;)
   {
     float blk = 0.f;
 #pragma unroll
     for (int k = 0; k < 16; k++)
     {
-      const REntry e = const_load_rentry(tree, (uint32_t)off);
+      const WRec e = const_load_wrec(tree, (uint32_t)off);
       const bool active = off >= myskip;
       const float r2 = sep2<PERIODIC>(e.x, e.y, e.z, px, py, pz, P);
       const bool open = r2 < e.crit2;
@@ -132,8 +132,9 @@ HBTX_DEV void tree_walk(const Entry *ent, int eb, int ee, bool valid, float px, 
         wc.visits += __popcll(wave_mask(active));
         wc.accepts += __popcll(wave_mask(accept));
       }
-      // entries of a subtree follow it, so rskip > off; max() keeps the walk finite regardless
-      const int32_t next = off + (int32_t)sizeof(Entry);
+      // records of a subtree follow it, so rskip > off; max() keeps the walk finite regardless
+      // (only a cell is ever opened: the next record then follows its 24 bytes)
+      const int32_t next = off + (e.cell ? 24 : 16);
       off = uniform(anyopen ? next : max(e.rskip, next));
       if (off >= span)
       {
@@ -146,9 +147,9 @@ HBTX_DEV void tree_walk(const Entry *ent, int eb, int ee, bool valid, float px, 
 }
 
 template <bool PERIODIC>
-HBTX_DEV float direct_term(const Entry *ent, int i, float px, float py, float pz, const DevParams &P, bool &near)
+HBTX_DEV float direct_term(const uint2 *wt, int i, float px, float py, float pz, const DevParams &P, bool &near)
 {
-  const float4 e = const_load((const float4 *)(ent + i), 0);
+  const float4 e = const_load_f4u(wt + 2 * i);
   const float r2 = sep2<PERIODIC>(e.x, e.y, e.z, px, py, pz, P);
   const float rinv = __builtin_amdgcn_rsqf(r2);
   float term = -e.w * rinv;
@@ -162,11 +163,14 @@ HBTX_DEV float direct_term(const Entry *ent, int i, float px, float py, float pz
   return term;
 }
 
+// Direct mode: the records are particles only (two units each).
 template <bool PERIODIC>
-HBTX_DEV void direct_sum(const Entry *ent, int eb, int ee, float px, float py, float pz, const DevParams &P,
+HBTX_DEV void direct_sum(const uint2 *wt, int ub, int ue, float px, float py, float pz, const DevParams &P,
                          Kahan &acc, WalkCounts &wc)
 {
-  int i = eb;
+  const uint2 *ent = wt + ub;
+  const int ee = (ue - ub) / 2;
+  int i = 0;
   for (; i + 4 <= ee; i += 4)
   {
     bool nr = false;
@@ -246,19 +250,19 @@ __global__ __launch_bounds__(64) void k_walk_energy(WalkArgs A)
     float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
     if (valid)
       x = A.posm[p];
-    const int eb = uniform(A.ent_begin[s]), ee = uniform(A.ent_end[s]);
+    const int eb = uniform(A.ent_begin[s]), ee = uniform(A.ent_end[s]); // walk-record units
     Kahan acc;
     acc.init();
     WalkCounts wc = {0u, 0u, 0u, 0u};
     if (eb < ee)
     {
       if (A.treemode[s])
-        tree_walk<PERIODIC, STATS>(A.ent, eb, ee, valid, x.x, x.y, x.z, P, acc, wc);
+        tree_walk<PERIODIC, STATS>(A.wt, eb, ee, valid, x.x, x.y, x.z, P, acc, wc);
       else
       {
-        direct_sum<PERIODIC>(A.ent, eb, ee, x.x, x.y, x.z, P, acc, wc);
+        direct_sum<PERIODIC>(A.wt, eb, ee, x.x, x.y, x.z, P, acc, wc);
         if (STATS)
-          st_pairs += (unsigned long long)__popcll(wave_mask(valid)) * (unsigned long long)(ee - eb);
+          st_pairs += (unsigned long long)__popcll(wave_mask(valid)) * (unsigned long long)((ee - eb) / 2);
       }
     }
     if (STATS)
