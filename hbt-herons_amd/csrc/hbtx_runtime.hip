@@ -256,7 +256,7 @@ struct hbtx_ctx
   int rounds = 0;
   // round graphs: head (plan, sources, octree, walk items) and tail (partition,
   // decision, frame), replayed around the directly launched walk
-  hipGraphExec_t g_head = nullptr, g_tail = nullptr;
+  hipGraphExec_t g_head[3] = {nullptr, nullptr, nullptr}, g_tail = nullptr; // head per round level
   GraphKey g_key;
   int graph_rounds = 0, graph_captures = 0;
   Batch B;
@@ -306,12 +306,18 @@ void reserve_sort(hbtx_ctx *c, int64_t n, int32_t nseg)
 
 // Sort (keys, vals) within each segment [seg_begin, seg_begin+seg_len); keys use
 // the low key_bits bits.
+// small_only: every segment is known to hold at most SORT_SMALL_MAX elements.
 void seg_sort(hbtx_ctx *c, uint64_t *keys, int32_t *vals, int32_t nseg, const int32_t *seg_begin,
-              const int32_t *seg_len, int64_t n_cap, int key_bits)
+              const int32_t *seg_len, int64_t n_cap, int key_bits, bool small_only = false)
 {
   reserve_sort(c, n_cap, nseg);
   k_sort_small<<<std::max(1, std::min(nseg, 16384)), SORT_THREADS, 0, c->stream>>>(keys, vals, seg_begin, seg_len,
                                                                                     nseg);
+  if (small_only)
+  {
+    HIPCHK(hipGetLastError());
+    return;
+  }
   k_sort_ntiles<<<blocks_for(nseg, 256, 1 << 20), 256, 0, c->stream>>>(seg_len, nseg, c->sort_ntile.p);
   scan32(c, c->sort_ntile.p, c->sort_toff.p, nseg);
   k_sort_tiles<<<blocks_for(nseg, 256, 1 << 20), 256, 0, c->stream>>>(seg_begin, seg_len, c->sort_toff.p, nseg,
@@ -626,6 +632,7 @@ View make_view(Batch &B, int64_t p0, int64_t n, int32_t s0, int32_t nsub, bool w
   S.bnd_tr = B.bnd_tr.p + s0;
   S.first_idx = B.first_idx.p + s0;
   S.active_count = B.misc.p + 0;
+  S.max_active_n = B.misc.p + 6;
   S.warm = B.warm.p + s0;
   S.npro = B.npro.p + s0;
   S.reset = B.reset_.p + s0;
@@ -681,7 +688,8 @@ TreeArgs tree_args(Batch &B, View &V, const DevParams &P)
   return T;
 }
 
-void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, const int32_t *nt, bool graph);
+void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, const int32_t *nt, bool graph,
+                        int level);
 // device round counter (1 in the first round), advanced by k_round_begin
 int32_t *round_ptr(Batch &B)
 {
@@ -741,7 +749,7 @@ const T *tile_scan(hbtx_ctx *c, View &V, const int4 *tiles, const int32_t *nt, c
 }
 
 // Sources of the round -> dense list, then (tree-mode subhaloes) the octree.
-void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool graph)
+void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool graph, int level)
 {
   hipStream_t st = c->stream;
   const int64_t n = V.n;
@@ -763,12 +771,13 @@ void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, 
   // (without subsampling the candidate rank is irrelevant: every candidate is a source)
   k_mark_sources<<<tile_grid(c, V), 256, 0, st>>>(V.S, V.Q, P, B.stiles.p, nt, round_ptr(B),
                                                   P.mss > 0 ? B.crank.p : nullptr, B.flag.p);
-  build_sources_tree(c, B, V, P, nt, graph);
+  build_sources_tree(c, B, V, P, nt, graph, level);
 }
 
 // Flags in B.flag (on the slots of the active tiles) -> dense source list, tree
 // cube, keys, sort, octree entries.
-void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, const int32_t *nt, bool graph)
+void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, const int32_t *nt, bool graph,
+                        int level)
 {
   hipStream_t st = c->stream;
   const int64_t n = V.n;
@@ -785,7 +794,7 @@ void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, cons
   k_cube<<<gs, 256, 0, st>>>(V.S);
   k_keys<<<gb, 256, 0, st>>>(V.S, P, V.Q.posm, B.src.p, B.src_sub.p, s_total, B.keys.p);
   k_sort_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
-  seg_sort(c, B.keys.p, B.src.p, V.nsub, B.seg_begin.p, B.seg_len.p, n, 63);
+  seg_sort(c, B.keys.p, B.src.p, V.nsub, B.seg_begin.p, B.seg_len.p, n, 63, level >= 1);
   k_targets_from_sources<<<gb, 256, 0, st>>>(V.S, V.Q, B.src.p, B.src_sub.p, s_total);
   TreeArgs T = tree_args(B, V, P);
   k_fill_i32<<<1, 64, 0, st>>>(B.misc.p + 4, 1, 0);
@@ -813,6 +822,11 @@ void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, cons
   k_tree_emit<<<blocks_for(n, 256, 8 * 1024), 256, 0, st>>>(T);
   k_sub_entries<<<gs, 256, 0, st>>>(V.S, B.cscan.p, B.ent_cap);
   k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, B.ent_cap, B.misc.p + 2);
+  if (level >= 2)
+  {
+    HIPCHK(hipGetLastError());
+    return; // direct sums only: no cells
+  }
   // moments bottom-up, one pass per depth over that depth's cells
   B.cells.alloc(std::max<int64_t>(B.ent_cap, 1));
   B.depth_cnt.alloc(3 * DEPTH_BINS + 1);
@@ -1087,7 +1101,7 @@ void run_final(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot
     const int32_t *nt = build_slot_tiles(c, B, V);
     k_slot_of<<<gb, 256, 0, st>>>(V.S, n, V.fvals, V.Q.cur2);
     k_mark_refine<<<gb, 256, 0, st>>>(V.S, V.Q, n, V.Q.cur2, B.flag.p);
-    build_sources_tree(c, B, V, P, nt, false);
+    build_sources_tree(c, B, V, P, nt, false, 0);
     c->walk_items_hint = INT64_MAX; // (other targets than the rounds')
     round_walk(c, B, V, P);
     k_refine_keys<<<gb, 256, 0, st>>>(V.S, n, V.fvals, B.eref.p + V.p0, B.keys.p, B.seg_begin.p, B.seg_len.p);
@@ -1147,14 +1161,17 @@ void init_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool warm)
 }
 
 // First part of a round, up to the walk: plan, sources, octrees, work items.
-// graph: no host synchronisation (the part is being captured).
-void round_head(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool graph, bool warm_round)
+// graph: no host synchronisation (the part is being captured).  level (a bound
+// the host knows from the previous round): 0 any subhalo; 1 every active
+// subhalo fits the LDS sort (no radix passes); 2 every active subhalo also takes
+// the direct sum (no tree cells, no moment passes).
+void round_head(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool graph, bool warm_round, int level = 0)
 {
   hipStream_t st = c->stream;
   const int gs = blocks_for(V.nsub, 256, 1 << 20);
-  k_round_begin<<<1, 64, 0, st>>>(V.S.active_count, round_ptr(B));
+  k_round_begin<<<1, 64, 0, st>>>(V.S.active_count, round_ptr(B), V.S.max_active_n);
   k_plan<<<gs, 256, 0, st>>>(V.S, P, round_ptr(B));
-  round_sources_and_tree(c, B, V, P, graph);
+  round_sources_and_tree(c, B, V, P, graph, level);
   if (warm_round)
   {
     // warm round: every target in key order of the proposal's tree cube
@@ -1167,11 +1184,15 @@ void round_head(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool graph, 
 
 void drop_graphs(hbtx_ctx *c)
 {
-  if (c->g_head)
-    (void)hipGraphExecDestroy(c->g_head);
+  for (auto &g : c->g_head)
+  {
+    if (g)
+      (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
   if (c->g_tail)
     (void)hipGraphExecDestroy(c->g_tail);
-  c->g_head = c->g_tail = nullptr;
+  c->g_tail = nullptr;
   c->g_key = GraphKey();
 }
 
@@ -1206,7 +1227,7 @@ hipGraphExec_t capture(hbtx_ctx *c, F &&record)
 // the parameters changed.  Capture only records: every buffer the round touches
 // already has its size from the first (direct) round.  Returns false (rounds
 // then run directly) if the capture could not be made.
-bool ensure_round_graphs(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
+bool ensure_round_graphs(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int level)
 {
   GraphKey k;
   k.p0 = V.p0;
@@ -1218,13 +1239,15 @@ bool ensure_round_graphs(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   k.ent_cap = B.ent_cap;
   k.P = P;
   k.gen = g_alloc_gen.load();
-  if (c->g_head && c->g_tail && k == c->g_key)
+  if (c->g_head[level] && c->g_tail && k == c->g_key)
     return true;
-  drop_graphs(c);
+  if (!(k == c->g_key))
+    drop_graphs(c);
   try
   {
-    c->g_head = capture(c, [&] { round_head(c, B, V, P, true, false); });
-    c->g_tail = capture(c, [&] { round_partition(c, B, V, P, false); });
+    c->g_head[level] = capture(c, [&] { round_head(c, B, V, P, true, false, level); });
+    if (!c->g_tail)
+      c->g_tail = capture(c, [&] { round_partition(c, B, V, P, false); });
   }
   catch (const Error &)
   {
@@ -1256,23 +1279,23 @@ int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, 
   }
   init_view(c, B, V, P, warm);
   c->walk_items_hint = INT64_MAX;
-  int round = 0;
+  int round = 0, level = 0;
   const int max_rounds = 2000;
   while (true)
   {
     REQUIRE(round < max_rounds, HBTX_E_INVALID, "unbinding did not converge within the round limit");
     round++;
-    const bool replay = graphs && round >= 2 && ensure_round_graphs(c, B, V, P);
+    const bool replay = graphs && round >= 2 && ensure_round_graphs(c, B, V, P, level);
     if (replay)
     {
-      HIPCHK(hipGraphLaunch(c->g_head, st));
+      HIPCHK(hipGraphLaunch(c->g_head[level], st));
       round_walk_launch(c, B, V, P);
       HIPCHK(hipGraphLaunch(c->g_tail, st));
       c->graph_rounds++;
     }
     else
     {
-      round_head(c, B, V, P, false, warm && round == 1);
+      round_head(c, B, V, P, false, warm && round == 1, round == 1 ? 0 : level);
       round_walk_launch(c, B, V, P);
       round_partition(c, B, V, P, warm && round == 1);
     }
@@ -1283,6 +1306,9 @@ int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, 
     HIPCHK(hipEventElapsedTime(&wms, c->wev0, c->wev1));
     c->walk_ms += wms;
     c->walk_items_hint = std::max(h_misc[7], 1);
+    // the next round's level: subhaloes only leave the active set
+    const int64_t maxn = h_misc[6];
+    level = (maxn <= P.direct_cutoff) ? 2 : (maxn <= SORT_SMALL_MAX ? 1 : 0);
     REQUIRE(h_misc[2] == 0, HBTX_E_CAPACITY, "tree entry capacity exceeded");
     if (h_misc[0] == 0)
       break;
@@ -1482,9 +1508,9 @@ void build_tree(hbtx_ctx *c, const hbtx_params &prm, int64_t n, const float *pos
   k_init_particles<<<blocks_for(n), 256, 0, st>>>(V.S, V.Q, n);
   k_init_subs<<<1, 64, 0, st>>>(V.S, B.ref_in.p, B.mb_in.p, nullptr);
   k_fill_i32<<<1, 64, 0, st>>>(B.misc.p, 8, 0);
-  k_round_begin<<<1, 64, 0, st>>>(V.S.active_count, round_ptr(B));
+  k_round_begin<<<1, 64, 0, st>>>(V.S.active_count, round_ptr(B), V.S.max_active_n);
   k_plan<<<1, 64, 0, st>>>(V.S, P, round_ptr(B));
-  round_sources_and_tree(c, B, V, P, false);
+  round_sources_and_tree(c, B, V, P, false, 0);
   int32_t h_misc[4];
   HIPCHK(hipMemcpyAsync(h_misc, B.misc.p, sizeof(h_misc), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

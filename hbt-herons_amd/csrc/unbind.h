@@ -31,6 +31,7 @@ struct SubArrays
   int32_t *bnd_cnt, *bnd_tr;
   int32_t *first_idx;       // smallest candidate index (periodic reference particle)
   int32_t *active_count;    // [1]
+  int32_t *max_active_n;    // [1] particles of the largest subhalo still active after the round
   // learned warm start: warm = 0 off, 1 first (proposal-sourced) round pending, 2 done;
   // npro = proposed particles; reset = proposal rejected, restart from all particles
   int32_t *warm, *npro, *reset;
@@ -107,12 +108,13 @@ __global__ void k_init_subs(SubArrays S, const float *ref_in, const float *mb_in
 }
 
 // Start of a round: clear the active count, advance the device round counter.
-__global__ void k_round_begin(int32_t *active, int32_t *round_p)
+__global__ void k_round_begin(int32_t *active, int32_t *round_p, int32_t *max_active_n)
 {
   if (blockIdx.x == 0 && threadIdx.x == 0)
   {
     *active = 0;
     *round_p += 1;
+    *max_active_n = 0;
   }
 }
 
@@ -864,7 +866,10 @@ __global__ void k_round_finish(SubArrays S, DevParams P, const double *red)
   if (!S.reset[s] && nb >= nlast * P.prec)
     S.status[s] = ST_CONVERGED;
   else
+  {
     atomicAdd(S.active_count, 1);
+    atomicMax(S.max_active_n, S.off[s + 1] - S.off[s]);
+  }
 }
 
 __global__ void k_round_finish_periodic(SubArrays S, PartArrays Q, DevParams P, const double *red)
