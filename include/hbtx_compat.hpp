@@ -15,6 +15,9 @@
 //                              one batch per level of the hierarchies
 //   hbtx::TruncateSource(sub, relax_factor)
 //                              Subhalo_t::TruncateSource    src/subhalo_unbind.cpp:842-852
+//   hbtx::SubhaloStatistics(subs, snapshot_id)
+//                              SubhaloSnapshot_t::PrintSubhaloStatistics (the counts)
+//                              src/subhalo_unbind.cpp:940-980
 //   hbtx::CalculateProfileProperties(ctx, prm, subs, snapshot_id, shape)
 //                              Subhalo_t::CalculateProfileProperties / CalculateShape
 //                              src/subhalo.cpp:175-309
@@ -366,6 +369,29 @@ void Unbind(Context &ctx, const hbtx_params &prm, SubhaloT &sub, int snapshot_id
 {
   std::vector<SubhaloT *> one{&sub};
   RefineParticles(ctx, prm, one, snapshot_id, HBTX_PLAIN, save_binding_energies, save_potential_energies);
+}
+
+// SubhaloSnapshot_t::PrintSubhaloStatistics (src/subhalo_unbind.cpp:940-980):
+// the counts it prints for one rank's subhaloes after RefineParticles.
+struct SubhaloCounts
+{
+  int64_t sunk = 0;      // SnapshotOfDeath == SnapshotOfSink == now: merged this snapshot
+  int64_t disrupted = 0; // born earlier, died now, not sunk
+  int64_t fake = 0;      // born and died now: never self-bound
+  int64_t newborn = 0;   // born now and alive
+};
+template <class SubhaloT>
+SubhaloCounts SubhaloStatistics(const std::vector<SubhaloT> &subs, int snapshot_id)
+{
+  SubhaloCounts c;
+  for (const SubhaloT &s : subs)
+  {
+    c.sunk += (s.SnapshotOfDeath == snapshot_id) & (s.SnapshotOfSink == snapshot_id);
+    c.disrupted += (s.SnapshotOfBirth < snapshot_id) & (s.SnapshotOfDeath == snapshot_id) & (s.SnapshotOfSink == -1);
+    c.fake += (s.SnapshotOfBirth == snapshot_id) & (s.SnapshotOfDeath == snapshot_id);
+    c.newborn += (s.SnapshotOfBirth == snapshot_id) & (s.SnapshotOfDeath == -1);
+  }
+  return c;
 }
 
 // Subhalo_t::CalculateProfileProperties (src/subhalo.cpp:175-243) and, with

@@ -43,7 +43,7 @@ struct Subhalo_t
   float SpecificSelfPotentialEnergy = 0, SpecificSelfKineticEnergy = 0, SpecificAngularMomentum[3] = {0, 0, 0};
   HBTxyz ComovingAveragePosition, PhysicalAverageVelocity, ComovingMostBoundPosition, PhysicalMostBoundVelocity;
   int64_t MostBoundParticleId = -1;
-  int SnapshotOfDeath = -1, SnapshotOfSink = -1;
+  int SnapshotOfDeath = -1, SnapshotOfSink = -1, SnapshotOfBirth = 0;
   int64_t SinkTrackId = -1, DescendantTrackId = -1;
   HBTxyz CoreComovingPosition, CorePhysicalVelocity;
   float CoreComovingSigmaR = 0, CorePhysicalSigmaV = 0;
@@ -257,7 +257,19 @@ int main(int argc, char **argv)
       for (auto &p : S.Particles)
         wr(o, &p.Id, 1);
     fclose(o);
-    printf("levels %d merged %d identical %d truncate_ok %d\n", levels, g_merged, ok ? 1 : 0, trunc_ok ? 1 : 0);
+    // PrintSubhaloStatistics' counts; subhalo 0 born this snapshot
+    A[0].SnapshotOfBirth = g_snap;
+    const hbtx::SubhaloCounts sc = hbtx::SubhaloStatistics(A, g_snap);
+    int64_t sunk = 0, dis = 0;
+    for (int64_t s = 0; s < nsub; s++)
+    {
+      sunk += A[s].SnapshotOfSink == g_snap && A[s].SnapshotOfDeath == g_snap;
+      dis += s != 0 && A[s].SnapshotOfDeath == g_snap && A[s].SnapshotOfSink == -1;
+    }
+    const bool stats_ok = sc.sunk == sunk && sc.disrupted == dis &&
+                          sc.newborn + sc.fake == 1 && (sc.fake == 1) == (A[0].SnapshotOfDeath == g_snap);
+    printf("levels %d merged %d identical %d truncate_ok %d stats_ok %d sunk %ld\n", levels, g_merged, ok ? 1 : 0,
+           trunc_ok ? 1 : 0, stats_ok ? 1 : 0, (long)sc.sunk);
     return ok ? 0 : 1;
   }
   catch (const hbtx::Error &e)

@@ -55,7 +55,8 @@ def test_recursive_unbind_matches_oracle_recursion(tmp_path, merge_trapped):
     H = synth.hierarchy(seed=5)
     info, gpu, lists = run_driver(H, tmp_path, merge_trapped)
     # the level-batched result equals the depth-first recursion, and the case is not trivial
-    assert info["identical"] == 1 and info["truncate_ok"] == 1
+    assert info["identical"] == 1 and info["truncate_ok"] == 1 and info["stats_ok"] == 1
+    assert info["sunk"] >= 2
     assert info["levels"] >= 4 and info["merged"] >= 2
     ref = HO.run(H, O.default_params(), 7, merge_trapped)
     # the same mergers (sink track, snapshots) in every subhalo
