@@ -101,6 +101,30 @@ def plain_gpu_estimate(hbtx, pop, prm, share):
                  f"ones (weight {weight_small:.1f})")
 
 
+def other_configs(hbtx, synth, ctx, log):
+    """The other workloads, timed as batched unbinding (plain method, inputs
+    resident): one NFW halo of 10,000 particles + 2,000 interlopers, 20,000
+    power-law subhaloes, and 5,000 hydro-mixture subhaloes with thermal energy
+    (their parity with the oracle is tested in tests/)."""
+    out = {}
+    cases = [
+        ("single_nfw_12k", synth.single_nfw(seed=0), {}),
+        ("population_20000_subhalos", synth.population(seed=2, nsub=20000), {}),
+        ("hydro_mixture_5000_subhalos", synth.population(seed=3, nsub=5000, hydro=True), dict(dm_only=0, thermal=1)),
+    ]
+    for name, pop, kw in cases:
+        prm = hbtx.workload_params(direct_cutoff=-1, **kw)
+        ctx.load(pop)
+        ms = timed_steps(ctx, 3, 1, lambda: ctx.run(prm))
+        res = ctx.fetch()
+        rec = {"particles": int(pop.n), "subhalos": int(pop.nsub), "ms_per_step": float(np.mean(ms)),
+               "particles_per_s": pop.n / (float(np.mean(ms)) / 1e3),
+               "bound_particles": int(res["result"]["nbound"].sum())}
+        out[name] = rec
+        log(f"{name}: {rec['ms_per_step']:.2f} ms, {rec['particles_per_s']:.3e} particles/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +147,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tree", action="store_true", help="skip the host+subhaloes tree-potential line")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the single-halo, 20,000-subhalo and hydro-mixture lines")
     ap.add_argument("--state", default=None,
                     help="npz cache of the trained weights and the previous snapshot's outputs: written "
                          "after training, read instead of labelling and training when it exists")
@@ -132,7 +158,7 @@ def main():
                     help="time only the learned step (for kernel profiles); implies --no-plain --no-cpu")
     args = ap.parse_args()
     if args.learned_only:
-        args.no_plain = args.no_cpu = args.no_tree = True
+        args.no_plain = args.no_cpu = args.no_tree = args.no_configs = True
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; how the path would shard over "
                                    "several is described in DESIGN.md, not built"}))
@@ -331,6 +357,9 @@ def main():
             "particles_per_s": host.n / ((build_ms + walk_ms_c1) / 1e3),
             "note": "GravityTree_t Build + EvaluatePotential at every particle, host<->device transfers included"}
         log(f"host+subs tree: build {build_ms:.1f} ms, potential {walk_ms_c1:.1f} ms for {host.n} particles")
+
+    if not args.no_configs:
+        line["other_configs"] = other_configs(hbtx, synth, ctx, log)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
