@@ -10,6 +10,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <queue>
 #include <vector>
 
 namespace
@@ -1345,4 +1346,111 @@ extern "C" void oracle_profiles(const ounb_params *prm, int64_t n, const float *
 #pragma omp parallel for schedule(dynamic, 1)
   for (int64_t s = 0; s < nsub; s++)
     profile_one(prm, pos + 3 * offsets[s], mass + offsets[s], nbound[s], mbound[s], out + s);
+}
+
+// ---------------------------------------------------------------------------
+// SubhaloSnapshot_t::ReassignParticles (src/subhalo_reassign_gas.cpp:63-197) with
+// GeoTree_t::NearestNeighbours / Search (src/geometric_tree.cpp:108-213) and
+// NumNearestNeighboursCollector_t (src/datatypes.h:287-324).  The tree search is
+// replaced by a scan over the group's tracers: it collects the same set (every
+// tracer with r^2 < radius^2); only the order of collection, and so the order
+// among tracers at exactly equal float distances, can differ.
+// ---------------------------------------------------------------------------
+namespace
+{
+struct Located
+{
+  int64_t index;
+  float d2;
+};
+struct LocatedLess
+{
+  bool operator()(const Located &a, const Located &b) const
+  {
+    return a.d2 < b.d2;
+  }
+};
+} // namespace
+
+extern "C" void oracle_reassign_targets(const ounb_params *prm, int64_t n, const float *pos, const uint8_t *is_tracer,
+                                        const int32_t *sub, int64_t ngroups, const int64_t *group_offsets, int32_t k,
+                                        float rguess, int32_t *out_target)
+{
+  (void)n;
+  const bool periodic = prm->periodic != 0;
+  const double box = prm->box_size, half = prm->box_size / 2.0;
+  auto nearest = [&](double d) { return (d > half) ? d - box : ((d < -half) ? d + box : d); };
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t g = 0; g < ngroups; g++)
+  {
+    const int64_t a = group_offsets[g], e = group_offsets[g + 1];
+    std::vector<int64_t> tracers;
+    for (int64_t i = a; i < e; i++)
+      if (is_tracer[i])
+        tracers.push_back(i);
+    const int64_t nt = (int64_t)tracers.size();
+    for (int64_t i = a; i < e; i++)
+    {
+      out_target[i] = -1;
+      if (is_tracer[i] || nt == 0)
+        continue;
+      const int64_t nr_to_find = std::min<int64_t>(nt, k);
+      std::priority_queue<Located, std::vector<Located>, LocatedLess> heap;
+      const double x0 = pos[3 * i], y0 = pos[3 * i + 1], z0 = pos[3 * i + 2];
+      float radius = rguess;
+      auto search = [&](float r) {
+        const double h2 = r * r; // float product, as in the reference
+        for (int64_t t : tracers)
+        {
+          double dx = pos[3 * t] - x0, dy = pos[3 * t + 1] - y0, dz = pos[3 * t + 2] - z0;
+          if (periodic)
+          {
+            dx = nearest(dx);
+            dy = nearest(dy);
+            dz = nearest(dz);
+          }
+          // per-axis box test before the sphere test (src/geometric_tree.cpp:160-176)
+          if (dx > r || dx < -r || dy > r || dy < -r || dz > r || dz < -r)
+            continue;
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 < h2)
+          {
+            const Located lp{t, (float)r2};
+            if ((int64_t)heap.size() < k)
+              heap.push(lp);
+            else if (lp.d2 < heap.top().d2)
+            {
+              heap.push(lp);
+              heap.pop();
+            }
+          }
+        }
+      };
+      search(radius);
+      while ((int64_t)heap.size() < nr_to_find)
+      {
+        radius = (float)(radius * 1.26);
+        search(radius);
+      }
+      std::vector<int64_t> list;
+      while (!heap.empty())
+      {
+        list.push_back(heap.top().index);
+        heap.pop();
+      }
+      bool found = false;
+      for (int64_t t : list)
+        if (sub[t] == sub[i])
+        {
+          found = true;
+          break;
+        }
+      if (!found)
+      {
+        const int32_t ns = sub[list.back()];
+        if (ns >= 0 && ns != sub[i])
+          out_target[i] = ns;
+      }
+    }
+  }
 }

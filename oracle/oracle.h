@@ -136,6 +136,19 @@ typedef struct
 void oracle_profiles(const ounb_params *prm, int64_t n, const float *pos, const float *mass, int64_t nsub,
                      const int64_t *offsets, const int64_t *nbound, const float *mbound, ounb_profile *out);
 
+/* SubhaloSnapshot_t::ReassignParticles (src/subhalo_reassign_gas.cpp:63-197) for
+ * the particle selection: groups are contiguous ranges [group_offsets[g],
+ * group_offsets[g+1]) of the particle arrays, in the reference's order (the
+ * group's subhaloes in turn, each subhalo's particles).  Every non-tracer whose
+ * k nearest tracers of its group, found by GeoTree_t::NearestNeighbours
+ * (src/geometric_tree.cpp:108-141: all tracers within a radius that starts at
+ * rguess and grows by 1.26 until enough were collected, collected again at every
+ * radius), include none of its own subhalo gets out_target = the subhalo of the
+ * nearest tracer; others get -1.  A group without tracers moves nothing. */
+void oracle_reassign_targets(const ounb_params *prm, int64_t n, const float *pos, const uint8_t *is_tracer,
+                             const int32_t *sub, int64_t ngroups, const int64_t *group_offsets, int32_t k,
+                             float rguess, int32_t *out_target);
+
 #ifdef __cplusplus
 }
 #endif

@@ -118,6 +118,9 @@ def lib():
         L.oracle_tree_dfs.argtypes = [C.POINTER(Params), C.c_int64, P, P, C.c_int64, P, P, P, P, P, P, P]
         L.oracle_core_phase_space.restype = None
         L.oracle_core_phase_space.argtypes = [C.POINTER(Params), C.c_int64, P, P, P, P, P, P, P, P]
+        L.oracle_reassign_targets.restype = None
+        L.oracle_reassign_targets.argtypes = [C.POINTER(Params), C.c_int64, P, P, P, C.c_int64, P, C.c_int32,
+                                              C.c_float, P]
         L.oracle_profiles.restype = None
         L.oracle_profiles.argtypes = [C.POINTER(Params), C.c_int64, P, P, C.c_int64, P, P, P, P]
         _lib = L
@@ -229,4 +232,19 @@ def profiles(pos, mass, offsets, nbound, mbound, params=None):
     mb = _f32(mbound)
     out = np.zeros(len(nb), PROFILE_DTYPE)
     lib().oracle_profiles(C.byref(prm), len(mass), _p(pos), _p(mass), len(nb), _p(off), _p(nb), _p(mb), _p(out))
+    return out
+
+
+def reassign_targets(pos, is_tracer, sub, group_offsets, k=10, rguess=None, params=None):
+    """SubhaloSnapshot_t::ReassignParticles' selection: target subhalo per particle, -1 to stay."""
+    prm = params if isinstance(params, Params) else make_params(**(params or {}))
+    pos = _f32(pos)
+    tr = np.ascontiguousarray(is_tracer, dtype=np.uint8)
+    sb = np.ascontiguousarray(sub, dtype=np.int32)
+    go = np.ascontiguousarray(group_offsets, dtype=np.int64)
+    if rguess is None:
+        rguess = np.float32(prm.softening * 0.1)
+    out = np.zeros(len(tr), np.int32)
+    lib().oracle_reassign_targets(C.byref(prm), len(tr), _p(pos), _p(tr), _p(sb), len(go) - 1, _p(go), int(k),
+                                  float(rguess), _p(out))
     return out

@@ -1,0 +1,118 @@
+"""Pins the oracle's restatement of SubhaloSnapshot_t::ReassignParticles
+(src/subhalo_reassign_gas.cpp:63-197) with GeoTree_t::NearestNeighbours
+(src/geometric_tree.cpp:108-141) on configurations whose outcome follows from
+the rules: stay when an own tracer is among the neighbours, else move to the
+subhalo of the nearest tracer; no tracers, no move; and the reference's search,
+which collects the tracers inside every radius again as the radius grows."""
+import numpy as np
+
+import pyoracle as O
+
+
+def run(pts, tracer, sub, groups=None, **kw):
+    pts = np.asarray(pts, np.float32)
+    n = len(pts)
+    go = [0, n] if groups is None else groups
+    return O.reassign_targets(pts, np.asarray(tracer), np.asarray(sub), go, params=O.default_params(**kw),
+                              rguess=kw.pop("rguess", None) if "rguess" in kw else None)
+
+
+def cloud(rng, centre, n, r):
+    return np.asarray(centre) + rng.normal(0, r, (n, 3))
+
+
+def test_particles_near_own_tracers_stay():
+    rng = np.random.default_rng(1)
+    a_t, b_t = cloud(rng, [1, 1, 1], 50, 0.01), cloud(rng, [2, 1, 1], 50, 0.01)
+    a_g, b_g = cloud(rng, [1, 1, 1], 20, 0.01), cloud(rng, [2, 1, 1], 20, 0.01)
+    pts = np.concatenate([a_t, a_g, b_t, b_g])
+    tracer = np.r_[np.ones(50), np.zeros(20), np.ones(50), np.zeros(20)].astype(np.uint8)
+    sub = np.r_[np.zeros(70), np.ones(70)].astype(np.int32)
+    out = run(pts, tracer, sub)
+    assert np.all(out == -1)
+
+
+def test_particle_inside_other_subhalo_moves_to_nearest_tracer():
+    rng = np.random.default_rng(2)
+    a_t, b_t = cloud(rng, [1, 1, 1], 50, 0.01), cloud(rng, [2, 1, 1], 50, 0.01)
+    c_t = cloud(rng, [2.3, 1, 1], 50, 0.01)
+    stray = np.array([[2.0, 1.0, 1.0], [2.3, 1.0, 1.0]])  # gas of subhalo 0 inside subhaloes 1 and 2
+    pts = np.concatenate([a_t, stray, b_t, c_t])
+    tracer = np.r_[np.ones(50), np.zeros(2), np.ones(100)].astype(np.uint8)
+    sub = np.r_[np.zeros(52), np.ones(50), 2 * np.ones(50)].astype(np.int32)
+    out = run(pts, tracer, sub)
+    assert out[50] == 1 and out[51] == 2
+    assert np.all(np.delete(out, [50, 51]) == -1)
+
+
+def test_no_tracers_no_moves_and_groups_are_separate():
+    pts = np.array([[0, 0, 0], [1, 0, 0], [5, 5, 5], [5.001, 5, 5]], np.float32)
+    tracer = np.array([0, 0, 1, 0], np.uint8)
+    sub = np.array([0, 1, 2, 3], np.int32)
+    out = run(pts, tracer, sub, groups=[0, 2, 4])
+    # group 0 has no tracers; in group 1 the gas particle's only neighbour is subhalo 2's tracer
+    assert list(out) == [-1, -1, -1, 2]
+
+
+def test_radius_growth_collects_again():
+    """The reference collects the tracers inside each radius again as the radius
+    grows (the collector is not emptied), so near tracers fill the list twice.
+    Five tracers of subhalo 1 within the first radius and twenty of the query's
+    own subhalo 0 just beyond it: the true ten nearest hold five own tracers,
+    but the reference's list holds the five near ones twice, and the particle
+    moves to subhalo 1."""
+    eps = 0.001
+    r0 = np.float32(eps * 0.1)
+    q = np.array([[3.0, 3.0, 3.0]])
+    near = q + np.array([[0.3, 0, 0], [-0.3, 0, 0], [0, 0.3, 0], [0, -0.3, 0], [0, 0, 0.3]]) * r0
+    rng = np.random.default_rng(3)
+    d = rng.normal(size=(20, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    own = q + d * (1.1 * r0)  # between the first radius and 1.26 times it
+    pts = np.concatenate([q, near, own])
+    tracer = np.r_[0, np.ones(5), np.ones(20)].astype(np.uint8)
+    sub = np.r_[0, np.ones(5), np.zeros(20)].astype(np.int32)
+    out = run(pts, tracer, sub, softening=eps)
+    assert out[0] == 1
+    # with the own tracers inside the first radius too, the list holds some of them: stay
+    own_close = q + d * (0.6 * r0)
+    pts2 = np.concatenate([q, near, own_close])
+    assert run(pts2, tracer, sub, softening=eps)[0] == -1
+
+
+def test_periodic_wrap():
+    box = 10.0
+    rng = np.random.default_rng(4)
+    b_t = cloud(rng, [0.02, 5, 5], 30, 0.003)  # just above x = 0
+    a_t = cloud(rng, [5, 5, 5], 30, 0.005)
+    gas = np.array([[9.999, 5, 5]])  # subhalo 0's gas, next to subhalo 1 across the boundary
+    pts = np.concatenate([a_t, gas, b_t])
+    tracer = np.r_[np.ones(30), 0, np.ones(30)].astype(np.uint8)
+    sub = np.r_[np.zeros(31), np.ones(30)].astype(np.int32)
+    assert run(pts, tracer, sub, periodic=1, box_size=box)[30] == 1
+    assert run(pts, tracer, sub, periodic=0, box_size=box)[30] == -1
+
+
+def test_single_search_equals_brute_force_knn():
+    """With a first radius that holds every tracer, one search collects each once,
+    and the rule reduces to: move to the nearest tracer's subhalo unless one of
+    the k nearest tracers is the particle's own."""
+    rng = np.random.default_rng(5)
+    n, k = 600, 4
+    centres = rng.uniform(1, 2, (6, 3))
+    sub = rng.integers(0, 6, n).astype(np.int32)
+    pts = (centres[sub] + rng.normal(0, 0.25, (n, 3))).astype(np.float32)
+    tracer = (rng.random(n) < 0.6).astype(np.uint8)
+    out = O.reassign_targets(pts, tracer, sub, [0, n], k=k, rguess=100.0, params=O.default_params())
+    tix = np.nonzero(tracer)[0]
+    p64 = pts.astype(np.float64)
+    moved = 0
+    for i in np.nonzero(tracer == 0)[0]:
+        d2 = ((p64[tix] - p64[i]) ** 2).sum(1).astype(np.float32)
+        order = np.argsort(d2, kind="stable")[:k]
+        own = np.any(sub[tix[order]] == sub[i])
+        want = -1 if own else int(sub[tix[order[0]]])
+        assert out[i] == want, i
+        moved += want >= 0
+    assert moved > 10  # the case exercises moves
+    assert np.all(out[tracer == 1] == -1)
