@@ -14,6 +14,7 @@
 #include "learn.h"
 #include "tiles.h"
 #include "profile.h"
+#include "reassign.h"
 
 #define WALK_MAX_CHUNKS 16
 // entries whose walk records (at most 3 units each) keep unit offsets in int32
@@ -260,6 +261,19 @@ struct hbtx_ctx
   GraphKey g_key;
   int graph_rounds = 0, graph_captures = 0;
   Batch B;
+  // particle reassignment buffers (grow-only, reused across calls)
+  struct
+  {
+    DArr<float> pos, radii;
+    DArr<uint8_t> tracer;
+    DArr<int32_t> sub, goff, gid, tflag, cnt, loff, qtiles, tval, qval, seg, out, tot;
+    DArr<RsGroup> grp;
+    DArr<uint32_t> bbox;
+    DArr<uint64_t> tkey, qkey;
+    DArr<float4> trec, nodes;
+    DArr<int64_t> lbase;
+    DArr<int4> tiles;
+  } rs;
 };
 
 namespace
@@ -2081,6 +2095,152 @@ int hbtx_profiles(hbtx_ctx *c, const hbtx_params *prm, int64_t n, const float *p
     run_profiles(c, *prm, A, n, (int32_t)nsub, dsub.p, sb.p, sl.p, keys.p, vals.p, dout.p);
     HIPCHK(hipMemcpyAsync(out, dout.p, nsub * sizeof(hbtx_sub_profile), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+  });
+}
+
+// ---------------------------------------------------------------------------
+// particle reassignment
+// ---------------------------------------------------------------------------
+int hbtx_reassign_targets(hbtx_ctx *c, const hbtx_params *prm, int64_t n, const float *pos, const uint8_t *is_tracer,
+                          const int32_t *sub, int64_t ngroups, const int64_t *group_offsets, int32_t k, float rguess,
+                          int32_t *out_target, double *elapsed_ms)
+{
+  if (!c || !prm || !group_offsets || ngroups < 1 || (n > 0 && (!pos || !is_tracer || !sub || !out_target)))
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    REQUIRE(n < (int64_t)0x20000000 && ngroups < (int64_t)0x20000000, HBTX_E_INVALID, "too large");
+    REQUIRE(k >= 1 && k <= 32, HBTX_E_INVALID, "k (NumNeighboursForReassignment) must lie in 1..32");
+    REQUIRE(group_offsets[0] == 0 && group_offsets[ngroups] == n, HBTX_E_INVALID,
+            "group offsets must start at 0 and end at n");
+    for (int64_t g = 0; g < ngroups; g++)
+      REQUIRE(group_offsets[g] <= group_offsets[g + 1], HBTX_E_INVALID, "group offsets must ascend");
+    // GeoTree_t::NearestNeighbours(centre, k, SofteningHalo * 0.1) (src/subhalo_reassign_gas.cpp:144)
+    const float r0 = (rguess > 0.f) ? rguess : (float)(prm->softening * 0.1);
+    REQUIRE(r0 > 0.f && std::isfinite(r0), HBTX_E_INVALID, "the first search radius must be positive");
+    std::vector<float> rad(RS_RADII);
+    rad[0] = r0;
+    for (int j = 1; j < RS_RADII; j++)
+      rad[j] = (float)((double)rad[j - 1] * 1.26); // rguess *= 1.26 (src/geometric_tree.cpp:127)
+    REQUIRE(rad[RS_RADII - 1] >= 1e30f, HBTX_E_INVALID, "the first search radius is too small");
+    if (elapsed_ms)
+      *elapsed_ms = 0.0;
+    if (n == 0)
+      return HBTX_OK;
+    auto &R = c->rs;
+    const int32_t G = (int32_t)ngroups;
+    std::vector<int32_t> hoff(G + 1);
+    for (int32_t g = 0; g <= G; g++)
+      hoff[g] = (int32_t)group_offsets[g];
+    R.pos.alloc(3 * n);
+    R.radii.alloc(RS_RADII);
+    R.tracer.alloc(n);
+    R.sub.alloc(n);
+    R.goff.alloc(G + 1);
+    R.gid.alloc(n);
+    R.tflag.alloc(n + 1);
+    R.cnt.alloc((int64_t)RS_LEVELS * G);
+    R.loff.alloc((int64_t)RS_LEVELS * (G + 1));
+    R.qtiles.alloc(G + 1);
+    R.tval.alloc(n);
+    R.qval.alloc(n);
+    R.tkey.alloc(n);
+    R.qkey.alloc(n);
+    R.seg.alloc(4 * (int64_t)G);
+    R.out.alloc(n);
+    R.tot.alloc(RS_LEVELS + 2);
+    R.grp.alloc(G);
+    R.bbox.alloc(7 * (int64_t)G);
+    R.trec.alloc(n);
+    R.lbase.alloc(RS_LEVELS);
+    hipStream_t st = c->stream;
+    HIPCHK(hipMemcpyAsync(R.pos.p, pos, 3 * n * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(R.tracer.p, is_tracer, n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(R.sub.p, sub, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(R.goff.p, hoff.data(), (G + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(R.radii.p, rad.data(), RS_RADII * sizeof(float), hipMemcpyHostToDevice, st));
+    RsArgs A;
+    A.n = (int32_t)n;
+    A.ngroups = G;
+    A.pos = R.pos.p;
+    A.tracer = R.tracer.p;
+    A.sub = R.sub.p;
+    A.goff = R.goff.p;
+    A.box = prm->box_size;
+    A.half = prm->box_size / 2;
+    A.periodic = prm->periodic ? 1 : 0;
+    A.k = k;
+    A.gid = R.gid.p;
+    A.tflag = R.tflag.p;
+    A.grp = R.grp.p;
+    A.bbox = R.bbox.p;
+    A.cnt = R.cnt.p;
+    A.loff = R.loff.p;
+    A.qtiles_n = R.qtiles.p;
+    A.tkey = R.tkey.p;
+    A.qkey = R.qkey.p;
+    A.tval = R.tval.p;
+    A.qval = R.qval.p;
+    A.seg = R.seg.p;
+    A.trec = R.trec.p;
+    A.nodes = nullptr;
+    A.lbase = R.lbase.p;
+    A.out = R.out.p;
+    HIPCHK(hipEventRecord(c->ev0, st));
+    k_rs_flags<<<blocks_for(n), 256, 0, st>>>(A);
+    scan32(c, R.tflag.p, R.tflag.p, n);
+    k_rs_groups<<<blocks_for(G, 256, 1 << 20), 256, 0, st>>>(A);
+    k_rs_scatter<<<blocks_for(n, RS_THREADS), RS_THREADS, 0, st>>>(A);
+    for (int l = 0; l < RS_LEVELS; l++)
+      scan32(c, R.cnt.p + (int64_t)l * G, R.loff.p + (int64_t)l * (G + 1), G);
+    scan32(c, R.qtiles.p, R.qtiles.p, G);
+    k_rs_totals<<<1, 64, 0, st>>>(A, R.tot.p);
+    int32_t tot[RS_LEVELS + 2];
+    HIPCHK(hipMemcpyAsync(tot, R.tot.p, sizeof(tot), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const int32_t NT = tot[RS_LEVELS], ntiles = tot[RS_LEVELS + 1];
+    int64_t lb[RS_LEVELS], nodes = 0;
+    for (int l = 0; l < RS_LEVELS; l++)
+    {
+      lb[l] = nodes;
+      nodes += tot[l];
+    }
+    R.nodes.alloc(2 * std::max<int64_t>(nodes, 1));
+    R.tiles.alloc(std::max<int32_t>(ntiles, 1));
+    A.nodes = R.nodes.p;
+    HIPCHK(hipMemcpyAsync(R.lbase.p, lb, sizeof(lb), hipMemcpyHostToDevice, st));
+    k_rs_keys<<<blocks_for(n), 256, 0, st>>>(A, NT);
+    k_rs_group_finish<<<blocks_for(G, 256, 1 << 20), 256, 0, st>>>(A);
+    seg_sort(c, R.tkey.p, R.tval.p, G, R.seg.p, R.seg.p + G, std::max<int32_t>(NT, 1), 63);
+    seg_sort(c, R.qkey.p, R.qval.p, G, R.seg.p + 2 * (int64_t)G, R.seg.p + 3 * (int64_t)G,
+             std::max<int64_t>(n - NT, 1), 63);
+    if (NT > 0)
+      k_rs_records<<<blocks_for(NT), 256, 0, st>>>(A, NT);
+    for (int l = 0; l < RS_LEVELS; l++)
+      if (tot[l] > 0)
+        k_rs_level<<<blocks_for(tot[l]), 256, 0, st>>>(A, l, tot[l]);
+    if (ntiles > 0)
+    {
+      k_rs_tiles<<<blocks_for(G, 256, 1 << 20), 256, 0, st>>>(A, R.tiles.p);
+      const int wpb = RS_THREADS / HBTX_WAVE;
+      const int grid = (int)std::min<int64_t>((ntiles + wpb - 1) / wpb, 1 << 20);
+      if (k <= 8)
+        k_rs_knn<8><<<grid, RS_THREADS, 0, st>>>(A, R.tiles.p, ntiles, R.radii.p);
+      else if (k <= 16)
+        k_rs_knn<16><<<grid, RS_THREADS, 0, st>>>(A, R.tiles.p, ntiles, R.radii.p);
+      else
+        k_rs_knn<32><<<grid, RS_THREADS, 0, st>>>(A, R.tiles.p, ntiles, R.radii.p);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev1, st));
+    HIPCHK(hipMemcpyAsync(out_target, R.out.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (elapsed_ms)
+    {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      *elapsed_ms = ms;
+    }
   });
 }
 

@@ -147,6 +147,8 @@ def lib():
         L.hbtx_segmented_sort.argtypes = [P, i64, P, P, i64, P]
         L.hbtx_batch_profiles.argtypes = [P, C.POINTER(Params), P]
         L.hbtx_profiles.argtypes = [P, C.POINTER(Params), i64, P, P, i64, P, P, P, P]
+        L.hbtx_reassign_targets.argtypes = [P, C.POINTER(Params), i64, P, P, P, i64, P, C.c_int32, C.c_float, P,
+                                            C.POINTER(C.c_double)]
         L.hbtx_mlp_create.argtypes = [P, C.c_uint64, C.POINTER(C.c_void_p)]
         L.hbtx_mlp_free.argtypes = [P]
         L.hbtx_mlp_free.restype = None
@@ -323,6 +325,21 @@ class Context:
         self._check(lib().hbtx_profiles(self._h, C.byref(prm), len(mass), _p(pos), _p(mass), len(off) - 1, _p(off),
                                         _p(nb), _p(mb), _p(out)))
         return out
+
+    def reassign_targets(self, pos, is_tracer, sub, group_offsets, k=10, rguess=None, params=None):
+        """SubhaloSnapshot_t::ReassignParticles' choice per particle: the subhalo to
+        move it to, or -1.  Particles are grouped by host halo (group_offsets).
+        Returns (targets int32[n], device milliseconds)."""
+        prm = params if isinstance(params, Params) else make_params(**(params or {}))
+        pos = _f32(pos)
+        tr = np.ascontiguousarray(is_tracer, dtype=np.uint8)
+        sb = np.ascontiguousarray(sub, dtype=np.int32)
+        go = np.ascontiguousarray(group_offsets, dtype=np.int64)
+        out = np.zeros(len(tr), np.int32)
+        ms = C.c_double(0.0)
+        self._check(lib().hbtx_reassign_targets(self._h, C.byref(prm), len(tr), _p(pos), _p(tr), _p(sb), len(go) - 1,
+                                                _p(go), int(k), float(rguess or 0.0), _p(out), C.byref(ms)))
+        return out, ms.value
 
     def segmented_sort(self, keys, vals, offsets):
         k = np.ascontiguousarray(keys, dtype=np.uint64).copy()

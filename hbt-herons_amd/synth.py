@@ -469,3 +469,57 @@ def evolve_iter(pop, nsnap=8, seed=4, shrink=0.9, kick=1.5, jitter=0.02, dt=0.00
 def evolve(pop, nsnap=8, seed=4, shrink=0.9, kick=1.5, jitter=0.02, dt=0.002):
     """evolve_iter as a list."""
     return list(evolve_iter(pop, nsnap, seed, shrink, kick, jitter, dt))
+
+
+class HostHalos:
+    """Host haloes for particle reassignment: particles host halo by host halo
+    (group_offsets), each halo's subhaloes contiguous inside it, a tracer flag and
+    the subhalo of every particle."""
+
+    def __init__(self, pos, is_tracer, sub, group_offsets, sub_offsets):
+        self.pos, self.is_tracer, self.sub = pos, is_tracer, sub
+        self.group_offsets, self.sub_offsets = group_offsets, sub_offsets
+
+    @property
+    def n(self):
+        return len(self.sub)
+
+
+def host_haloes(seed=7, ngroups=40, subs=(1, 8), n_sub=(50, 3000), tracer_frac=0.6, stray=0.2, rs=0.02,
+                box=None, spread=0.05):
+    """Host haloes of a hydrodynamical run: every subhalo an NFW clump around a
+    centre near its host's, a fraction tracer_frac of its particles tracers (dark
+    matter and stars), and a fraction `stray` of its gas drawn from a neighbouring
+    subhalo's clump (the particles that reassignment moves).  With box set, host
+    centres are uniform in the box and positions wrapped into it."""
+    rng = np.random.default_rng(seed)
+    pos_l, tr_l, sub_l, goff, soff = [], [], [], [0], [0]
+    sid = 0
+    for g in range(ngroups):
+        hc = rng.uniform(0, box, 3) if box else rng.uniform(0.5, 1.5, 3) + 3.0 * g
+        ns = int(rng.integers(subs[0], subs[1] + 1))
+        centres = hc + rng.normal(0, spread, (ns, 3)) * (np.arange(ns) > 0)[:, None]
+        sizes = rng.integers(n_sub[0], n_sub[1] + 1, ns)
+        scale = rs * (sizes / sizes.max()) ** (1 / 3) + 1e-3
+        for s in range(ns):
+            m = int(sizes[s])
+            r = scale[s] * nfw_inverse_radius(rng.random(m), 10.0)
+            p = centres[s] + r[:, None] * _unit_vectors(rng, m)
+            tr = rng.random(m) < tracer_frac
+            gas = np.nonzero(~tr)[0]
+            if ns > 1 and len(gas):
+                move = gas[rng.random(len(gas)) < stray]
+                other = (s + rng.integers(1, ns, len(move))) % ns
+                rr = scale[other] * nfw_inverse_radius(rng.random(len(move)), 10.0)
+                p[move] = centres[other] + rr[:, None] * _unit_vectors(rng, len(move))
+            pos_l.append(p)
+            tr_l.append(tr)
+            sub_l.append(np.full(m, sid, np.int32))
+            sid += 1
+            soff.append(soff[-1] + m)
+        goff.append(soff[-1])
+    pos = np.concatenate(pos_l)
+    if box:
+        pos = np.mod(pos, box)
+    return HostHalos(pos.astype(np.float32), np.concatenate(tr_l).astype(np.uint8), np.concatenate(sub_l),
+                     np.asarray(goff, np.int64), np.asarray(soff, np.int64))

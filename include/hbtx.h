@@ -14,6 +14,8 @@
  *   hbtx_tree_binding_energy GravityTree_t::BindingEnergy                  src/gravity_tree.cpp:168-178
  *   hbtx_tree_free           OctTree_t::Clear                              src/oct_tree.tpp:158-164
  *   hbtx_params              HBTConfig fields used by unbinding            src/config_parser.h:98-172, :200-212, :641-649
+ *   hbtx_reassign_targets    SubhaloSnapshot_t::ReassignParticles (the     src/subhalo_reassign_gas.cpp:65-197,
+ *                            choice of target) with GeoTree_t's kNN        src/geometric_tree.cpp:108-210
  */
 #ifndef HBTX_H
 #define HBTX_H
@@ -156,6 +158,27 @@ int hbtx_batch_profiles(hbtx_ctx *ctx, const hbtx_params *prm, hbtx_sub_profile 
  * the first nbound[s] of them bound (the first is the centre), Mbound mbound[s]. */
 int hbtx_profiles(hbtx_ctx *ctx, const hbtx_params *prm, int64_t n, const float *pos, const float *mass, int64_t nsub,
                   const int64_t *offsets, const int64_t *nbound, const float *mbound, hbtx_sub_profile *out);
+
+/* Particle reassignment, the choice made by SubhaloSnapshot_t::ReassignParticles
+ * (src/subhalo_reassign_gas.cpp:65-197): for every non-tracer particle, the k
+ * nearest tracers of its host halo as GeoTree_t::NearestNeighbours finds them
+ * (src/geometric_tree.cpp:108-141: a sphere of radius rguess grown by 1.26 until
+ * min(tracers, k) are collected, the tracers of every sphere collected again into
+ * one bounded heap); the particle moves to the subhalo of the nearest tracer
+ * unless one of those neighbours belongs to its own subhalo.
+ * Particles come host halo by host halo: halo g owns [group_offsets[g],
+ * group_offsets[g+1]) (ngroups+1 ascending values, 0 .. n), in the order the
+ * reference visits them (the subhaloes of MemberTable.SubGroups[g] in order, each
+ * one's Particles).  pos float[n][3] comoving, is_tracer uint8[n]
+ * (Particle_t::IsTracer), sub int32[n] the particle's subhalo.
+ * out_target int32[n]: the subhalo to append the particle to, or -1 (tracers,
+ * particles that stay).  k = NumNeighboursForReassignment, 1..32; rguess <= 0
+ * selects SofteningHalo * 0.1 (prm->softening).  prm supplies the periodic box.
+ * *elapsed_ms (optional): device time from the uploaded inputs to the targets. */
+int hbtx_reassign_targets(hbtx_ctx *ctx, const hbtx_params *prm, int64_t n, const float *pos,
+                          const uint8_t *is_tracer, const int32_t *sub, int64_t ngroups,
+                          const int64_t *group_offsets, int32_t k, float rguess, int32_t *out_target,
+                          double *elapsed_ms);
 
 /* Kernel-level counters of the last run (work counts need HBTX_COUNT): stats[0..15], see hbtx_stat_name;
  * 12 walk ms, 13 rounds, 14 rounds replayed from graphs, 15 graph captures. */
