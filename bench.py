@@ -104,7 +104,8 @@ def plain_gpu_estimate(hbtx, pop, prm, share):
 def other_configs(hbtx, synth, ctx, log):
     """The other workloads, timed as batched unbinding (plain method, inputs
     resident): one NFW halo of 10,000 particles + 2,000 interlopers, 20,000
-    power-law subhaloes, and 5,000 hydro-mixture subhaloes with thermal energy
+    power-law subhaloes, and 5,000 hydro-mixture subhaloes with thermal energy;
+    and particle reassignment over 200 host haloes of about 3.5e7 particles
     (their parity with the oracle is tested in tests/)."""
     out = {}
     cases = [
@@ -122,7 +123,27 @@ def other_configs(hbtx, synth, ctx, log):
                "bound_particles": int(res["result"]["nbound"].sum())}
         out[name] = rec
         log(f"{name}: {rec['ms_per_step']:.2f} ms, {rec['particles_per_s']:.3e} particles/s")
+    out["reassign_host_halos"] = reassign_line(synth, ctx, log)
     return out
+
+
+def reassign_line(synth, ctx, log, reps=3):
+    """Particle reassignment (SubhaloSnapshot_t::ReassignParticles) over 200 host
+    haloes with the hydro mixture's types: 79% tracers (dark matter, stars), 21% gas
+    and black holes; device time from the uploaded inputs to the targets."""
+    h = synth.host_haloes(seed=21, ngroups=200, subs=(1, 12), n_sub=(50, 50000), tracer_frac=0.79)
+    ms = []
+    for rep in range(reps + 1):
+        tgt, m = ctx.reassign_targets(h.pos, h.is_tracer, h.sub, h.group_offsets, k=10)
+        if rep:
+            ms.append(m)
+    rec = {"particles": int(h.n), "host_halos": int(len(h.group_offsets) - 1),
+           "subhalos": int(len(h.sub_offsets) - 1), "non_tracers": int((h.is_tracer == 0).sum()),
+           "k": 10, "ms_per_step": float(np.mean(ms)), "particles_per_s": h.n / (float(np.mean(ms)) / 1e3),
+           "moved": int((tgt >= 0).sum()), "timing": "device, inputs uploaded"}
+    log(f"reassign_host_halos: {rec['ms_per_step']:.2f} ms, {rec['particles_per_s']:.3e} particles/s, "
+        f"{rec['moved']} moved")
+    return rec
 
 
 def main():
@@ -156,6 +177,8 @@ def main():
                     help="launch every round's kernels directly instead of replaying the captured round graphs")
     ap.add_argument("--learned-only", action="store_true",
                     help="time only the learned step (for kernel profiles); implies --no-plain --no-cpu")
+    ap.add_argument("--reassign-only", action="store_true",
+                    help="time only particle reassignment (for kernel profiles) and print its record")
     args = ap.parse_args()
     if args.learned_only:
         args.no_plain = args.no_cpu = args.no_tree = args.no_configs = True
@@ -163,6 +186,11 @@ def main():
         print(json.dumps({"error": "this build runs on exactly one GPU; how the path would shard over "
                                    "several is described in DESIGN.md, not built"}))
         sys.exit(2)
+    if args.reassign_only:
+        import hbtx
+        import synth
+        print(json.dumps({"reassign_host_halos": reassign_line(synth, hbtx.Context(0), log)}))
+        return
 
     import hbtx
     import learned

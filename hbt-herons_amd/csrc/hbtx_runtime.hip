@@ -2190,7 +2190,7 @@ int hbtx_reassign_targets(hbtx_ctx *c, const hbtx_params *prm, int64_t n, const 
     k_rs_flags<<<blocks_for(n), 256, 0, st>>>(A);
     scan32(c, R.tflag.p, R.tflag.p, n);
     k_rs_groups<<<blocks_for(G, 256, 1 << 20), 256, 0, st>>>(A);
-    k_rs_scatter<<<blocks_for(n, RS_THREADS), RS_THREADS, 0, st>>>(A);
+    k_rs_scatter<<<blocks_for(n, RS_CHUNK), RS_THREADS, 0, st>>>(A);
     for (int l = 0; l < RS_LEVELS; l++)
       scan32(c, R.cnt.p + (int64_t)l * G, R.loff.p + (int64_t)l * (G + 1), G);
     scan32(c, R.qtiles.p, R.qtiles.p, G);
@@ -2211,9 +2211,9 @@ int hbtx_reassign_targets(hbtx_ctx *c, const hbtx_params *prm, int64_t n, const 
     HIPCHK(hipMemcpyAsync(R.lbase.p, lb, sizeof(lb), hipMemcpyHostToDevice, st));
     k_rs_keys<<<blocks_for(n), 256, 0, st>>>(A, NT);
     k_rs_group_finish<<<blocks_for(G, 256, 1 << 20), 256, 0, st>>>(A);
-    seg_sort(c, R.tkey.p, R.tval.p, G, R.seg.p, R.seg.p + G, std::max<int32_t>(NT, 1), 63);
+    seg_sort(c, R.tkey.p, R.tval.p, G, R.seg.p, R.seg.p + G, std::max<int32_t>(NT, 1), 3 * RS_KEY_BITS);
     seg_sort(c, R.qkey.p, R.qval.p, G, R.seg.p + 2 * (int64_t)G, R.seg.p + 3 * (int64_t)G,
-             std::max<int64_t>(n - NT, 1), 63);
+             std::max<int64_t>(n - NT, 1), 3 * RS_KEY_BITS);
     if (NT > 0)
       k_rs_records<<<blocks_for(NT), 256, 0, st>>>(A, NT);
     for (int l = 0; l < RS_LEVELS; l++)
@@ -2221,11 +2221,13 @@ int hbtx_reassign_targets(hbtx_ctx *c, const hbtx_params *prm, int64_t n, const 
         k_rs_level<<<blocks_for(tot[l]), 256, 0, st>>>(A, l, tot[l]);
     if (ntiles > 0)
     {
-      k_rs_tiles<<<blocks_for(G, 256, 1 << 20), 256, 0, st>>>(A, R.tiles.p);
+      k_rs_tiles<<<blocks_for(ntiles), 256, 0, st>>>(A, R.tiles.p, ntiles);
       const int wpb = RS_THREADS / HBTX_WAVE;
       const int grid = (int)std::min<int64_t>((ntiles + wpb - 1) / wpb, 1 << 20);
       if (k <= 8)
         k_rs_knn<8><<<grid, RS_THREADS, 0, st>>>(A, R.tiles.p, ntiles, R.radii.p);
+      else if (k <= 10) // the reference's default NumNeighboursForReassignment
+        k_rs_knn<10><<<grid, RS_THREADS, 0, st>>>(A, R.tiles.p, ntiles, R.radii.p);
       else if (k <= 16)
         k_rs_knn<16><<<grid, RS_THREADS, 0, st>>>(A, R.tiles.p, ntiles, R.radii.p);
       else

@@ -16,8 +16,8 @@
 // leading k entries.
 //
 // Layout per host halo g (particles [goff[g], goff[g+1])):
-//  * tracers sorted by a 63-bit Morton key of their offset from the halo's first
-//    particle (periodic minimum image), records trec = (x, y, z, subhalo bits);
+//  * tracers sorted by a Morton key of their offset from the halo's first
+//    particle (periodic minimum image), records trec = (x, y, z, particle index bits);
 //  * leaves of RS_LEAF consecutive sorted tracers, RS_FAN leaves per node one
 //    level up, until one root: levels stored level by level, boxes (lo, hi) in
 //    offset coordinates widened by a rounding margin;
@@ -166,78 +166,136 @@ HBTX_DEV void rs_wave_min_max(float &mn, float &mx)
 }
 
 // Pass 2 over particles: tracer / non-tracer lists (stable), box of the tracers'
-// offsets and largest |offset| per host halo (wave-aggregated atomics when the
-// wave lies in one host halo).
+// offsets and largest |offset| per host halo.  A block takes RS_CHUNK consecutive
+// particles; every thread keeps running extrema for the host halo it is in and
+// flushes them with atomics when the host halo changes, and at the end the block
+// combines them first when all its threads end in one host halo (the usual case),
+// so a host halo receives about one update per chunk.
+#define RS_CHUNK (RS_THREADS * 16)
+struct RsBox
+{
+  float l[3], h[3], e;
+  HBTX_DEV void reset()
+  {
+    for (int d = 0; d < 3; d++)
+    {
+      l[d] = INFINITY;
+      h[d] = -INFINITY;
+    }
+    e = 0.f;
+  }
+  HBTX_DEV void flush(uint32_t *bb) const
+  {
+    if (l[0] <= h[0])
+      for (int d = 0; d < 3; d++)
+      {
+        atomicMin(bb + d, float_to_ordered(l[d]));
+        atomicMax(bb + 3 + d, float_to_ordered(h[d]));
+      }
+    atomicMax(bb + 6, __float_as_uint(e));
+  }
+};
+
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(RsArgs A)
 {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < A.n; base += stride)
+  __shared__ float red[7][RS_THREADS / HBTX_WAVE];
+  __shared__ int first_s, same_s;
+  const int64_t nchunk = ((int64_t)A.n + RS_CHUNK - 1) / RS_CHUNK;
+  for (int64_t ch = blockIdx.x; ch < nchunk; ch += gridDim.x)
   {
-    const int64_t i = base + threadIdx.x;
-    const bool valid = i < A.n;
-    int g = valid ? A.gid[i] : -1;
-    bool tr = false;
-    float ox = 0.f, oy = 0.f, oz = 0.f;
-    if (valid)
+    RsBox B;
+    B.reset();
+    int cur = -1;
+    for (int r = 0; r < RS_CHUNK / RS_THREADS; r++)
     {
+      const int64_t i = ch * RS_CHUNK + (int64_t)r * RS_THREADS + threadIdx.x;
+      if (i >= A.n)
+        break;
+      const int g = A.gid[i];
+      if (g != cur)
+      {
+        if (cur >= 0)
+          B.flush(A.bbox + 7 * (int64_t)cur);
+        B.reset();
+        cur = g;
+      }
       const RsGroup &G = A.grp[g];
-      tr = A.tracer[i] != 0;
+      const bool tr = A.tracer[i] != 0;
       const int s = A.tflag[i];
       if (tr)
         A.tval[s] = (int32_t)i;
       else
         A.qval[i - s] = (int32_t)i;
-      ox = rs_offset(A.pos[3 * i], G.p0x, A);
-      oy = rs_offset(A.pos[3 * i + 1], G.p0y, A);
-      oz = rs_offset(A.pos[3 * i + 2], G.p0z, A);
-    }
-    const float ext = fmaxf(fabsf(ox), fmaxf(fabsf(oy), fabsf(oz)));
-    const int g0 = __shfl(g, 0, HBTX_WAVE);
-    const bool same = ballot(valid && g == g0) == ballot(valid);
-    if (same)
-    {
-      // one host halo: reduce over the wave, lane 0 updates
-      float lx = tr ? ox : INFINITY, hx = tr ? ox : -INFINITY;
-      float ly = tr ? oy : INFINITY, hy = tr ? oy : -INFINITY;
-      float lz = tr ? oz : INFINITY, hz = tr ? oz : -INFINITY;
-      float e0 = valid ? ext : 0.f, e1 = e0;
-      rs_wave_min_max(lx, hx);
-      rs_wave_min_max(ly, hy);
-      rs_wave_min_max(lz, hz);
-      rs_wave_min_max(e1, e0);
-      if (lane_id() == 0 && g0 >= 0)
+      const float o[3] = {rs_offset(A.pos[3 * i], G.p0x, A), rs_offset(A.pos[3 * i + 1], G.p0y, A),
+                          rs_offset(A.pos[3 * i + 2], G.p0z, A)};
+      for (int d = 0; d < 3; d++)
       {
-        uint32_t *bb = A.bbox + 7 * (int64_t)g0;
-        if (lx <= hx)
+        if (tr)
         {
-          atomicMin(bb + 0, float_to_ordered(lx));
-          atomicMin(bb + 1, float_to_ordered(ly));
-          atomicMin(bb + 2, float_to_ordered(lz));
-          atomicMax(bb + 3, float_to_ordered(hx));
-          atomicMax(bb + 4, float_to_ordered(hy));
-          atomicMax(bb + 5, float_to_ordered(hz));
+          B.l[d] = fminf(B.l[d], o[d]);
+          B.h[d] = fmaxf(B.h[d], o[d]);
         }
-        atomicMax(bb + 6, __float_as_uint(e0));
+        B.e = fmaxf(B.e, fabsf(o[d]));
       }
     }
-    else if (valid)
+    // block combine when every thread ends in the host halo of thread 0
+    if (threadIdx.x == 0)
     {
-      uint32_t *bb = A.bbox + 7 * (int64_t)g;
-      if (tr)
-      {
-        atomicMin(bb + 0, float_to_ordered(ox));
-        atomicMin(bb + 1, float_to_ordered(oy));
-        atomicMin(bb + 2, float_to_ordered(oz));
-        atomicMax(bb + 3, float_to_ordered(ox));
-        atomicMax(bb + 4, float_to_ordered(oy));
-        atomicMax(bb + 5, float_to_ordered(oz));
-      }
-      atomicMax(bb + 6, __float_as_uint(ext));
+      first_s = cur;
+      same_s = 1;
     }
+    __syncthreads();
+    const int first = first_s;
+    if (cur != first)
+      same_s = 0; // every writer stores the same value
+    __syncthreads();
+    const bool same_all = same_s != 0; // uniform over the block
+    __syncthreads();
+    if (same_all && first >= 0)
+    {
+      float v[7] = {B.l[0], B.l[1], B.l[2], -B.h[0], -B.h[1], -B.h[2], -B.e};
+#pragma unroll
+      for (int j = 0; j < 7; j++)
+        for (int d = 1; d < HBTX_WAVE; d <<= 1)
+          v[j] = fminf(v[j], __shfl_xor(v[j], d, HBTX_WAVE));
+      __syncthreads();
+      if (lane_id() == 0)
+        for (int j = 0; j < 7; j++)
+          red[j][threadIdx.x / HBTX_WAVE] = v[j];
+      __syncthreads();
+      if (threadIdx.x == 0)
+      {
+        RsBox T;
+        for (int d = 0; d < 3; d++)
+        {
+          T.l[d] = red[d][0];
+          T.h[d] = -red[3 + d][0];
+        }
+        T.e = -red[6][0];
+        for (int w = 1; w < RS_THREADS / HBTX_WAVE; w++)
+        {
+          for (int d = 0; d < 3; d++)
+          {
+            T.l[d] = fminf(T.l[d], red[d][w]);
+            T.h[d] = fmaxf(T.h[d], -red[3 + d][w]);
+          }
+          T.e = fmaxf(T.e, -red[6][w]);
+        }
+        T.flush(A.bbox + 7 * (int64_t)first);
+      }
+    }
+    else if (cur >= 0)
+      B.flush(A.bbox + 7 * (int64_t)cur);
+    __syncthreads();
   }
 }
 
-// Morton cube of a host halo: origin and cells per unit length.
+// Morton cube of a host halo: origin and cells per unit length.  Keys use
+// RS_KEY_BITS bits per axis: they order the tracers into compact leaves and the
+// queries into compact waves (any order is correct; a coarser key only sorts faster).
+#ifndef RS_KEY_BITS
+#define RS_KEY_BITS 10
+#endif
 HBTX_DEV void rs_cube(const uint32_t *bb, float &x0, float &y0, float &z0, double &scale)
 {
   x0 = ordered_to_float(bb[0]);
@@ -245,13 +303,13 @@ HBTX_DEV void rs_cube(const uint32_t *bb, float &x0, float &y0, float &z0, doubl
   z0 = ordered_to_float(bb[2]);
   const double side = fmax((double)ordered_to_float(bb[3]) - x0,
                            fmax((double)ordered_to_float(bb[4]) - y0, (double)ordered_to_float(bb[5]) - z0));
-  scale = (side > 0.0) ? 2097151.0 / side : 0.0;
+  scale = (side > 0.0) ? (double)((1 << RS_KEY_BITS) - 1) / side : 0.0;
 }
 
 HBTX_DEV uint32_t rs_cell(float o, float o0, double scale)
 {
   const double c = ((double)o - (double)o0) * scale;
-  return (uint32_t)fmin(fmax(c, 0.0), 2097151.0);
+  return (uint32_t)fmin(fmax(c, 0.0), (double)((1 << RS_KEY_BITS) - 1));
 }
 
 HBTX_DEV uint64_t rs_key(float ox, float oy, float oz, const uint32_t *bb)
@@ -305,14 +363,14 @@ __global__ void k_rs_group_finish(RsArgs A)
   G.top = top;
 }
 
-// Tracer records in sorted order: original position and subhalo.
+// Tracer records in sorted order: original position and particle index.
 __global__ void k_rs_records(RsArgs A, int32_t nt_all)
 {
   GRID_STRIDE(s, nt_all)
   {
     const int i = A.tval[s];
     A.trec[s] = make_float4(A.pos[3 * (int64_t)i], A.pos[3 * (int64_t)i + 1], A.pos[3 * (int64_t)i + 2],
-                            __int_as_float(A.sub[i]));
+                            __int_as_float(i));
   }
 }
 
@@ -382,18 +440,16 @@ __global__ void k_rs_totals(RsArgs A, int32_t *tot)
     tot[l] = A.qtiles_n[A.ngroups];
 }
 
-// tile = (host halo, first query, end query, 0)
-__global__ void k_rs_tiles(RsArgs A, int4 *tiles)
+// tile t = (host halo, first query, end query, 0), one thread per tile
+__global__ void k_rs_tiles(RsArgs A, int4 *tiles, int32_t ntiles)
 {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= A.ngroups)
-    return;
-  const RsGroup &G = A.grp[g];
-  int t = A.qtiles_n[g];
-  if (G.te == G.tb)
-    return;
-  for (int q = G.qb; q < G.qe; q += HBTX_WAVE, t++)
+  GRID_STRIDE(t, ntiles)
+  {
+    const int g = rs_group_of(A.qtiles_n, A.ngroups, (int)t);
+    const RsGroup &G = A.grp[g];
+    const int q = G.qb + ((int)t - A.qtiles_n[g]) * HBTX_WAVE;
     tiles[t] = make_int4(g, q, min(G.qe, q + HBTX_WAVE), 0);
+  }
 }
 
 // Squared pair distance as the reference computes it: double differences of
@@ -433,13 +489,67 @@ HBTX_DEV float rs_gap(float q, float lo, float hi)
   return fmaxf(0.f, fmaxf(lo - q, q - hi));
 }
 
-// Union walk of one wave over its host halo's boxes: every lane keeps the k
-// smallest squared distances (ascending in d[K-k..K-1]; d[0..K-k) hold -inf and
-// never move), a box is opened when any lane's k-th distance reaches it, and
-// leaves are read at wave-uniform addresses.  Then the move rule per lane.
+HBTX_DEV float rd_lane(float v, int l)
+{
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+HBTX_DEV int rd_lane(int v, int l)
+{
+  return __builtin_amdgcn_readlane(v, l);
+}
+HBTX_DEV float wr_lane(float v, int l, float old)
+{
+  return (lane_id() == l) ? v : old;
+}
+HBTX_DEV int wr_lane(int v, int l, int old)
+{
+  return (lane_id() == l) ? v : old;
+}
+
+// Squared lower bound of the distance from a query offset to a box.
+HBTX_DEV float rs_box_gap2(float ux, float uy, float uz, float lx, float ly, float lz, float hx, float hy,
+                           float hz, bool wrap, float box)
+{
+  float gx = rs_gap(ux, lx, hx), gy = rs_gap(uy, ly, hy), gz = rs_gap(uz, lz, hz);
+  if (wrap)
+  {
+    gx = fminf(gx, fminf(rs_gap(ux + box, lx, hx), rs_gap(ux - box, lx, hx)));
+    gy = fminf(gy, fminf(rs_gap(uy + box, ly, hy), rs_gap(uy - box, ly, hy)));
+    gz = fminf(gz, fminf(rs_gap(uz + box, lz, hz), rs_gap(uz - box, lz, hz)));
+    // q +- box is rounded to the float spacing of box
+    const float slack = box * 2.5e-7f;
+    gx = fmaxf(0.f, gx - slack);
+    gy = fmaxf(0.f, gy - slack);
+    gz = fmaxf(0.f, gz - slack);
+  }
+  return (gx * gx + gy * gy + gz * gz) * (1.0f - 1.0e-6f);
+}
+
+// Neighbours are ordered by (squared distance, particle index), one 64-bit key
+// (the float bits of a non-negative distance order as integers): the reference
+// orders exactly equal float distances by its tree's traversal order and its
+// heap's internals, which no other search reproduces.
+HBTX_DEV uint64_t rs_nkey(float d2, int pid)
+{
+  return ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)pid;
+}
+HBTX_DEV float rs_nkey_d2(uint64_t key)
+{
+  return __uint_as_float((uint32_t)(key >> 32));
+}
+
+// Union walk of one wave over its host halo's boxes.  Every lane keeps the k
+// nearest (keys ascending in key[K-k..K-1]; key[0..K-k) hold 0 and never move).  The wave first reads the leaves its queries' keys fall in (the
+// queries and tracers share one Morton order, so these are a short run), then
+// walks the boxes depth first with an explicit stack held across the lanes
+// (entry e in lane e: level, index and box; at most 8 + 7 (levels - 2) entries):
+// a popped box is opened when any lane's k-th distance reaches it, its children's
+// boxes arrive in one vector load, and a leaf's 32 tracers arrive in one vector
+// load and are broadcast lane by lane.  A float lower bound screens every pair
+// before the exact double distance.  Then the move rule per lane.
 template <int K>
-__global__ __launch_bounds__(RS_THREADS) void k_rs_knn(RsArgs A, const int4 *__restrict__ tiles,
-                                                       int32_t ntiles, const float *__restrict__ radii)
+__global__ __launch_bounds__(RS_THREADS) void k_rs_knn(RsArgs A, const int4 *__restrict__ tiles, int32_t ntiles,
+                                                       const float *__restrict__ radii)
 {
   __shared__ float rad[RS_RADII];
   for (int j = threadIdx.x; j < RS_RADII; j += RS_THREADS)
@@ -454,48 +564,75 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_knn(RsArgs A, const int4 *__r
     const int g = uniform(tl.x);
     const RsGroup G = A.grp[g];
     const int q = tl.y + lane;
-    const bool active = q < tl.z;
-    const int pi = active ? A.qval[q] : 0;
+    const int nact = uniform(tl.z - tl.y); // active lanes are 0 .. nact-1
+    const bool active = lane < nact;
+    const int pi = active ? A.qval[q] : A.qval[tl.y];
     const float qx = A.pos[3 * (int64_t)pi], qy = A.pos[3 * (int64_t)pi + 1], qz = A.pos[3 * (int64_t)pi + 2];
     const int own = A.sub[pi];
     const float ux = rs_offset(qx, G.p0x, A), uy = rs_offset(qy, G.p0y, A), uz = rs_offset(qz, G.p0z, A);
     const int nt = G.te - G.tb;
-    float d[K];
-    int id[K];
+    const bool wrap = G.wrap != 0;
+    // lane l < RS_LEVELS: first node and node count of level l in this host halo
+    int lb_lo = 0, lb_hi = 0, lcnt = 0;
+    if (lane < RS_LEVELS)
+    {
+      const int64_t b = A.lbase[lane] + A.loff[(int64_t)lane * (A.ngroups + 1) + g];
+      lb_lo = (int)(uint32_t)b;
+      lb_hi = (int)(b >> 32);
+      lcnt = A.cnt[lane * A.ngroups + g];
+    }
+    auto level_base = [&](int l) -> int64_t {
+      return ((int64_t)rd_lane(lb_hi, l) << 32) | (uint32_t)rd_lane(lb_lo, l);
+    };
+    uint64_t key[K];
 #pragma unroll
     for (int j = 0; j < K; j++)
-    {
-      d[j] = (j < K - k) ? -INFINITY : INFINITY;
-      id[j] = -1;
-    }
-    auto insert = [&](float x, int tid) {
+      key[j] = (j < K - k) ? 0ull : ~0ull;
+    float kth = INFINITY; // distance part of key[K-1]
+    auto insert = [&](uint64_t x) {
 #pragma unroll
       for (int j = K - 1; j > 0; j--)
       {
-        const bool sh = d[j - 1] > x;
-        const bool pl = !sh && d[j] > x;
-        d[j] = sh ? d[j - 1] : (pl ? x : d[j]);
-        id[j] = sh ? id[j - 1] : (pl ? tid : id[j]);
+        const bool sh = x < key[j - 1];
+        const bool pl = !sh && x < key[j];
+        key[j] = sh ? key[j - 1] : (pl ? x : key[j]);
       }
-      if (d[0] > x)
+      if (x < key[0])
+        key[0] = x;
+      kth = (key[K - 1] == ~0ull) ? INFINITY : rs_nkey_d2(key[K - 1]); // +inf until k are held
+    };
+    const float slack = A.periodic ? A.box * 2.5e-7f : 0.f;
+    auto consider = [&](float tx, float ty, float tz, int pid) {
+      // float lower bound: each difference is rounded once (and wrapped with an
+      // error below the float spacing of box), the sum once more
+      float ex = tx - qx, ey = ty - qy, ez = tz - qz;
+      if (A.periodic)
       {
-        d[0] = x;
-        id[0] = tid;
+        ex = fmaxf(fabsf(nearest_f(ex, A.box, A.half)) - slack, 0.f);
+        ey = fmaxf(fabsf(nearest_f(ey, A.box, A.half)) - slack, 0.f);
+        ez = fmaxf(fabsf(nearest_f(ez, A.box, A.half)) - slack, 0.f);
+      }
+      const float e2 = (ex * ex + ey * ey + ez * ez) * (1.0f - 4.0e-6f);
+      if (active && e2 <= kth)
+      {
+        double dx, dy, dz;
+        const uint64_t x = rs_nkey((float)rs_r2(tx, ty, tz, qx, qy, qz, A, dx, dy, dz), pid);
+        if (x < key[K - 1])
+          insert(x);
       }
     };
-    auto consider = [&](int s) {
-      const float4 r = A.trec[s];
-      double dx, dy, dz;
-      const float d2 = (float)rs_r2(r.x, r.y, r.z, qx, qy, qz, A, dx, dy, dz);
-      if (d2 < d[K - 1])
-        insert(d2, s);
+    auto leaf = [&](int li) {
+      const int b = G.tb + li * RS_LEAF, n = min(G.te - b, RS_LEAF);
+      float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (lane < n)
+        r = A.trec[b + lane];
+      for (int j = 0; j < n; j++)
+        consider(rd_lane(r.x, j), rd_lane(r.y, j), rd_lane(r.z, j), __float_as_int(rd_lane(r.w, j)));
     };
-    // home leaf: where the query's key falls among the sorted tracer keys
-    const int c0 = A.cnt[g];
-    int home = -1;
-    if (active)
+    // the leaves the queries' keys fall in
+    int home = 0;
     {
-      const uint64_t key = A.qkey[q];
+      const uint64_t key = A.qkey[active ? q : tl.y];
       int lo = G.tb, hi = G.te;
       while (lo < hi)
       {
@@ -505,76 +642,82 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_knn(RsArgs A, const int4 *__r
         else
           hi = mid;
       }
-      home = min((lo - G.tb) / RS_LEAF, c0 - 1);
-      const int b = G.tb + home * RS_LEAF, e = min(G.te, b + RS_LEAF);
-      for (int s = b; s < e; s++)
-        consider(s);
+      home = min((lo - G.tb) / RS_LEAF, rd_lane(lcnt, 0) - 1);
     }
-    // stackless walk: node (l, i); children of (l, i) are (l-1, 8i ..)
-    const int top = G.top;
-    int l = top, i = 0;
-    while (true)
+    int h[4];
     {
-      const int32_t *lo = A.loff + (int64_t)l * (A.ngroups + 1);
-      const int64_t node = A.lbase[l] + lo[g] + i;
-      const float4 bl = A.nodes[2 * node], bh = A.nodes[2 * node + 1];
-      float gx = rs_gap(ux, bl.x, bh.x), gy = rs_gap(uy, bl.y, bh.y), gz = rs_gap(uz, bl.z, bh.z);
-      if (G.wrap)
+      const int h0 = rd_lane(home, 0), h3 = rd_lane(home, nact - 1);
+      if (h3 - h0 < 4)
       {
-        gx = fminf(gx, fminf(rs_gap(ux + A.box, bl.x, bh.x), rs_gap(ux - A.box, bl.x, bh.x)));
-        gy = fminf(gy, fminf(rs_gap(uy + A.box, bl.y, bh.y), rs_gap(uy - A.box, bl.y, bh.y)));
-        gz = fminf(gz, fminf(rs_gap(uz + A.box, bl.z, bh.z), rs_gap(uz - A.box, bl.z, bh.z)));
-        // q +- box is rounded to the float spacing of box
-        const float slack = A.box * 2.5e-7f;
-        gx = fmaxf(0.f, gx - slack);
-        gy = fmaxf(0.f, gy - slack);
-        gz = fmaxf(0.f, gz - slack);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          h[j] = (h0 + j <= h3) ? h0 + j : -1;
       }
-      const float gap2 = (gx * gx + gy * gy + gz * gz) * (1.0f - 1.0e-6f);
-      const bool need = active && gap2 <= d[K - 1];
-      if (ballot(need))
+      else
       {
-        if (l > 0)
-        {
-          l--;
-          i *= RS_FAN;
-          continue;
-        }
-        // the lanes whose home leaf this is have read it already
-        const bool mine = need && i != home;
-        if (ballot(mine))
-        {
-          const int b = G.tb + i * RS_LEAF, e = min(G.te, b + RS_LEAF);
-          for (int s = b; s < e; s++)
-          {
-            const float4 r = A.trec[s];
-            double dx, dy, dz;
-            const float d2 = (float)rs_r2(r.x, r.y, r.z, qx, qy, qz, A, dx, dy, dz);
-            if (mine && d2 < d[K - 1])
-              insert(d2, s);
-          }
-        }
+        h[0] = h0;
+        h[1] = rd_lane(home, (nact - 1) / 3);
+        h[2] = rd_lane(home, 2 * (nact - 1) / 3);
+        h[3] = h3;
+        h[1] = (h[1] == h[0]) ? -1 : h[1];
+        h[2] = (h[2] == h[0] || h[2] == h[1]) ? -1 : h[2];
+        h[3] = (h[3] == h[0] || h[3] == h[1] || h[3] == h[2]) ? -1 : h[3];
       }
-      // next node: the following sibling, else up
-      bool done = false;
-      while (true)
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (h[j] >= 0)
+        leaf(h[j]);
+    // depth-first walk from the root
+    int s_code = 0;
+    float s_lx = 0.f, s_ly = 0.f, s_lz = 0.f, s_hx = 0.f, s_hy = 0.f, s_hz = 0.f;
+    int sp = 0;
+    auto push = [&](int code, float lx, float ly, float lz, float hx, float hy, float hz) {
+      s_code = wr_lane(code, sp, s_code);
+      s_lx = wr_lane(lx, sp, s_lx);
+      s_ly = wr_lane(ly, sp, s_ly);
+      s_lz = wr_lane(lz, sp, s_lz);
+      s_hx = wr_lane(hx, sp, s_hx);
+      s_hy = wr_lane(hy, sp, s_hy);
+      s_hz = wr_lane(hz, sp, s_hz);
+      sp++;
+    };
+    {
+      const int64_t root = level_base(G.top);
+      const float4 bl = A.nodes[2 * root], bh = A.nodes[2 * root + 1];
+      push(G.top << 24, bl.x, bl.y, bl.z, bh.x, bh.y, bh.z);
+    }
+    while (sp > 0)
+    {
+      sp--;
+      const int code = rd_lane(s_code, sp);
+      const int l = code >> 24, i = code & 0xffffff;
+      const float gap2 = rs_box_gap2(ux, uy, uz, rd_lane(s_lx, sp), rd_lane(s_ly, sp), rd_lane(s_lz, sp),
+                                     rd_lane(s_hx, sp), rd_lane(s_hy, sp), rd_lane(s_hz, sp), wrap, A.box);
+      if (!ballot(active && gap2 <= kth))
+        continue;
+      if (l == 0)
       {
-        if (l == top)
-        {
-          done = true;
-          break;
-        }
-        const int cl = A.cnt[l * A.ngroups + g];
-        if ((i % RS_FAN) != RS_FAN - 1 && i + 1 < cl)
-        {
-          i++;
-          break;
-        }
-        i /= RS_FAN;
-        l++;
+        if (i != h[0] && i != h[1] && i != h[2] && i != h[3])
+          leaf(i);
+        continue;
       }
-      if (done)
-        break;
+      const int nc = min(rd_lane(lcnt, l - 1) - RS_FAN * i, RS_FAN);
+      const int64_t cb = level_base(l - 1) + RS_FAN * i;
+      float4 cl = make_float4(0.f, 0.f, 0.f, 0.f), ch = cl;
+      if (lane < nc)
+      {
+        cl = A.nodes[2 * (cb + lane)];
+        ch = A.nodes[2 * (cb + lane) + 1];
+      }
+      for (int c = nc - 1; c >= 0; c--) // child 0 ends on top
+      {
+        const float lx = rd_lane(cl.x, c), ly = rd_lane(cl.y, c), lz = rd_lane(cl.z, c);
+        const float hx = rd_lane(ch.x, c), hy = rd_lane(ch.y, c), hz = rd_lane(ch.z, c);
+        const float cg = rs_box_gap2(ux, uy, uz, lx, ly, lz, hx, hy, hz, wrap, A.box);
+        if (ballot(active && cg <= kth))
+          push(((l - 1) << 24) | (RS_FAN * i + c), lx, ly, lz, hx, hy, hz);
+      }
     }
     if (!active)
       continue;
@@ -587,11 +730,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_knn(RsArgs A, const int4 *__r
     {
       je[j] = RS_RADII;
       se[j] = -1;
-      if (j >= K - k && id[j] >= 0)
+      if (j >= K - k && key[j] != ~0ull)
       {
-        const float4 r = A.trec[id[j]];
+        const int64_t p = (int32_t)(uint32_t)key[j];
         double dx, dy, dz;
-        const double r2 = rs_r2(r.x, r.y, r.z, qx, qy, qz, A, dx, dy, dz);
+        const double r2 = rs_r2(A.pos[3 * p], A.pos[3 * p + 1], A.pos[3 * p + 2], qx, qy, qz, A, dx, dy, dz);
         int a = 0, b = RS_RADII - 1; // smallest index that collects it (the last entry always does)
         while (a < b)
         {
@@ -602,7 +745,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_knn(RsArgs A, const int4 *__r
             a = m + 1;
         }
         je[j] = a;
-        se[j] = __float_as_int(r.w);
+        se[j] = A.sub[p];
         jmin = min(jmin, a);
       }
     }

@@ -21,6 +21,11 @@
 //   hbtx::CalculateProfileProperties(ctx, prm, subs, snapshot_id, shape)
 //                              Subhalo_t::CalculateProfileProperties / CalculateShape
 //                              src/subhalo.cpp:175-309
+//   hbtx::ReassignParticles(ctx, prm, Subhalos, SubGroups, num_neighbours)
+//                              SubhaloSnapshot_t::ReassignParticles src/subhalo_reassign_gas.cpp:65-197
+//   hbtx::KickNullParticles(sub, prm), hbtx::CountParticleTypes(sub, prm)
+//                              Subhalo_t::KickNullParticles / CountParticleTypes
+//                              src/subhalo.cpp:394-426, :311-392
 //
 // The templates accept any subhalo type with the reference's member names
 // (Particles with Id / ComovingPosition / PhysicalVelocity / Mass and, when
@@ -642,6 +647,160 @@ void RecursiveUnbind(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT>
     if (!again.empty())
       RefineParticles(ctx, prm, again, snapshot_id, flags, save_binding_energies, save_potential_energies);
   }
+}
+
+namespace detail
+{
+// Particle_t::IsTracer (src/snapshot.h:90-97)
+template <class ParticleT>
+bool IsTracer(const ParticleT &p, const hbtx_params &prm)
+{
+  if (prm.dm_only)
+    return true;
+  if constexpr (has_type<ParticleT>::value)
+    return ((1 << (int)p.Type) & prm.tracer_mask) != 0;
+  else
+    return true;
+}
+} // namespace detail
+
+// Subhalo_t::CountParticleTypes (src/subhalo.cpp:311-392): bound particle counts
+// and masses per type (float sums in particle order), and TracerIndex = the
+// first bound tracer, else 0.  Without particle types (DM_ONLY) TracerIndex = 0.
+template <class SubhaloT>
+void CountParticleTypes(SubhaloT &S, const hbtx_params &prm)
+{
+  using ParticleT = typename std::decay<decltype(S.Particles[0])>::type;
+  if (prm.dm_only || !detail::has_type<ParticleT>::value)
+  {
+    S.TracerIndex = 0;
+    return;
+  }
+  if constexpr (detail::has_NboundType<SubhaloT>::value)
+    for (auto &x : S.NboundType)
+      x = 0;
+  if constexpr (detail::has_MboundType<SubhaloT>::value)
+    for (auto &x : S.MboundType)
+      x = 0.f;
+  int64_t tracer = -1;
+  const int64_t nb = std::min<int64_t>((int64_t)S.Nbound, (int64_t)S.Particles.size());
+  for (int64_t i = 0; i < nb; i++)
+  {
+    const auto &p = S.Particles[i];
+    if constexpr (detail::has_type<ParticleT>::value)
+    {
+      if constexpr (detail::has_NboundType<SubhaloT>::value)
+        S.NboundType[(int)p.Type]++;
+      if constexpr (detail::has_MboundType<SubhaloT>::value)
+        S.MboundType[(int)p.Type] += p.Mass;
+    }
+    if (tracer < 0 && detail::IsTracer(p, prm))
+      tracer = i;
+  }
+  S.TracerIndex = (tracer < 0) ? 0 : tracer;
+}
+
+// Subhalo_t::KickNullParticles (src/subhalo.cpp:394-426): drop particles whose Id
+// is NullParticleId (-1), keeping the order and the bound/unbound split; returns
+// the number dropped.  A no-op in DM_ONLY builds.
+template <class SubhaloT>
+int64_t KickNullParticles(SubhaloT &S, const hbtx_params &prm)
+{
+  if (prm.dm_only)
+    return 0;
+  const int64_t np = (int64_t)S.Particles.size();
+  const int64_t nb = std::min<int64_t>((int64_t)S.Nbound, np);
+  int64_t keep = 0, keep_bound = 0;
+  for (int64_t i = 0; i < np; i++)
+  {
+    if (S.Particles[i].Id == -1)
+      continue;
+    if (i != keep)
+      S.Particles[keep] = std::move(S.Particles[i]);
+    keep++;
+    if (i < nb)
+      keep_bound = keep;
+  }
+  S.Nbound = keep_bound;
+  S.Particles.resize(keep);
+  CountParticleTypes(S, prm);
+  return np - keep;
+}
+
+// SubhaloSnapshot_t::ReassignParticles (src/subhalo_reassign_gas.cpp:65-197) on
+// one rank's host haloes: SubGroups[h] lists the subhaloes of host halo h
+// (MemberTable.SubGroups).  The GPU chooses each non-tracer particle's target
+// (hbtx_reassign_targets: the k = num_neighbours nearest tracers of the host
+// halo as GeoTree_t::NearestNeighbours finds them from 0.1 SofteningHalo); then,
+// as in the reference, a particle none of whose neighbours belongs to its own
+// subhalo is appended to the nearest tracer's subhalo and nulled, host halo by
+// host halo in SubGroups order, and every subhalo of a host halo that has
+// non-tracers gets Nbound clamped to its size and KickNullParticles.  Returns
+// the number of particles moved.
+template <class SubhaloT, class GroupList>
+int64_t ReassignParticles(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT> &Subhalos,
+                          const GroupList &SubGroups, int num_neighbours = 10, double *device_ms = nullptr)
+{
+  const int64_t ng = (int64_t)SubGroups.size();
+  if (ng == 0)
+    return 0;
+  std::vector<int64_t> goff(ng + 1, 0);
+  for (int64_t h = 0; h < ng; h++)
+  {
+    int64_t c = 0;
+    for (auto s : SubGroups[h])
+      c += (int64_t)Subhalos[s].Particles.size();
+    goff[h + 1] = goff[h] + c;
+  }
+  const int64_t n = goff[ng];
+  std::vector<float> pos(3 * std::max<int64_t>(n, 1));
+  std::vector<uint8_t> tracer(std::max<int64_t>(n, 1));
+  std::vector<int32_t> sub(std::max<int64_t>(n, 1)), target(std::max<int64_t>(n, 1));
+  std::vector<char> has_gas(ng, 0);
+  for (int64_t h = 0, i = 0; h < ng; h++)
+    for (auto s : SubGroups[h])
+      for (const auto &p : Subhalos[s].Particles)
+      {
+        detail::put3(&pos[3 * i], p.ComovingPosition);
+        tracer[i] = detail::IsTracer(p, prm) ? 1 : 0;
+        has_gas[h] |= !tracer[i];
+        sub[i] = (int32_t)s;
+        i++;
+      }
+  ctx.check(hbtx_reassign_targets(ctx.get(), &prm, n, pos.data(), tracer.data(), sub.data(), ng, goff.data(),
+                                  num_neighbours, 0.f, target.data(), device_ms));
+  int64_t moved = 0;
+  for (int64_t h = 0; h < ng; h++)
+  {
+    if (!has_gas[h])
+      continue; // no non-tracers: the reference skips the host halo
+    std::vector<int64_t> sublen;
+    for (auto s : SubGroups[h])
+      sublen.push_back((int64_t)Subhalos[s].Particles.size());
+    int64_t i = goff[h], k = 0;
+    for (auto s : SubGroups[h])
+    {
+      for (int64_t j = 0; j < sublen[k]; j++, i++)
+      {
+        const int32_t t = target[i];
+        if (t < 0)
+          continue;
+        auto &part = Subhalos[s].Particles[j];
+        Subhalos[t].Particles.push_back(part);
+        part.Id = -1;
+        moved++;
+      }
+      k++;
+    }
+    for (auto s : SubGroups[h])
+    {
+      auto &S = Subhalos[s];
+      if ((int64_t)S.Nbound > (int64_t)S.Particles.size())
+        S.Nbound = S.Particles.size();
+      KickNullParticles(S, prm);
+    }
+  }
+  return moved;
 }
 
 } // namespace hbtx

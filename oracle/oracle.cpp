@@ -1353,8 +1353,11 @@ extern "C" void oracle_profiles(const ounb_params *prm, int64_t n, const float *
 // GeoTree_t::NearestNeighbours / Search (src/geometric_tree.cpp:108-213) and
 // NumNearestNeighboursCollector_t (src/datatypes.h:287-324).  The tree search is
 // replaced by a scan over the group's tracers: it collects the same set (every
-// tracer with r^2 < radius^2); only the order of collection, and so the order
-// among tracers at exactly equal float distances, can differ.
+// tracer inside the per-axis and sphere tests).  The reference orders tracers at
+// exactly equal float distances by its tree's traversal order and the heap's
+// internals (and places tracers closer than TreeNodeResolution at random), so
+// that order is not reproducible; here the heap orders by (distance, particle
+// index), the rule the GPU implements too.  Parity unpinned for such ties.
 // ---------------------------------------------------------------------------
 namespace
 {
@@ -1367,7 +1370,7 @@ struct LocatedLess
 {
   bool operator()(const Located &a, const Located &b) const
   {
-    return a.d2 < b.d2;
+    return a.d2 < b.d2 || (a.d2 == b.d2 && a.index < b.index);
   }
 };
 } // namespace
@@ -1418,7 +1421,7 @@ extern "C" void oracle_reassign_targets(const ounb_params *prm, int64_t n, const
             const Located lp{t, (float)r2};
             if ((int64_t)heap.size() < k)
               heap.push(lp);
-            else if (lp.d2 < heap.top().d2)
+            else if (LocatedLess()(lp, heap.top()))
             {
               heap.push(lp);
               heap.pop();

@@ -29,6 +29,16 @@ def test_host_haloes_match_oracle(ctx, seed, k):
     assert np.all(g[h.is_tracer == 1] == -1)
 
 
+def test_far_from_origin_many_ties(ctx):
+    """Host haloes near 6000: float32 positions are spaced 4.9e-4 apart there, so
+    one query in ten has two of its nearest tracers at exactly equal float
+    distances, and the (distance, particle index) order decides."""
+    h = synth.host_haloes(seed=9, ngroups=4, n_sub=(30, 2000))
+    pos = (h.pos.astype(np.float64) + 6000.0).astype(np.float32)
+    g, o = both(ctx, pos, h.is_tracer, h.sub, h.group_offsets)
+    np.testing.assert_array_equal(g, o)
+
+
 def test_periodic_host_haloes_match_oracle(ctx):
     h = synth.host_haloes(seed=11, ngroups=8, n_sub=(30, 1500), box=2.0, spread=0.08)
     g, o = both(ctx, h.pos, h.is_tracer, h.sub, h.group_offsets, periodic=1, box_size=2.0)
@@ -109,3 +119,133 @@ def test_invalid_arguments(ctx):
         ctx.reassign_targets(h.pos, h.is_tracer, h.sub, h.group_offsets, k=33)
     with pytest.raises(hbtx.HbtxError):
         ctx.reassign_targets(h.pos, h.is_tracer, h.sub, [0, h.n - 1])
+
+
+
+
+def _bin():
+    import os
+    import hbtx
+    return os.path.join(os.path.dirname(hbtx.LIB_PATH), "reassign_test")
+
+
+def _reference_moves(ids, types, masses, nbound, members, glen, targets, tracer_types=(1, 4)):
+    """The reference's bookkeeping around the targets (src/subhalo_reassign_gas.cpp:
+    131-189, KickNullParticles and CountParticleTypes, src/subhalo.cpp:311-426),
+    restated over per-subhalo lists."""
+    subs = [list(zip(i, t, m)) for i, t, m in zip(ids, types, masses)]
+    nb = list(nbound)
+    tidy = np.zeros(len(subs), bool)  # KickNullParticles ran (else the fields keep their values)
+    moved, i, m = 0, 0, 0
+    for h in range(len(glen)):
+        mem = members[m:m + glen[h]]
+        m += glen[h]
+        gas = any(t not in tracer_types for s in mem for (_, t, _) in subs[s])
+        sublen = [len(subs[s]) for s in mem]
+        if not gas:
+            i += sum(sublen)
+            continue
+        for s, L in zip(mem, sublen):
+            for j in range(L):
+                t = targets[i]
+                i += 1
+                if t >= 0:
+                    subs[t].append(subs[s][j])
+                    subs[s][j] = (-1,) + subs[s][j][1:]
+                    moved += 1
+        for s in mem:
+            tidy[s] = True
+            nb[s] = min(nb[s], len(subs[s]))
+            bound = [p for p in subs[s][:nb[s]] if p[0] != -1]
+            rest = [p for p in subs[s][nb[s]:] if p[0] != -1]
+            subs[s] = bound + rest
+            nb[s] = len(bound)
+    out = []
+    for s, ps in enumerate(subs):
+        nt = np.zeros(6, np.int64)
+        mt = np.zeros(6, np.float32)
+        tix = -1
+        for j, (_, t, mm) in enumerate(ps[:nb[s]] if tidy[s] else []):
+            nt[t] += 1
+            mt[t] = np.float32(mt[t] + np.float32(mm))
+            if tix < 0 and t in tracer_types:
+                tix = j
+        out.append((nb[s], max(tix, 0), nt, mt, [p[0] for p in ps]))
+    return moved, out
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_cpp_reassign_particles(ctx, tmp_path, periodic):
+    """hbtx::ReassignParticles on Particle_t / Subhalo_t-like types against the
+    oracle's targets applied by the reference's rules."""
+    import subprocess
+    box = 2.0 if periodic else 100.0
+    h = synth.host_haloes(seed=31 + periodic, ngroups=6, n_sub=(20, 1500), box=box if periodic else None)
+    rng = np.random.default_rng(40)
+    n = h.n
+    types = np.where(h.is_tracer == 1, np.where(rng.random(n) < 0.8, 1, 4), 0).astype(np.int32)
+    # one host halo of tracers only (the reference skips it)
+    g_last = slice(h.group_offsets[-2], h.group_offsets[-1])
+    types[g_last] = 1
+    masses = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    ids = rng.permutation(n).astype(np.int64) + 1000
+    nsub = len(h.sub_offsets) - 1
+    so = h.sub_offsets
+    nbound = np.array([rng.integers(0, so[s + 1] - so[s] + 1) for s in range(nsub)], np.int64)
+    # subhaloes in host-halo order, in a shuffled order inside every host halo
+    members, glen = [], []
+    sub_of_group = np.searchsorted(h.group_offsets, so[:-1], side="right") - 1
+    for g in range(len(h.group_offsets) - 1):
+        mem = list(rng.permutation(np.nonzero(sub_of_group == g)[0]))
+        members += mem
+        glen.append(len(mem))
+    soft = 0.002
+    inp = tmp_path / "in.bin"
+    with open(inp, "wb") as f:
+        np.array([nsub, len(glen), 10], np.int64).tofile(f)
+        np.array([periodic], np.int32).tofile(f)
+        np.array([box, soft], np.float32).tofile(f)
+        nbound.tofile(f)
+        np.diff(so).astype(np.int64).tofile(f)
+        np.asarray(glen, np.int64).tofile(f)
+        np.asarray(members, np.int64).tofile(f)
+        rec = np.zeros(n, [("id", np.int64), ("pos", np.float32, 3), ("mass", np.float32), ("type", np.int32)])
+        rec["id"], rec["pos"], rec["mass"], rec["type"] = ids, h.pos, masses, types
+        rec.tofile(f)
+    outp = tmp_path / "out.bin"
+    print(f"reassign_test periodic={periodic} n={n}", flush=True)
+    r = subprocess.run([_bin(), str(inp), str(outp)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    # expected: the oracle's targets on the particles as the host layer orders them
+    order = np.concatenate([np.arange(so[s], so[s + 1]) for s in members])
+    goff = np.concatenate([[0], np.cumsum([sum(so[s + 1] - so[s] for s in members[a:a + L])
+                                           for a, L in zip(np.cumsum([0] + glen[:-1]), glen)])]).astype(np.int64)
+    tr = np.isin(types, (1, 4)).astype(np.uint8)
+    sub_pos = np.repeat(np.arange(nsub), np.diff(so))
+    prm = dict(softening=soft, periodic=periodic, box_size=box)
+    targets = O.reassign_targets(h.pos[order], tr[order], sub_pos[order].astype(np.int32), goff, k=10,
+                                 params=O.default_params(**prm))
+    per = lambda a: [a[so[s]:so[s + 1]] for s in range(nsub)]
+    moved, exp = _reference_moves(per(ids), per(types), per(masses), nbound, members, glen, targets)
+    assert moved > 50
+    got = np.fromfile(outp, np.uint8)
+    off = 0
+
+    def take(dt, cnt):
+        nonlocal off
+        a = np.frombuffer(got[off:off + np.dtype(dt).itemsize * cnt].tobytes(), dt)
+        off += np.dtype(dt).itemsize * cnt
+        return a
+    assert take(np.int64, 1)[0] == moved
+    for s in range(nsub):
+        nb_, ti = take(np.int64, 2)
+        nt = take(np.int64, 6)
+        mt = take(np.float32, 6)
+        npart = take(np.int64, 1)[0]
+        pid = take(np.int64, npart)
+        e = exp[s]
+        assert nb_ == e[0] and ti == e[1], s
+        np.testing.assert_array_equal(nt, e[2])
+        np.testing.assert_array_equal(mt, e[3])
+        np.testing.assert_array_equal(pid, e[4])
+    assert off == len(got)

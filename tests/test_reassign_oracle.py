@@ -9,12 +9,11 @@ import numpy as np
 import pyoracle as O
 
 
-def run(pts, tracer, sub, groups=None, **kw):
+def run(pts, tracer, sub, groups=None, k=10, **kw):
     pts = np.asarray(pts, np.float32)
     n = len(pts)
     go = [0, n] if groups is None else groups
-    return O.reassign_targets(pts, np.asarray(tracer), np.asarray(sub), go, params=O.default_params(**kw),
-                              rguess=kw.pop("rguess", None) if "rguess" in kw else None)
+    return O.reassign_targets(pts, np.asarray(tracer), np.asarray(sub), go, k=k, params=O.default_params(**kw))
 
 
 def cloud(rng, centre, n, r):
@@ -116,3 +115,19 @@ def test_single_search_equals_brute_force_knn():
         moved += want >= 0
     assert moved > 10  # the case exercises moves
     assert np.all(out[tracer == 1] == -1)
+
+
+def test_equal_distances_break_by_particle_index():
+    """Tracers at exactly the same float distance are ordered by particle index
+    (the reference's order among them follows its tree traversal and heap
+    internals and is not reproducible; parity is unpinned for such ties)."""
+    q = [5.0, 5.0, 5.0]
+    a, b = [5.25, 5.0, 5.0], [4.75, 5.0, 5.0]  # mirror images: identical squared distances
+    own = [5.0, 5.0, 7.0]
+    # nearest pair: subhalo 2 (index 1) and subhalo 1 (index 2); the own tracer is far
+    pts = [q, a, b, own]
+    tracer = [0, 1, 1, 1]
+    assert run(pts, tracer, [0, 2, 1, 0], k=1)[0] == 2
+    assert run(pts, tracer, [0, 1, 2, 0], k=1)[0] == 1
+    # a tie at the k-th slot: with k=2 both tied tracers fit, own excluded -> moves to index 1's subhalo
+    assert run(pts, tracer, [0, 2, 1, 0], k=2)[0] == 2
