@@ -21,6 +21,9 @@
 //   hbtx::CalculateProfileProperties(ctx, prm, subs, snapshot_id, shape)
 //                              Subhalo_t::CalculateProfileProperties / CalculateShape
 //                              src/subhalo.cpp:175-309
+//   hbtx::RefineSnapshot(ctx, prm, Subhalos, SubGroups, SubGroupsOfHeads, field, NumSubOld, ...)
+//                              SubhaloSnapshot_t::RefineParticles src/subhalo_unbind.cpp:855-919
+//                              (exclusive mass: hierarchies, field and new-born subhaloes, truncation)
 //   hbtx::ReassignParticles(ctx, prm, Subhalos, SubGroups, num_neighbours)
 //                              SubhaloSnapshot_t::ReassignParticles src/subhalo_reassign_gas.cpp:65-197
 //   hbtx::KickNullParticles(sub, prm), hbtx::CountParticleTypes(sub, prm)
@@ -39,6 +42,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -647,6 +651,69 @@ void RecursiveUnbind(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT>
     if (!again.empty())
       RefineParticles(ctx, prm, again, snapshot_id, flags, save_binding_energies, save_potential_energies);
   }
+}
+
+// SubhaloSnapshot_t::RefineParticles (src/subhalo_unbind.cpp:855-919, the
+// exclusive-mass build).  For every host halo h with subhaloes, the central
+// SubGroups[h][0] gets the host halo's other heads (SubGroupsOfHeads[h][1..])
+// appended to its NestedSubhalos for the duration of its RecursiveUnbind;
+// then the field subhaloes (MemberTable.SubGroups[-1]) and the subhaloes born
+// this snapshot (index >= NumSubOld) are unbound on their own, and every
+// subhalo's source is truncated (TruncateSource).  The hierarchies of all host
+// haloes go level by level in shared batches (RecursiveUnbind), the field and
+// new-born subhaloes in one more batch.  Returns PrintSubhaloStatistics' counts;
+// *elapsed_s receives the wall time (PrintTimeImbalanceStatistics' figure on
+// one rank).
+template <class SubhaloT, class GroupList>
+SubhaloCounts RefineSnapshot(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT> &Subhalos,
+                             const GroupList &SubGroups, const GroupList &SubGroupsOfHeads,
+                             const std::vector<int64_t> &FieldSubhalos, int64_t NumSubOld, int snapshot_id,
+                             bool merge_trapped, double source_sub_relax_factor, double *elapsed_s = nullptr,
+                             uint32_t flags = HBTX_BATCHED, bool save_binding_energies = false,
+                             bool save_potential_energies = false)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  if (SubGroupsOfHeads.size() != SubGroups.size())
+    throw Error(HBTX_E_INVALID, "RefineSnapshot: SubGroupsOfHeads must have one entry per host halo");
+  std::vector<int64_t> roots;
+  std::vector<std::pair<int64_t, size_t>> restore;
+  for (size_t h = 0; h < SubGroups.size(); h++)
+  {
+    if (SubGroups[h].size() == 0)
+      continue;
+    const int64_t c = (int64_t)SubGroups[h][0];
+    auto &nests = Subhalos[c].NestedSubhalos;
+    restore.push_back({c, nests.size()});
+    const auto &heads = SubGroupsOfHeads[h];
+    for (size_t j = 1; j < heads.size(); j++)
+      nests.push_back(heads[j]);
+    roots.push_back(c);
+  }
+  try
+  {
+    RecursiveUnbind(ctx, prm, Subhalos, roots, snapshot_id, merge_trapped, flags, save_binding_energies,
+                    save_potential_energies);
+  }
+  catch (...)
+  {
+    for (auto &r : restore)
+      Subhalos[r.first].NestedSubhalos.resize(r.second);
+    throw;
+  }
+  for (auto &r : restore)
+    Subhalos[r.first].NestedSubhalos.resize(r.second);
+  std::vector<SubhaloT *> rest;
+  for (int64_t s : FieldSubhalos)
+    rest.push_back(&Subhalos[s]);
+  for (int64_t s = NumSubOld; s < (int64_t)Subhalos.size(); s++)
+    rest.push_back(&Subhalos[s]);
+  if (!rest.empty())
+    RefineParticles(ctx, prm, rest, snapshot_id, flags, save_binding_energies, save_potential_energies);
+  for (auto &S : Subhalos)
+    TruncateSource(S, source_sub_relax_factor);
+  if (elapsed_s)
+    *elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return SubhaloStatistics(Subhalos, snapshot_id);
 }
 
 namespace detail

@@ -16,7 +16,9 @@
 //   output: per subhalo int64 nbound, sink, nparticles, mostbound_id;
 //           int32 snapshot_of_sink, snapshot_of_death; float core[8];
 //           then every subhalo's particle ids in order
-//   stdout: "levels L merged M identical 1 truncate_ok 1"
+//   stdout: "levels L merged M identical 1 truncate_ok 1 ... snapshot_identical 1"
+//   (the last: hbtx::RefineSnapshot against SubhaloSnapshot_t::RefineParticles'
+//   control flow restated with the same per-subhalo recursion)
 #include "hbtx_compat.hpp"
 
 #include <array>
@@ -268,9 +270,56 @@ int main(int argc, char **argv)
     }
     const bool stats_ok = sc.sunk == sunk && sc.disrupted == dis &&
                           sc.newborn + sc.fake == 1 && (sc.fake == 1) == (A[0].SnapshotOfDeath == g_snap);
-    printf("levels %d merged %d identical %d truncate_ok %d stats_ok %d sunk %ld\n", levels, g_merged, ok ? 1 : 0,
-           trunc_ok ? 1 : 0, stats_ok ? 1 : 0, (long)sc.sunk);
-    return ok ? 0 : 1;
+    // SubhaloSnapshot_t::RefineParticles (src/subhalo_unbind.cpp:855-919): the roots
+    // paired into host haloes (first of a pair the central, both heads); an odd last
+    // root and its hierarchy are field subhaloes, each unbound on its own
+    std::vector<std::vector<int64_t>> groups, heads;
+    std::vector<int64_t> field;
+    for (int64_t r = 0; r + 1 < nroots; r += 2)
+    {
+      groups.push_back({roots[r], roots[r + 1]});
+      heads.push_back({roots[r], roots[r + 1]});
+    }
+    if (nroots % 2)
+    {
+      std::vector<int64_t> st{roots[nroots - 1]};
+      while (!st.empty())
+      {
+        const int64_t x = st.back();
+        st.pop_back();
+        field.push_back(x);
+        for (size_t j = 0; j < child.size() && j < (size_t)(coff[x + 1] - coff[x]); j++)
+          st.push_back(child[coff[x] + j]);
+      }
+    }
+    std::vector<Subhalo_t> C, D;
+    make(C);
+    make(D);
+    double secs = 0.0;
+    hbtx::RefineSnapshot(ctx, g_prm, C, groups, heads, field, nsub, g_snap, g_merge, 3.0, &secs);
+    for (size_t h = 0; h < groups.size(); h++)
+    {
+      auto &nests = D[groups[h][0]].NestedSubhalos;
+      const size_t old = nests.size();
+      nests.insert(nests.end(), heads[h].begin() + 1, heads[h].end());
+      recursive_unbind(ctx, D, groups[h][0]);
+      nests.resize(old);
+    }
+    for (int64_t x : field)
+      hbtx::Unbind(ctx, g_prm, D[x], g_snap);
+    for (auto &S : D)
+    {
+      const size_t keep = (S.Nbound <= 1) ? (size_t)S.Nbound : std::min((size_t)(S.Nbound * 3), S.Particles.size());
+      S.Particles.resize(keep);
+    }
+    bool snap_ok = true;
+    for (int64_t s = 0; s < nsub; s++)
+      snap_ok = same(C[s], D[s], s) && snap_ok;
+    printf("levels %d merged %d identical %d truncate_ok %d stats_ok %d sunk %ld snapshot_identical %d "
+           "snapshot_groups %zu field %zu\n",
+           levels, g_merged, ok ? 1 : 0, trunc_ok ? 1 : 0, stats_ok ? 1 : 0, (long)sc.sunk, snap_ok ? 1 : 0,
+           groups.size(), field.size());
+    return (ok && snap_ok) ? 0 : 1;
   }
   catch (const hbtx::Error &e)
   {

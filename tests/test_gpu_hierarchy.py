@@ -56,6 +56,9 @@ def test_recursive_unbind_matches_oracle_recursion(tmp_path, merge_trapped):
     info, gpu, lists = run_driver(H, tmp_path, merge_trapped)
     # the level-batched result equals the depth-first recursion, and the case is not trivial
     assert info["identical"] == 1 and info["truncate_ok"] == 1 and info["stats_ok"] == 1
+    # the snapshot-level driver (RefineSnapshot: heads appended to the central, field
+    # subhaloes on their own, truncation) equals SubhaloSnapshot_t::RefineParticles' flow
+    assert info["snapshot_identical"] == 1 and info["snapshot_groups"] >= 1 and info["field"] >= 1
     assert info["sunk"] >= 2
     assert info["levels"] >= 4 and info["merged"] >= 2
     ref = HO.run(H, O.default_params(), 7, merge_trapped)
