@@ -149,6 +149,7 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
   for (int l = 0; l < cprev + 1; l++)
     L = (L >= A.P.res) ? L * 0.5 : L;
   const uint64_t kj = A.keys[j];
+  int hi_l = e; // cells nest: a deeper cell ends no later than the one above it
   for (int l = cprev + 1; l <= cj; l++)
   {
     const int pos = P + (l - cprev - 1);
@@ -157,8 +158,19 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
       bend = e - 1;
     else
     {
+      // the cell's last source: gallop from j (most cells are small), then bisect
       const uint64_t pj = key_prefix(kj, l);
-      int lo = j + 1, hi = e;
+      int lo = j + 1, hi = hi_l;
+      for (int span = 1; lo < hi; span <<= 1)
+      {
+        const int probe = min(j + span, hi - 1);
+        if (key_prefix(A.keys[probe], l) > pj)
+        {
+          hi = probe;
+          break;
+        }
+        lo = probe + 1;
+      }
       while (lo < hi)
       {
         const int mid = (lo + hi) >> 1;
@@ -168,6 +180,7 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
           lo = mid + 1;
       }
       bend = lo - 1;
+      hi_l = bend + 1;
     }
     const int skip = (bend + 1) + A.cscan[bend + 1];
     const float lf = (float)L;
