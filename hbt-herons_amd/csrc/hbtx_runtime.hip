@@ -307,13 +307,21 @@ void scan64(hbtx_ctx *c, const int64_t *in, int64_t *out, int64_t n)
   scan_excl<int64_t>(c, in, out, n, c->part64);
 }
 
+// Radix tiles of a segmented sort over n elements in nseg segments: a segment
+// longer than SORT_SMALL_MAX takes ceil(len / SORT_TILE) <= len / SORT_TILE + 1
+// tiles, and at most n / (SORT_SMALL_MAX + 1) segments are that long.
+int64_t sort_tile_bound(int64_t n, int64_t nseg)
+{
+  return n / SORT_TILE + std::min<int64_t>(nseg, n / (SORT_SMALL_MAX + 1)) + 2;
+}
+
 void reserve_sort(hbtx_ctx *c, int64_t n, int32_t nseg)
 {
   c->kalt.alloc(n);
   c->valt.alloc(n);
   c->sort_ntile.alloc(nseg + 1);
   c->sort_toff.alloc(nseg + 1);
-  const int64_t maxt = n / (SORT_SMALL_MAX) + 2;
+  const int64_t maxt = sort_tile_bound(n, nseg);
   c->sort_tiles.alloc(maxt);
   c->ghist.alloc(maxt * RADIX_MAX_BINS);
 }
@@ -337,7 +345,7 @@ void seg_sort(hbtx_ctx *c, uint64_t *keys, int32_t *vals, int32_t nseg, const in
   k_sort_tiles<<<blocks_for(nseg, 256, 1 << 20), 256, 0, c->stream>>>(seg_begin, seg_len, c->sort_toff.p, nseg,
                                                                        c->sort_tiles.p);
   const int32_t *ntiles_p = c->sort_toff.p + nseg;
-  const int64_t maxt = n_cap / SORT_SMALL_MAX + 2;
+  const int64_t maxt = sort_tile_bound(n_cap, nseg);
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxt, 4 * c->num_cu));
   // an even number of passes leaves the result in place; 9-bit digits where they save passes
   auto even_passes = [&](int rb) {

@@ -58,3 +58,42 @@ def test_radix_sort_is_stable_for_large_segments(ctx):
     k2, v2 = ctx.segmented_sort(keys, vals, np.array([0, n], np.int64))
     order = np.argsort(keys, kind="stable")
     assert np.array_equal(v2, vals[order])
+
+
+def test_many_segments_just_over_the_lds_size_fresh_context():
+    """Regression: every segment longer than the LDS sort takes a partial radix tile,
+    so segments just over 2048 elements need almost two tiles per 2048 elements; the
+    tile list and its histograms are sized for that (a fresh context, so no earlier
+    call has grown the buffers)."""
+    import hbtx
+    c = hbtx.Context(0)
+    try:
+        rng = np.random.default_rng(21)
+        sizes = rng.integers(2049, 2300, 1000)
+        off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        n = int(off[-1])
+        print(f"segmented sort: {len(sizes)} segments, {n} elements", flush=True)
+        keys = rng.integers(0, 2 ** 63, n, dtype=np.uint64)
+        vals = np.arange(n, dtype=np.int32)
+        k2, v2 = c.segmented_sort(keys, vals, off)
+        _check_sorted(keys, vals, k2, v2, off)
+    finally:
+        c.close()
+
+
+def test_direct_sums_of_subhaloes_just_over_the_lds_size_fresh_context():
+    """Regression: 300 subhaloes of 2048 members plus interlopers (about 2250
+    sources each) in direct-sum mode; batched and per-subhalo runs agree."""
+    import hbtx
+    import synth
+    c = hbtx.Context(0)
+    try:
+        pop = synth.population(seed=33, nsub=300, nmin=2048, nmax=2048)
+        print(f"direct-sum population: {pop.nsub} subhaloes, {pop.n} particles", flush=True)
+        prm = hbtx.workload_params(direct_cutoff=1 << 40)
+        a = c.unbind(pop, prm, hbtx.BATCHED)
+        b = c.unbind(pop, prm, hbtx.PLAIN)
+        assert np.array_equal(a["result"]["nbound"], b["result"]["nbound"])
+        assert np.array_equal(a["order"], b["order"])
+    finally:
+        c.close()
