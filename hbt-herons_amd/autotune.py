@@ -4,8 +4,10 @@ Picks, by timing complete unbindings:
   * direct_cutoff: the source count at or below which a subhalo's potentials
     come from the exact O(N^2) sum rather than the tree walk.  Populations of
     equal-size subhaloes are unbound with every subhalo on one path, then on the
-    other; the cutoff is the geometric midpoint between the largest size where
-    the direct sum wins and the next size;
+    other; the crossover is the geometric midpoint between the largest size where
+    the direct sum wins and the next size.  The cutoff is then the fastest of
+    the crossover and its neighbours at factors of sqrt(2) on a power-law
+    population, where both paths share the GPU;
   * walk_chunks: how many contiguous chunks the walk's item list is cut into
     for its XCD-aware work distribution, timed on a power-law population.
 
@@ -60,6 +62,19 @@ def tune_cutoff(ctx, particles, log):
     return cutoff, rows
 
 
+def refine_cutoff(ctx, crossover, particles, log):
+    """The crossover and its neighbours timed on a power-law population."""
+    pop = synth.population(seed=7, target_particles=int(particles))
+    ctx.load(pop)
+    cands = sorted({max(0, int(round(crossover * 2 ** (k / 2)))) for k in (-2, -1, 0, 1, 2)})
+    rows = []
+    for cut in cands:
+        ms = best_ms(ctx, hbtx.workload_params(direct_cutoff=cut))
+        rows.append(dict(direct_cutoff=cut, ms=ms))
+        log(f"  power-law population, cutoff {cut:6d}: {ms:8.2f} ms")
+    return min(rows, key=lambda r: r["ms"])["direct_cutoff"], rows
+
+
 def tune_chunks(ctx, cutoff, particles, log):
     pop = synth.population(seed=7, target_particles=int(particles))
     ctx.load(pop)
@@ -86,13 +101,15 @@ def main():
     ctx = hbtx.Context(0)
     t0 = time.time()
     log("direct-sum / tree cutoff:")
-    cutoff, rows_c = tune_cutoff(ctx, args.particles, log)
+    crossover, rows_c = tune_cutoff(ctx, args.particles, log)
+    log(f"-> crossover {crossover}")
+    cutoff, rows_m = refine_cutoff(ctx, crossover, args.particles, log)
     log(f"-> direct_cutoff {cutoff}")
     log("walk item chunks:")
     chunks, rows_w = tune_chunks(ctx, cutoff, args.particles, log)
     log(f"-> walk_chunks {chunks}")
     out = dict(direct_cutoff=cutoff, walk_chunks=chunks, seconds=round(time.time() - t0, 1),
-               cutoff_timings=rows_c, chunk_timings=rows_w)
+               crossover=crossover, cutoff_timings=rows_c, mixture_timings=rows_m, chunk_timings=rows_w)
     with open(args.out, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(dict(direct_cutoff=cutoff, walk_chunks=chunks, file=args.out)))
