@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_init_mass_tiles(SubArrays S, PartArrays
     }
     __syncthreads();
     atomicMin(&lmin, mymin);
-    block_reduce_vec(v, lds);
+    block_reduce_vec<9>(v, lds, wave_in_tile(tl.y, tl.z)); // v[0..8]
     if (threadIdx.x == 0)
     {
       for (int k = 0; k < RED_NV; k++)
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_upscale_tiles(SubArrays S, PartArrays Q
           v[7 + k] += (ty == k) ? m : 0.;
       }
     }
-    block_reduce_vec(v, lds);
+    block_reduce_vec<13>(v, lds, wave_in_tile(tl.y, tl.z)); // v[0..12]
     if (threadIdx.x == 0)
       for (int k = 0; k < RED_NV; k++)
         partial[(int64_t)t * RED_NV + k] = v[k];

@@ -700,26 +700,33 @@ __global__ void k_warm_segments(SubArrays S, int32_t *seg_begin, int32_t *seg_le
 #define RED_TILE 4096
 #define RED_NV 20
 
-HBTX_DEV void block_reduce_vec(double v[RED_NV], double *lds /* [RED_THREADS/64][RED_NV] */)
+// Sum the first NV of the RED_NV values over a 256-thread block (thread 0 gets
+// the totals, in wave order).  A wave whose lanes saw no element of the tile
+// holds zeros: it skips the shuffles, since its sum is zero.
+template <int NV>
+HBTX_DEV void block_reduce_vec(double v[RED_NV], double *lds /* [RED_THREADS/64][RED_NV] */, bool wave_has_data)
 {
-#pragma unroll
-  for (int k = 0; k < RED_NV; k++)
+  if (wave_has_data)
   {
-    double x = v[k];
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1)
-      x += __shfl_down(x, d, HBTX_WAVE);
-    v[k] = x;
+    for (int k = 0; k < NV; k++)
+    {
+      double x = v[k];
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1)
+        x += __shfl_down(x, d, HBTX_WAVE);
+      v[k] = x;
+    }
   }
   const int w = threadIdx.x / HBTX_WAVE;
   if (lane_id() == 0)
 #pragma unroll
-    for (int k = 0; k < RED_NV; k++)
+    for (int k = 0; k < NV; k++)
       lds[w * RED_NV + k] = v[k];
   __syncthreads();
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < RED_NV; k++)
+    for (int k = 0; k < NV; k++)
     {
       double x = 0.;
       for (int ww = 0; ww < RED_THREADS / HBTX_WAVE; ww++)
@@ -727,6 +734,12 @@ HBTX_DEV void block_reduce_vec(double v[RED_NV], double *lds /* [RED_THREADS/64]
       v[k] = x;
     }
   __syncthreads();
+}
+
+// does the wave of this thread hold elements of the tile [b, e) (strided by the block)?
+HBTX_DEV bool wave_in_tile(int b, int e)
+{
+  return b + (int)(threadIdx.x & ~(HBTX_WAVE - 1)) < e;
 }
 
 __global__ void k_red_ntiles(const int32_t *len, int32_t nsub, int32_t *ntile)
@@ -752,6 +765,7 @@ __global__ void k_red_tiles(const int32_t *begin, const int32_t *len, const int3
 // v[0]=M, v[1..3]=sum m x (float products, or double NEAREST offsets when periodic), v[4..6]=sum m v
 struct ComOp
 {
+  static constexpr int NV = 7; // mass, mass-weighted position, mass-weighted velocity
   HBTX_DEV static void acc(const SubArrays &S, const PartArrays &Q, const DevParams &P, int s, int q, double v[RED_NV])
   {
     const int p = q;
@@ -795,7 +809,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_red_tile(SubArrays S, PartArray
       v[k] = 0.;
     for (int q = tl.y + threadIdx.x; q < tl.z; q += RED_THREADS)
       Op::acc(S, Q, P, tl.x, q, v);
-    block_reduce_vec(v, lds);
+    block_reduce_vec<Op::NV>(v, lds, wave_in_tile(tl.y, tl.z));
     if (threadIdx.x == 0)
       for (int k = 0; k < RED_NV; k++)
         partial[(int64_t)t * RED_NV + k] = v[k];
@@ -972,7 +986,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_final_tile(SubArrays S, PartArr
         v[13 + k] += (ty == k) ? 1.0 : 0.;
       }
     }
-    block_reduce_vec(v, lds);
+    block_reduce_vec<19>(v, lds, wave_in_tile(tl.y, tl.z)); // v[0..18]
     if (threadIdx.x == 0)
       for (int k = 0; k < RED_NV; k++)
         partial[(int64_t)t * RED_NV + k] = v[k];
