@@ -506,18 +506,23 @@ __global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, const 
     for (int q = tl.y + threadIdx.x; q < tl.z; q += blockDim.x)
     {
       int64_t f = 0;
+      bool outside = false; // bound, but not in the proposal
       if (q < end)
       {
         const int p = Q.cur[q];
         if (Q.energy[p] < 0.f)
         {
           if (warm && !Q.proposal[p])
-            atomicAdd(warm_np + s, 1);
+            outside = true;
           else
             f = 1 | ((int64_t)(is_tracer(P, ptype_of(Q, p)) ? 1 : 0) << 32);
         }
       }
       flag[q] = f;
+      // the tile is one subhalo's: one counter update per wave
+      const uint64_t m = ballot(outside);
+      if (outside && (m & lanemask_lt()) == 0ull)
+        atomicAdd(warm_np + s, (int32_t)__popcll(m));
     }
   }
 }
