@@ -333,6 +333,8 @@ void seg_sort(hbtx_ctx *c, uint64_t *keys, int32_t *vals, int32_t nseg, const in
               const int32_t *seg_len, int64_t n_cap, int key_bits, bool small_only = false)
 {
   reserve_sort(c, n_cap, nseg);
+  k_sort_wave<<<std::max(1, std::min((nseg + 3) / 4, 16384)), SORT_THREADS, 0, c->stream>>>(keys, vals, seg_begin,
+                                                                                            seg_len, nseg);
   k_sort_small<<<std::max(1, std::min(nseg, 16384)), SORT_THREADS, 0, c->stream>>>(keys, vals, seg_begin, seg_len,
                                                                                     nseg);
   if (small_only)

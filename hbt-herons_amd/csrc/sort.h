@@ -1,5 +1,7 @@
 // Segmented sort of (uint64 key, int32 value) pairs.
-//  * segments of <= SORT_SMALL_MAX elements: one workgroup each, bitonic sort in LDS;
+//  * segments of <= SORT_WAVE_MAX elements: one wave each, bitonic sort in lanes;
+//  * segments of <= SORT_SMALL_MAX elements: one workgroup each, the same bitonic
+//    network in LDS;
 //  * larger segments: tiled LSD radix sort (8- or 9-bit digits, whichever needs
 //    fewer passes) where every tile of SORT_TILE elements ranks its keys stably
 //    with wave ballots (match of the digit bits) and LDS per-digit counters, so
@@ -14,6 +16,7 @@ namespace hbtx
 #define SORT_THREADS 256
 #endif
 #define SORT_SMALL_MAX 2048
+#define SORT_WAVE_MAX HBTX_WAVE // segments of at most one wave: k_sort_wave
 #ifndef SORT_TILE
 #define SORT_TILE 2048
 #endif
@@ -29,8 +32,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_small(uint64_t *__restric
   for (int s = blockIdx.x; s < nseg; s += gridDim.x)
   {
     const int len = seg_len[s];
-    if (len <= 1 || len > SORT_SMALL_MAX)
-      continue;
+    if (len <= SORT_WAVE_MAX || len > SORT_SMALL_MAX)
+      continue; // (one-wave segments: k_sort_wave)
     const int b = seg_begin[s];
     int P = 1;
     while (P < len)
@@ -69,6 +72,49 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_small(uint64_t *__restric
       vals[b + i] = sv[i];
     }
     __syncthreads();
+  }
+}
+
+// Segments of at most one wave (the common case for small subhaloes): one wave per
+// segment, element i in lane i, the same bitonic network as k_sort_small (so the
+// same permutation, ties included) through lane shuffles, with no barrier.
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_wave(uint64_t *__restrict__ keys, int32_t *__restrict__ vals,
+                                                           const int32_t *__restrict__ seg_begin,
+                                                           const int32_t *__restrict__ seg_len, int32_t nseg)
+{
+  const int lane = lane_id();
+  const int wpb = SORT_THREADS / HBTX_WAVE;
+  for (int s = uniform(blockIdx.x * wpb + (int)(threadIdx.x / HBTX_WAVE)); s < nseg; s += gridDim.x * wpb)
+  {
+    const int len = seg_len[s];
+    if (len <= 1 || len > SORT_WAVE_MAX)
+      continue;
+    const int b = seg_begin[s];
+    int P = 1;
+    while (P < len)
+      P <<= 1;
+    uint64_t key = (lane < len) ? keys[b + lane] : ~0ull;
+    int32_t val = (lane < len) ? vals[b + lane] : 0;
+    for (int k = 2; k <= P; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1)
+      {
+        const uint64_t ok = __shfl_xor(key, j, HBTX_WAVE);
+        const int32_t ov = __shfl_xor(val, j, HBTX_WAVE);
+        const bool low = (lane & j) == 0;     // this lane holds the pair's first element
+        const bool up = (lane & k) == 0;      // the pair's direction (the same for both lanes)
+        const uint64_t a = low ? key : ok, c = low ? ok : key;
+        const bool swap = (a > c) == up;
+        if (swap)
+        {
+          key = ok;
+          val = ov;
+        }
+      }
+    if (lane < len)
+    {
+      keys[b + lane] = key;
+      vals[b + lane] = val;
+    }
   }
 }
 
