@@ -146,6 +146,50 @@ def reassign_line(synth, ctx, log, reps=3):
     return rec
 
 
+CPU_TOP_BIN = 18  # subhaloes of 2^19 particles and more are left out of the CPU sample
+
+
+def cpu_baseline(pop, args, log):
+    """The oracle (the reference's plain method restated in C++) on the host cores,
+    on a sample stratified by subhalo size: per log2-size bin b up to CPU_TOP_BIN,
+    min(64, 2^(17-b)) random subhaloes (at least one), each bin run as one batch on
+    all cores.  The process CPU time per subhalo of a bin, times the bin's count,
+    estimates the CPU time of the whole population of those bins; the rate is their
+    particles over that time spread over the cores (perfect packing)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    cores = min(16, os.cpu_count() or 1)
+    sizes = np.diff(pop.offsets)
+    bins = np.floor(np.log2(np.maximum(sizes, 1))).astype(int)
+    rng = np.random.default_rng(321)
+    cpu_total = n_total = 0.0
+    sampled = sampled_n = 0
+    wall = 0.0
+    for b in range(int(bins.min()), CPU_TOP_BIN + 1):
+        members = np.nonzero(bins == b)[0]
+        if len(members) == 0:
+            continue
+        k = int(min(len(members), max(1, min(64, 2 ** (17 - b)))))
+        ids = np.sort(rng.choice(members, k, replace=False))
+        sub = pop.subset(ids)
+        c0, w0 = time.process_time(), time.perf_counter()
+        pyoracle.unbind_batch(sub, pyoracle.default_params(bound_mass_precision=0.995), nthreads=cores)
+        cpu = time.process_time() - c0
+        wall += time.perf_counter() - w0
+        cpu_total += cpu / k * len(members)
+        n_total += float(sizes[members].sum())
+        sampled += k
+        sampled_n += sub.n
+    rate = n_total / (cpu_total / cores)
+    left = int((bins > CPU_TOP_BIN).sum())
+    log(f"cpu baseline {rate:.3e} particles/s on {cores} cores ({sampled} subhaloes sampled, {wall:.1f}s)")
+    return {"value": rate, "unit": "particles/s", "cores": cores, "kind": "port",
+            "sample": f"{sampled} subhaloes ({sampled_n} particles) of snapshot {args.snapshot}, stratified by "
+                      f"log2 size up to 2^{CPU_TOP_BIN + 1} particles, plain method, {wall:.1f}s; the rate is the "
+                      f"sampled bins' population ({int(n_total)} particles, {left} larger subhaloes left out) "
+                      f"over its estimated CPU time on {cores} cores"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -390,25 +434,7 @@ def main():
         line["other_configs"] = other_configs(hbtx, synth, ctx, log)
 
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle
-        cores = min(16, os.cpu_count() or 1)
-        order = np.random.default_rng(321).permutation(pop.nsub)
-        # grow the random sample until its CPU time reaches the target
-        take = 64
-        while True:
-            ids = np.sort(order[:take])
-            sub = pop.subset(ids)
-            r = pyoracle.unbind_batch(sub, pyoracle.default_params(bound_mass_precision=0.995), nthreads=cores)
-            secs = r["seconds"]
-            n_done = sub.n
-            if secs >= args.cpu_seconds * 0.5 or take >= pop.nsub:
-                break
-            take = min(pop.nsub, int(take * max(2.0, min(8.0, args.cpu_seconds / max(secs, 1e-3)))))
-        line["cpu_baseline"] = {"value": n_done / secs, "unit": "particles/s", "cores": cores, "kind": "port",
-                                "sample": f"{take} random subhaloes ({n_done} particles) of snapshot "
-                                          f"{args.snapshot}, plain method, {secs:.1f}s of unbinding"}
-        log(f"cpu baseline {n_done / secs:.3e} particles/s on {cores} cores")
+        line["cpu_baseline"] = cpu_baseline(pop, args, log)
     ctx.close()
     print(json.dumps(line), flush=True)
 
