@@ -243,7 +243,7 @@ struct hbtx_ctx
   DArr<int32_t> part32;
   DArr<int64_t> part64;
   DArr<uint64_t> kalt;
-  DArr<int32_t> valt, sort_ntile, sort_toff, ghist;
+  DArr<int32_t> valt, sort_ntile, sort_toff, ghist, gchunk;
   DArr<int4> sort_tiles;
   DArr<int32_t> red_ntile, red_toff;
   DArr<int4> red_tiles;
@@ -324,6 +324,7 @@ void reserve_sort(hbtx_ctx *c, int64_t n, int32_t nseg)
   const int64_t maxt = sort_tile_bound(n, nseg);
   c->sort_tiles.alloc(maxt);
   c->ghist.alloc(maxt * RADIX_MAX_BINS);
+  c->gchunk.alloc(maxt * RADIX_MAX_BINS); // chunk totals and bases, stored at the chunks' head tiles
 }
 
 // Sort (keys, vals) within each segment [seg_begin, seg_begin+seg_len); keys use
@@ -366,17 +367,21 @@ void seg_sort(hbtx_ctx *c, uint64_t *keys, int32_t *vals, int32_t nseg, const in
     if (rb == 9)
     {
       k_radix_hist<9><<<grid, SORT_THREADS, 0, c->stream>>>(kin, c->sort_tiles.p, ntiles_p, shift, c->ghist.p);
-      k_radix_offsets<9><<<grid, SORT_THREADS, 0, c->stream>>>(seg_begin, c->sort_tiles.p, ntiles_p,
-                                                               c->sort_toff.p, c->sort_ntile.p, c->ghist.p);
+      k_radix_chunk_scan<9><<<grid, SORT_THREADS, 0, c->stream>>>(c->sort_tiles.p, ntiles_p, c->sort_toff.p,
+                                                                  c->sort_ntile.p, c->ghist.p, c->gchunk.p);
+      k_radix_seg_scan<9><<<grid, SORT_THREADS, 0, c->stream>>>(seg_begin, c->sort_tiles.p, ntiles_p,
+                                                                c->sort_toff.p, c->sort_ntile.p, c->gchunk.p);
       k_radix_scatter<9><<<grid, SORT_THREADS, 0, c->stream>>>(kin, vin, kout, vout, c->sort_tiles.p, ntiles_p,
-                                                               shift, c->ghist.p);
+                                                               shift, c->ghist.p, c->sort_toff.p, c->gchunk.p);
       continue;
     }
     k_radix_hist<8><<<grid, SORT_THREADS, 0, c->stream>>>(kin, c->sort_tiles.p, ntiles_p, shift, c->ghist.p);
-    k_radix_offsets<8><<<grid, SORT_THREADS, 0, c->stream>>>(seg_begin, c->sort_tiles.p, ntiles_p, c->sort_toff.p,
-                                                          c->sort_ntile.p, c->ghist.p);
+    k_radix_chunk_scan<8><<<grid, SORT_THREADS, 0, c->stream>>>(c->sort_tiles.p, ntiles_p, c->sort_toff.p,
+                                                                c->sort_ntile.p, c->ghist.p, c->gchunk.p);
+    k_radix_seg_scan<8><<<grid, SORT_THREADS, 0, c->stream>>>(seg_begin, c->sort_tiles.p, ntiles_p, c->sort_toff.p,
+                                                              c->sort_ntile.p, c->gchunk.p);
     k_radix_scatter<8><<<grid, SORT_THREADS, 0, c->stream>>>(kin, vin, kout, vout, c->sort_tiles.p, ntiles_p, shift,
-                                                          c->ghist.p);
+                                                          c->ghist.p, c->sort_toff.p, c->gchunk.p);
   }
   HIPCHK(hipGetLastError());
 }

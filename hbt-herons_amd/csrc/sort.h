@@ -170,18 +170,64 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint64_t *__r
   }
 }
 
-// Per large segment: turn tile histograms into global scatter bases.  Driven by
-// the radix tiles (a block takes the segments whose first tile it meets), so
-// launches with no large segment cost nothing; thread i owns digits
-// [PER i, PER i + PER), and each digit's column of tile counts is read 8 tiles
-// at a time.
+// Tile histograms -> global scatter bases, in two steps so that no block walks a
+// large segment's whole column of tile counts:
+//  * k_radix_chunk_scan: every chunk of SORT_CHUNK consecutive tiles of a segment
+//    turns its counts into exclusive prefixes within the chunk (in place) and
+//    stores the chunk's totals at its head tile in gchunk;
+//  * k_radix_seg_scan: per large segment, the chunk totals become exclusive
+//    prefixes over the segment's chunks plus the digit's base in the segment
+//    (segment start + the counts of the smaller digits).
+// A tile's base for digit d is then ghist[t][d] + gchunk[head(t)][d].  Both are
+// driven by the radix tiles, so launches with no large segment cost nothing;
+// thread i owns digits [PER i, PER i + PER).
+#define SORT_CHUNK 16
 template <int RB>
-__global__ __launch_bounds__(SORT_THREADS) void k_radix_offsets(const int32_t *__restrict__ seg_begin,
-                                                               const int4 *__restrict__ tiles,
-                                                               const int32_t *__restrict__ ntiles_p,
-                                                               const int32_t *__restrict__ tile_off,
-                                                               const int32_t *__restrict__ ntile,
-                                                               int32_t *__restrict__ ghist)
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_chunk_scan(const int4 *__restrict__ tiles,
+                                                                  const int32_t *__restrict__ ntiles_p,
+                                                                  const int32_t *__restrict__ tile_off,
+                                                                  const int32_t *__restrict__ ntile,
+                                                                  int32_t *__restrict__ ghist,
+                                                                  int32_t *__restrict__ gchunk)
+{
+  constexpr int BINS = 1 << RB, PER = BINS / SORT_THREADS;
+  const int d0 = threadIdx.x * PER;
+  const int nt_all = *ntiles_p;
+  for (int t = blockIdx.x; t < nt_all; t += gridDim.x)
+  {
+    const int s = tiles[t].x;
+    const int t0 = tile_off[s], te = t0 + ntile[s];
+    if ((t - t0) % SORT_CHUNK != 0)
+      continue; // not a chunk's head tile
+    const int ce = min(te, t + SORT_CHUNK);
+#pragma unroll
+    for (int j = 0; j < PER; j++)
+    {
+      const int d = d0 + j;
+      int c[SORT_CHUNK];
+#pragma unroll
+      for (int k = 0; k < SORT_CHUNK; k++)
+        c[k] = (t + k < ce) ? ghist[(int64_t)(t + k) * BINS + d] : 0;
+      int run = 0;
+#pragma unroll
+      for (int k = 0; k < SORT_CHUNK; k++)
+      {
+        if (t + k < ce)
+          ghist[(int64_t)(t + k) * BINS + d] = run;
+        run += c[k];
+      }
+      gchunk[(int64_t)t * BINS + d] = run;
+    }
+  }
+}
+
+template <int RB>
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_seg_scan(const int32_t *__restrict__ seg_begin,
+                                                                const int4 *__restrict__ tiles,
+                                                                const int32_t *__restrict__ ntiles_p,
+                                                                const int32_t *__restrict__ tile_off,
+                                                                const int32_t *__restrict__ ntile,
+                                                                int32_t *__restrict__ gchunk)
 {
   constexpr int BINS = 1 << RB, PER = BINS / SORT_THREADS;
   __shared__ int32_t lds[SORT_THREADS / HBTX_WAVE];
@@ -193,26 +239,18 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_offsets(const int32_t *_
     const int s = tl.x;
     if (tl.y != seg_begin[s])
       continue; // not the segment's first tile
-    const int t0 = tile_off[s], nt = ntile[s];
+    const int t0 = tile_off[s], te = t0 + ntile[s];
     int run[PER];
 #pragma unroll
     for (int j = 0; j < PER; j++)
     {
       const int d = d0 + j;
       run[j] = 0;
-      for (int u = t0; u < t0 + nt; u += 8)
+      for (int u = t0; u < te; u += SORT_CHUNK)
       {
-        int c[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-          c[k] = (u + k < t0 + nt) ? ghist[(int64_t)(u + k) * BINS + d] : 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-        {
-          if (u + k < t0 + nt)
-            ghist[(int64_t)(u + k) * BINS + d] = run[j];
-          run[j] += c[k];
-        }
+        const int c = gchunk[(int64_t)u * BINS + d];
+        gchunk[(int64_t)u * BINS + d] = run[j];
+        run[j] += c;
       }
     }
     int mine = 0;
@@ -225,17 +263,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_offsets(const int32_t *_
     for (int j = 0; j < PER; j++)
     {
       const int d = d0 + j;
-      for (int u = t0; u < t0 + nt; u += 8)
-      {
-        int c[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-          c[k] = (u + k < t0 + nt) ? ghist[(int64_t)(u + k) * BINS + d] : 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-          if (u + k < t0 + nt)
-            ghist[(int64_t)(u + k) * BINS + d] = c[k] + base;
-      }
+      for (int u = t0; u < te; u += SORT_CHUNK)
+        gchunk[(int64_t)u * BINS + d] += base;
       base += run[j];
     }
     __syncthreads();
@@ -255,7 +284,9 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
                                                                int32_t *__restrict__ vout,
                                                                const int4 *__restrict__ tiles,
                                                                const int32_t *__restrict__ ntiles_p, int shift,
-                                                               const int32_t *__restrict__ ghist)
+                                                               const int32_t *__restrict__ ghist,
+                                                               const int32_t *__restrict__ tile_off,
+                                                               const int32_t *__restrict__ gchunk)
 {
   constexpr int BINS = 1 << RB, PER = BINS / SORT_THREADS;
   __shared__ uint64_t sk[SORT_TILE];
@@ -346,11 +377,13 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
     }
     __syncthreads();
     const int32_t *off = ghist + (int64_t)t * BINS;
+    const int t0 = tile_off[tl.x];
+    const int32_t *coff = gchunk + (int64_t)(t0 + ((t - t0) / SORT_CHUNK) * SORT_CHUNK) * BINS;
     for (int i = tid; i < n; i += SORT_THREADS)
     {
       const uint64_t k = sk[i];
       const int d = (int)((k >> shift) & (BINS - 1));
-      const int dst = off[d] + (i - dbase[d]);
+      const int dst = off[d] + coff[d] + (i - dbase[d]);
       kout[dst] = k;
       vout[dst] = sv[i];
     }
