@@ -41,7 +41,7 @@ struct RsGroup
   float margin;    // box widening that covers the rounding of offsets
   int32_t wrap;    // offsets may differ from the minimum image (periodic, extent >= box/4)
   int32_t top;     // level of the root (-1: no tracers)
-  int32_t pad0, pad1;
+  int32_t pad0, pad1; // (48-byte records)
 };
 
 struct RsArgs
