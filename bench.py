@@ -227,8 +227,8 @@ def main():
     if args.learned_only:
         args.no_plain = args.no_cpu = args.no_tree = args.no_configs = True
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; how the path would shard over "
-                                   "several is described in DESIGN.md, not built"}))
+        print(json.dumps({"error": "hbtx runs on exactly one GPU: sharding the unbinding over several GPUs "
+                                   "is designed but not built"}))
         sys.exit(2)
     if args.reassign_only:
         import hbtx

@@ -32,7 +32,7 @@ def frame(x, v, m, idx):
 
 
 def brute_force_warm(pop, prec, prop):
-    """Warm start rules: round 1 = proposed sources, every particle a target, in
+    """Warm start rules: the first iteration = proposed sources, every particle a target, in
     the proposal's frame; bound = E < 0 and proposed; accepted if at most 0.1% as
     many outsiders as insiders are bound, at least half the proposal is bound and
     the bound count is at least 2 MinNumPartOfSub; the correction sources are then
