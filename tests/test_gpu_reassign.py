@@ -249,3 +249,25 @@ def test_cpp_reassign_particles(ctx, tmp_path, periodic):
         np.testing.assert_array_equal(mt, e[3])
         np.testing.assert_array_equal(pid, e[4])
     assert off == len(got)
+
+
+def test_large_host_halo_sampled_queries(ctx):
+    """One host halo of about 3e5 particles (a box hierarchy five levels deep):
+    the GPU's targets for 300 random non-tracers against the oracle run on the
+    halo's tracers and those particles only.  A non-tracer's target depends only
+    on the tracers of its host halo, and keeping the particles in index order
+    keeps the order among equal distances."""
+    h = synth.host_haloes(seed=51, ngroups=1, subs=(8, 8), n_sub=(30000, 45000), spread=0.08)
+    print(f"large host halo: {h.n} particles, {int(h.is_tracer.sum())} tracers", flush=True)
+    g, _ = ctx.reassign_targets(h.pos, h.is_tracer, h.sub, h.group_offsets)
+    rng = np.random.default_rng(52)
+    queries = rng.choice(np.nonzero(h.is_tracer == 0)[0], 300, replace=False)
+    keep = np.sort(np.concatenate([np.nonzero(h.is_tracer == 1)[0], queries]))
+    o = O.reassign_targets(h.pos[keep], h.is_tracer[keep], h.sub[keep], [0, len(keep)],
+                           params=O.default_params())
+    want = dict(zip(keep.tolist(), o.tolist()))
+    got = g[queries]
+    exp = np.array([want[int(q)] for q in queries])
+    np.testing.assert_array_equal(got, exp)
+    assert (exp >= 0).sum() > 5
+    assert np.all(g[h.is_tracer == 1] == -1)
