@@ -49,3 +49,22 @@ def test_walk_chunking_changes_no_result(ctx):
         assert np.array_equal(o["order"], outs[0]["order"])
         assert np.array_equal(o["energy"], outs[0]["energy"])
         assert np.array_equal(o["result"], outs[0]["result"])
+
+
+def test_autotuner_runs_end_to_end(tmp_path):
+    """The autotuner on a small budget (2e5 particles per population): every
+    equal-size population, the power-law refinement and the chunk sweep run, and
+    the file it writes holds one of its candidates.  Its 2048-source populations
+    have about 2250 sources per subhalo, segments just over the LDS sort size."""
+    import subprocess
+    import sys
+    out = tmp_path / "tune.json"
+    script = os.path.join(os.path.dirname(hbtx.LIB_PATH), "..", "autotune.py")
+    print("autotune --particles 2e5", flush=True)
+    r = subprocess.run([sys.executable, script, "--particles", "2e5", "--out", str(out)], capture_output=True,
+                       text=True, timeout=180)
+    assert r.returncode == 0, r.stderr[-2000:]
+    d = json.loads(out.read_text())
+    cands = {row["direct_cutoff"] for row in d["mixture_timings"]}
+    assert d["direct_cutoff"] in cands and d["walk_chunks"] in (1, 4, 8, 16)
+    assert len(d["cutoff_timings"]) == 8
