@@ -299,7 +299,16 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
   const int w = tid / HBTX_WAVE;
   const int lane = lane_id();
   const int per_wave = SCAT_ITEMS * HBTX_WAVE;
-  for (int t = blockIdx.x; t < nt; t += gridDim.x)
+  // Workgroups are dispatched round-robin over the 8 XCDs.  With a grid that is a
+  // multiple of 8, XCD x takes the x-th eighth of the tiles, so the short digit
+  // runs that neighbouring tiles write next to each other meet in one L2 instead
+  // of leaving partial lines in eight of them.
+  const bool by_xcd = (gridDim.x & 7) == 0;
+  const int xcd = by_xcd ? (int)(blockIdx.x & 7) : 0;
+  const int step = by_xcd ? (int)(gridDim.x >> 3) : (int)gridDim.x;
+  const int per_xcd = by_xcd ? (nt + 7) / 8 : nt;
+  const int t_lo = xcd * per_xcd, t_hi = min(nt, t_lo + per_xcd);
+  for (int t = t_lo + (by_xcd ? (int)(blockIdx.x >> 3) : (int)blockIdx.x); t < t_hi; t += step)
   {
     const int4 tl = tiles[t];
     const int n = tl.z - tl.y;
