@@ -97,3 +97,50 @@ def test_direct_sums_of_subhaloes_just_over_the_lds_size_fresh_context():
         assert np.array_equal(a["order"], b["order"])
     finally:
         c.close()
+
+
+def _bitonic_reference(keys, vals):
+    """The sort network of the in-lane and LDS sorts, run in numpy: the segment is
+    padded with all-ones keys to the next power of two P, and for k = 2..P,
+    j = k/2..1 the pair (i, i^j), i < i^j, is exchanged when (a > c) equals
+    (i & k) == 0."""
+    n = len(keys)
+    P = 1
+    while P < n:
+        P <<= 1
+    k_ = np.full(P, np.iinfo(np.uint64).max, np.uint64)
+    v_ = np.zeros(P, np.int32)
+    k_[:n], v_[:n] = keys, vals
+    i = np.arange(P)
+    k = 2
+    while k <= P:
+        j = k >> 1
+        while j > 0:
+            ixj = i ^ j
+            lo = i[ixj > i]
+            hi = lo ^ j
+            a, c = k_[lo], k_[hi]
+            sw = (a > c) == ((lo & k) == 0)
+            ls, hs = lo[sw], hi[sw]
+            k_[ls], k_[hs] = k_[hs].copy(), k_[ls].copy()
+            v_[ls], v_[hs] = v_[hs].copy(), v_[ls].copy()
+            j >>= 1
+        k <<= 1
+    return k_[:n], v_[:n]
+
+
+def test_small_sorts_follow_the_bitonic_network(ctx):
+    """Segments of up to 2048 keys: keys and the order of tied keys' values equal
+    the bitonic network's, whichever barriers separate its stages on the GPU."""
+    rng = np.random.default_rng(11)
+    sizes = [33, 64, 65, 100, 128, 129, 255, 300, 512, 513, 700, 1024, 1500, 2047, 2048]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(off[-1])
+    keys = rng.integers(0, 8, n).astype(np.uint64)  # many ties
+    vals = np.arange(n, dtype=np.int32)
+    k2, v2 = ctx.segmented_sort(keys, vals, off)
+    for s in range(len(sizes)):
+        a, b = off[s], off[s + 1]
+        ek, ev = _bitonic_reference(keys[a:b], vals[a:b])
+        assert np.array_equal(k2[a:b], ek), sizes[s]
+        assert np.array_equal(v2[a:b], ev), sizes[s]
