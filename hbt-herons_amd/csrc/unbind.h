@@ -747,6 +747,38 @@ HBTX_DEV bool wave_in_tile(int b, int e)
   return b + (int)(threadIdx.x & ~(HBTX_WAVE - 1)) < e;
 }
 
+// A tile [b, e) of at most RED_THREADS slots reduced by one wave, with no barrier:
+// the lanes take the block's waves in turn (slot b + 64 w + lane), reduce each with
+// the same shuffle tree and add the wave sums in wave order, so lane 0 gets exactly
+// the totals block_reduce_vec gives when each thread of a block holds one slot.
+// acc(q, v) adds slot q's terms to v.
+template <int NV, class Acc>
+HBTX_DEV void wave_reduce_tile(int b, int e, double tot[RED_NV], Acc acc)
+{
+#pragma unroll
+  for (int k = 0; k < RED_NV; k++)
+    tot[k] = 0.;
+  for (int w = 0; w < RED_THREADS / HBTX_WAVE && b + w * HBTX_WAVE < e; w++)
+  {
+    double v[RED_NV];
+#pragma unroll
+    for (int k = 0; k < RED_NV; k++)
+      v[k] = 0.;
+    const int q = b + w * HBTX_WAVE + lane_id();
+    if (q < e)
+      acc(q, v);
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+    {
+      double x = v[k];
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1)
+        x += __shfl_down(x, d, HBTX_WAVE);
+      tot[k] += x;
+    }
+  }
+}
+
 __global__ void k_red_ntiles(const int32_t *len, int32_t nsub, int32_t *ntile)
 {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -805,9 +837,25 @@ __global__ __launch_bounds__(RED_THREADS) void k_red_tile(SubArrays S, PartArray
 {
   __shared__ double lds[(RED_THREADS / HBTX_WAVE) * RED_NV];
   const int nt = *ntiles_p;
+  // tiles of at most RED_THREADS slots (most subhaloes): one wave each
+  const int wpb = RED_THREADS / HBTX_WAVE;
+  for (int t = uniform((int)(blockIdx.x * wpb + threadIdx.x / HBTX_WAVE)); t < nt; t += gridDim.x * wpb)
+  {
+    const int4 tl = tiles[t];
+    if (tl.z - tl.y > RED_THREADS)
+      continue;
+    double v[RED_NV];
+    wave_reduce_tile<Op::NV>(tl.y, tl.z, v, [&](int q, double *vv) { Op::acc(S, Q, P, tl.x, q, vv); });
+    if (lane_id() == 0)
+      for (int k = 0; k < RED_NV; k++)
+        partial[(int64_t)t * RED_NV + k] = v[k];
+  }
+  // larger tiles: the whole block
   for (int t = blockIdx.x; t < nt; t += gridDim.x)
   {
     const int4 tl = tiles[t];
+    if (tl.z - tl.y <= RED_THREADS)
+      continue;
     double v[RED_NV];
 #pragma unroll
     for (int k = 0; k < RED_NV; k++)
