@@ -132,7 +132,7 @@ struct Batch
   DArr<int8_t> ptype;
   DArr<int32_t> sub_of, sub_zero, off, off_pairs, order, order_zero;
   DArr<int32_t> state, cur, cur2, src, src_sub, fvals, ford;
-  DArr<float> energy, selfm, fenergy, fpot, eref;
+  DArr<float> energy, esl, selfm, fenergy, fpot, eref;
   DArr<uint8_t> proposal;
   DArr<int32_t> flag, fscan, crank, cnt, cscan;
   DArr<int64_t> flag64, fscan64;
@@ -558,6 +558,7 @@ void batch_reserve(hbtx_ctx *c, Batch &B)
   B.fvals.alloc(n);
   B.ford.alloc(n);
   B.energy.alloc(n);
+  B.esl.alloc(n);
   B.selfm.alloc(n);
   B.fenergy.alloc(n);
   B.fpot.alloc(n);
@@ -672,6 +673,7 @@ View make_view(Batch &B, int64_t p0, int64_t n, int32_t s0, int32_t nsub, bool w
   Q.ptype = B.has_type ? B.ptype.p + p0 : nullptr;
   Q.state = B.state.p + p0;
   Q.energy = B.energy.p + p0;
+  Q.esl = B.esl.p + p0;
   Q.selfm = B.selfm.p + p0;
   Q.cur = B.cur.p + p0;
   Q.cur2 = B.cur2.p + p0;
@@ -892,6 +894,7 @@ WalkArgs walk_args(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   W.ref = V.S.ref;
   W.oldref = V.S.oldref;
   W.energy = V.Q.energy;
+  W.esl = V.Q.esl;
   W.eref = B.eref.p + V.p0;
   W.phi_out = nullptr;
   W.stats = c->stats.p;

@@ -46,6 +46,7 @@ struct PartArrays
   const uint8_t *proposal; // warm-start proposal (1 = proposed bound), may be null
   int32_t *state;
   float *energy;
+  float *esl; // the energies of the round's targets again, by slot (the walk writes both)
   float *selfm;
   int32_t *cur, *cur2;
 };
@@ -507,16 +508,14 @@ __global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, const 
     {
       int64_t f = 0;
       bool outside = false; // bound, but not in the proposal
-      if (q < end)
+      // the energy by slot: a contiguous read (the particle's own is a gather)
+      if (q < end && Q.esl[q] < 0.f)
       {
-        const int p = Q.cur[q];
-        if (Q.energy[p] < 0.f)
-        {
-          if (warm && !Q.proposal[p])
-            outside = true;
-          else
-            f = 1 | ((int64_t)(is_tracer(P, ptype_of(Q, p)) ? 1 : 0) << 32);
-        }
+        const int p = (warm || Q.ptype) ? Q.cur[q] : 0;
+        if (warm && !Q.proposal[p])
+          outside = true;
+        else
+          f = 1 | ((int64_t)(is_tracer(P, ptype_of(Q, p)) ? 1 : 0) << 32);
       }
       flag[q] = f;
       // the tile is one subhalo's: one counter update per wave

@@ -55,6 +55,7 @@ struct WalkArgs
   const float *ref;         // per subhalo frame [6]
   const float *oldref;      // per subhalo previous frame [6]
   float *energy;
+  float *esl;               // the same energies by slot (off[s] + target index)
   float *eref;              // refinement energies (MODE_REFINE targets)
   float *phi_out;           // optional: specific potential per particle (float), may be null
   unsigned long long *stats;
@@ -246,7 +247,8 @@ __global__ __launch_bounds__(64) void k_walk_energy(WalkArgs A)
     const int t = item.y + lane;
     const int nt = A.ntgt[s];
     const bool valid = t < nt;
-    const int p = valid ? A.cur[A.off[s] + t] : 0;
+    const int q = A.off[s] + t; // the target's slot
+    const int p = valid ? A.cur[q] : 0;
     float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
     if (valid)
       x = A.posm[p];
@@ -296,6 +298,7 @@ __global__ __launch_bounds__(64) void k_walk_energy(WalkArgs A)
         if (P.thermal)
           E += v.w;
         A.energy[p] = E;
+        A.esl[q] = E;
       }
     }
     else
@@ -306,7 +309,9 @@ __global__ __launch_bounds__(64) void k_walk_energy(WalkArgs A)
       const float dK = (float)(0.5 * (double)(dV[0] * dV[0] + dV[1] * dV[1] + dV[2] * dV[2]));
       rel_vel(P, x.x, x.y, x.z, v.x, v.y, v.z, of, of + 3, ov);
       const float tkin = (ov[0] * dV[0] + ov[1] * dV[1] + ov[2] * dV[2]) + dK;
-      A.energy[p] = (float)((double)A.energy[p] + ((double)tkin - phi));
+      const float E = (float)((double)A.energy[p] + ((double)tkin - phi));
+      A.energy[p] = E;
+      A.esl[q] = E;
     }
   }
   if (STATS && A.stats && lane == 0)
