@@ -101,31 +101,27 @@ inline hbtx_params DefaultParams()
 
 namespace detail
 {
-template <class T, class = void>
-struct has_internal_energy : std::false_type
+// C++11 member detection (the reference builds with CMAKE_CXX_STANDARD 11)
+template <class... Ts>
+struct make_void
 {
+  typedef void type;
 };
-template <class T>
-struct has_internal_energy<T, std::void_t<decltype(std::declval<T>().InternalEnergy)>> : std::true_type
-{
-};
-template <class T, class = void>
-struct has_type : std::false_type
-{
-};
-template <class T>
-struct has_type<T, std::void_t<decltype(std::declval<T>().Type)>> : std::true_type
-{
-};
+template <class... Ts>
+using void_t = typename make_void<Ts...>::type;
+template <bool B>
+using bool_c = std::integral_constant<bool, B>;
+
 #define HBTX_MEMBER_TRAIT(name)                                                                                  \
   template <class T, class = void>                                                                             \
   struct has_##name : std::false_type                                                                          \
   {                                                                                                            \
   };                                                                                                           \
   template <class T>                                                                                           \
-  struct has_##name<T, std::void_t<decltype(std::declval<T>().name)>> : std::true_type                         \
+  struct has_##name<T, void_t<decltype(std::declval<T>().name)>> : std::true_type                              \
   {                                                                                                            \
   };
+HBTX_MEMBER_TRAIT(InternalEnergy)
 HBTX_MEMBER_TRAIT(NboundType)
 HBTX_MEMBER_TRAIT(MboundType)
 HBTX_MEMBER_TRAIT(NumberUnbindingIterations)
@@ -138,6 +134,157 @@ HBTX_MEMBER_TRAIT(SnapshotOfSink)
 HBTX_MEMBER_TRAIT(Type)
 HBTX_MEMBER_TRAIT(InertialTensor)
 #undef HBTX_MEMBER_TRAIT
+template <class T>
+struct has_internal_energy : has_InternalEnergy<T>
+{
+};
+template <class T>
+struct has_type : has_Type<T>
+{
+};
+
+// Optional members, by tag dispatch: the first overload of each pair runs when
+// the type has the member(s), the second is a no-op.
+template <class P>
+float internal_energy(const P &p, std::true_type)
+{
+  return (float)p.InternalEnergy;
+}
+template <class P>
+float internal_energy(const P &, std::false_type)
+{
+  return 0.f;
+}
+template <class P>
+int32_t particle_type(const P &p, std::true_type)
+{
+  return (int32_t)p.Type;
+}
+template <class P>
+int32_t particle_type(const P &, std::false_type)
+{
+  return 1;
+}
+template <class S>
+void set_nbound_type(S &sub, const int64_t *v, std::true_type)
+{
+  for (int t = 0; t < 6; t++)
+    sub.NboundType[t] = v ? v[t] : 0;
+}
+template <class S>
+void set_nbound_type(S &, const int64_t *, std::false_type)
+{
+}
+template <class S>
+void set_mbound_type(S &sub, const float *v, std::true_type)
+{
+  for (int t = 0; t < 6; t++)
+    sub.MboundType[t] = v ? v[t] : 0.f;
+}
+template <class S>
+void set_mbound_type(S &, const float *, std::false_type)
+{
+}
+template <class S>
+void set_iterations(S &sub, int32_t it, std::true_type)
+{
+  sub.NumberUnbindingIterations = it;
+}
+template <class S>
+void set_iterations(S &, int32_t, std::false_type)
+{
+}
+// :646-668, SnapshotOfDeath when a subhalo disrupts
+template <class S>
+void mark_death(S &sub, int snapshot_id, std::true_type)
+{
+  if (sub.SnapshotOfDeath == -1)
+    sub.SnapshotOfDeath = snapshot_id;
+}
+template <class S>
+void mark_death(S &, int, std::false_type)
+{
+}
+// :690-697, a resolved subhalo is alive again
+template <class S>
+void revive(S &sub, std::true_type)
+{
+  if (sub.SnapshotOfDeath != -1)
+  {
+    sub.SnapshotOfDeath = -1;
+    sub.DescendantTrackId = -1;
+  }
+}
+template <class S>
+void revive(S &, std::false_type)
+{
+}
+// :698-702, and no longer trapped
+template <class S>
+void clear_sink(S &sub, std::true_type)
+{
+  if (sub.SinkTrackId != -1)
+  {
+    sub.SnapshotOfSink = -1;
+    sub.SinkTrackId = -1;
+  }
+}
+template <class S>
+void clear_sink(S &, std::false_type)
+{
+}
+template <class S>
+void store_binding_energies(S &sub, const float *e, int64_t nb, std::true_type)
+{
+  sub.ParticleBindingEnergies.assign(e, e + nb);
+}
+template <class S>
+void store_binding_energies(S &, const float *, int64_t, std::false_type)
+{
+}
+template <class S>
+void store_potential_energies(S &sub, const float *e, int64_t nb, std::true_type)
+{
+  sub.ParticlePotentialEnergies.assign(e, e + nb);
+}
+template <class S>
+void store_potential_energies(S &, const float *, int64_t, std::false_type)
+{
+}
+template <class S>
+void store_shape(S &sub, const hbtx_sub_profile &P, std::true_type)
+{
+  for (int k = 0; k < 6; k++)
+  {
+    sub.InertialTensor[k] = P.inertial_tensor[k];
+    sub.InertialTensorWeighted[k] = P.inertial_tensor_weighted[k];
+  }
+}
+template <class S>
+void store_shape(S &, const hbtx_sub_profile &, std::false_type)
+{
+}
+// per-type bound count and mass of one particle (CountParticleTypes)
+template <class S, class P>
+void add_type(S &sub, const P &p, std::true_type, std::true_type)
+{
+  sub.NboundType[(int)p.Type]++;
+  sub.MboundType[(int)p.Type] += p.Mass;
+}
+template <class S, class P>
+void add_type(S &sub, const P &p, std::true_type, std::false_type)
+{
+  sub.NboundType[(int)p.Type]++;
+}
+template <class S, class P>
+void add_type(S &sub, const P &p, std::false_type, std::true_type)
+{
+  sub.MboundType[(int)p.Type] += p.Mass;
+}
+template <class S, class P>
+void add_type(S &, const P &, std::false_type, std::false_type)
+{
+}
 
 template <class V>
 inline void put3(float *dst, const V &v)
@@ -285,10 +432,10 @@ void RefineParticles(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT 
       detail::put3(&vel[3 * k], p.PhysicalVelocity);
       mass[k] = (float)p.Mass;
       id[k] = (int64_t)p.Id;
-      if constexpr (thermal)
-        u[k] = (float)p.InternalEnergy;
-      if constexpr (typed)
-        type[k] = (int32_t)p.Type;
+      if (thermal)
+        u[k] = detail::internal_energy(p, detail::bool_c<thermal>());
+      if (typed)
+        type[k] = detail::particle_type(p, detail::bool_c<typed>());
     }
   }
   std::vector<int64_t> order(n);
@@ -314,12 +461,8 @@ void RefineParticles(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT 
     }
     S.Nbound = R.nbound;
     S.Mbound = R.mbound;
-    if constexpr (detail::has_NboundType<SubhaloT>::value)
-      for (int t = 0; t < 6; t++)
-        S.NboundType[t] = R.nbound_type[t];
-    if constexpr (detail::has_MboundType<SubhaloT>::value)
-      for (int t = 0; t < 6; t++)
-        S.MboundType[t] = R.mbound_type[t];
+    detail::set_nbound_type(S, R.nbound_type, detail::has_NboundType<SubhaloT>());
+    detail::set_mbound_type(S, R.mbound_type, detail::has_MboundType<SubhaloT>());
     detail::get3(S.ComovingAveragePosition, R.com_pos);
     detail::get3(S.PhysicalAverageVelocity, R.com_vel);
     detail::get3(S.ComovingMostBoundPosition, R.mostbound_pos);
@@ -333,41 +476,28 @@ void RefineParticles(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT 
     detail::get3(S.CorePhysicalVelocity, R.core_vel);
     S.CoreComovingSigmaR = R.core_sigma_r;
     S.CorePhysicalSigmaV = R.core_sigma_v;
-    if constexpr (detail::has_NumberUnbindingIterations<SubhaloT>::value)
-      S.NumberUnbindingIterations = R.iterations;
+    detail::set_iterations(S, R.iterations, detail::has_NumberUnbindingIterations<SubhaloT>());
     if (N == 0)
       continue; // orphan: nothing else changes (:471-489)
     if (R.disrupted)
     {
       // :646-668 (MostBoundParticleId keeps its previous value)
-      if constexpr (detail::has_SnapshotOfDeath<SubhaloT>::value)
-        if (S.SnapshotOfDeath == -1)
-          S.SnapshotOfDeath = snapshot_id;
+      detail::mark_death(S, snapshot_id, detail::has_SnapshotOfDeath<SubhaloT>());
     }
     else
     {
       // :690-702, :791-792
-      if constexpr (detail::has_SnapshotOfDeath<SubhaloT>::value && detail::has_DescendantTrackId<SubhaloT>::value)
-        if (S.SnapshotOfDeath != -1)
-        {
-          S.SnapshotOfDeath = -1;
-          S.DescendantTrackId = -1;
-        }
-      if constexpr (detail::has_SinkTrackId<SubhaloT>::value && detail::has_SnapshotOfSink<SubhaloT>::value)
-        if (S.SinkTrackId != -1)
-        {
-          S.SnapshotOfSink = -1;
-          S.SinkTrackId = -1;
-        }
+      detail::revive(S, detail::bool_c<detail::has_SnapshotOfDeath<SubhaloT>::value &&
+                                       detail::has_DescendantTrackId<SubhaloT>::value>());
+      detail::clear_sink(S, detail::bool_c<detail::has_SinkTrackId<SubhaloT>::value &&
+                                           detail::has_SnapshotOfSink<SubhaloT>::value>());
       S.MostBoundParticleId = R.mostbound_id;
     }
     const int64_t nb = R.nbound;
-    if constexpr (detail::has_ParticleBindingEnergies<SubhaloT>::value)
-      if (save_binding_energies)
-        S.ParticleBindingEnergies.assign(energy.begin() + a, energy.begin() + a + nb);
-    if constexpr (detail::has_ParticlePotentialEnergies<SubhaloT>::value)
-      if (save_potential_energies)
-        S.ParticlePotentialEnergies.assign(pot.begin() + a, pot.begin() + a + nb);
+    if (save_binding_energies)
+      detail::store_binding_energies(S, energy.data() + a, nb, detail::has_ParticleBindingEnergies<SubhaloT>());
+    if (save_potential_energies)
+      detail::store_potential_energies(S, pot.data() + a, nb, detail::has_ParticlePotentialEnergies<SubhaloT>());
   }
 }
 
@@ -461,13 +591,8 @@ void CalculateProfileProperties(Context &ctx, const hbtx_params &prm, std::vecto
         S.RmaxComovingOfLastMaxVmax = S.RmaxComoving;
       }
     }
-    if constexpr (detail::has_InertialTensor<SubhaloT>::value)
-      if (shape)
-        for (int k = 0; k < 6; k++)
-        {
-          S.InertialTensor[k] = P.inertial_tensor[k];
-          S.InertialTensorWeighted[k] = P.inertial_tensor_weighted[k];
-        }
+    if (shape)
+      detail::store_shape(S, P, detail::has_InertialTensor<SubhaloT>());
   }
 }
 
@@ -529,12 +654,8 @@ void MergeTo(SubhaloT &sub, SubhaloT &host)
   sub.Particles.resize(0);
   // CountParticles (src/subhalo.cpp:435-455) over no particles
   sub.Mbound = 0.f;
-  if constexpr (detail::has_NboundType<SubhaloT>::value)
-    for (auto &x : sub.NboundType)
-      x = 0;
-  if constexpr (detail::has_MboundType<SubhaloT>::value)
-    for (auto &x : sub.MboundType)
-      x = 0.f;
+  detail::set_nbound_type(sub, nullptr, detail::has_NboundType<SubhaloT>());
+  detail::set_mbound_type(sub, nullptr, detail::has_MboundType<SubhaloT>());
 }
 
 // Subhalo_t::MergeRecursively (src/subhalo_merge.cpp:45-91): depth first over the
@@ -588,9 +709,9 @@ void TruncateSource(SubhaloT &sub, double source_sub_relax_factor)
 // NestedSubhalos order (:823); after it, its hierarchy is checked for phase-space
 // mergers, and a subhalo that gained particles by one is unbound again (:833-837).
 // Trees must be disjoint; a subhalo reached twice is an error.
-template <class SubhaloT>
+template <class SubhaloT, class RootList>
 void RecursiveUnbind(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT> &Subhalos,
-                     const std::vector<int64_t> &roots, int snapshot_id, bool merge_trapped,
+                     const RootList &roots, int snapshot_id, bool merge_trapped,
                      uint32_t flags = HBTX_BATCHED, bool save_binding_energies = false,
                      bool save_potential_energies = false)
 {
@@ -600,8 +721,9 @@ void RecursiveUnbind(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT>
   std::vector<std::vector<int64_t>> levels;
   // iterative post-order walk of every tree
   std::vector<std::pair<int64_t, size_t>> stack;
-  for (int64_t r : roots)
+  for (size_t ri = 0; ri < (size_t)roots.size(); ri++)
   {
+    const int64_t r = (int64_t)roots[ri];
     if (r < 0 || r >= ns || seen[r])
       throw Error(HBTX_E_INVALID, "RecursiveUnbind: bad or repeated root " + std::to_string(r));
     seen[r] = 1;
@@ -664,20 +786,20 @@ void RecursiveUnbind(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT>
 // new-born subhaloes in one more batch.  Returns PrintSubhaloStatistics' counts;
 // *elapsed_s receives the wall time (PrintTimeImbalanceStatistics' figure on
 // one rank).
-template <class SubhaloT, class GroupList>
+template <class SubhaloT, class GroupList, class HeadList, class FieldList>
 SubhaloCounts RefineSnapshot(Context &ctx, const hbtx_params &prm, std::vector<SubhaloT> &Subhalos,
-                             const GroupList &SubGroups, const GroupList &SubGroupsOfHeads,
-                             const std::vector<int64_t> &FieldSubhalos, int64_t NumSubOld, int snapshot_id,
+                             const GroupList &SubGroups, const HeadList &SubGroupsOfHeads,
+                             const FieldList &FieldSubhalos, int64_t NumSubOld, int snapshot_id,
                              bool merge_trapped, double source_sub_relax_factor, double *elapsed_s = nullptr,
                              uint32_t flags = HBTX_BATCHED, bool save_binding_energies = false,
                              bool save_potential_energies = false)
 {
   const auto t0 = std::chrono::steady_clock::now();
-  if (SubGroupsOfHeads.size() != SubGroups.size())
+  if ((size_t)SubGroupsOfHeads.size() != (size_t)SubGroups.size())
     throw Error(HBTX_E_INVALID, "RefineSnapshot: SubGroupsOfHeads must have one entry per host halo");
   std::vector<int64_t> roots;
   std::vector<std::pair<int64_t, size_t>> restore;
-  for (size_t h = 0; h < SubGroups.size(); h++)
+  for (size_t h = 0; h < (size_t)SubGroups.size(); h++)
   {
     if (SubGroups[h].size() == 0)
       continue;
@@ -685,7 +807,7 @@ SubhaloCounts RefineSnapshot(Context &ctx, const hbtx_params &prm, std::vector<S
     auto &nests = Subhalos[c].NestedSubhalos;
     restore.push_back({c, nests.size()});
     const auto &heads = SubGroupsOfHeads[h];
-    for (size_t j = 1; j < heads.size(); j++)
+    for (size_t j = 1; j < (size_t)heads.size(); j++)
       nests.push_back(heads[j]);
     roots.push_back(c);
   }
@@ -703,8 +825,8 @@ SubhaloCounts RefineSnapshot(Context &ctx, const hbtx_params &prm, std::vector<S
   for (auto &r : restore)
     Subhalos[r.first].NestedSubhalos.resize(r.second);
   std::vector<SubhaloT *> rest;
-  for (int64_t s : FieldSubhalos)
-    rest.push_back(&Subhalos[s]);
+  for (size_t i = 0; i < (size_t)FieldSubhalos.size(); i++)
+    rest.push_back(&Subhalos[FieldSubhalos[i]]);
   for (int64_t s = NumSubOld; s < (int64_t)Subhalos.size(); s++)
     rest.push_back(&Subhalos[s]);
   if (!rest.empty())
@@ -722,12 +844,9 @@ namespace detail
 template <class ParticleT>
 bool IsTracer(const ParticleT &p, const hbtx_params &prm)
 {
-  if (prm.dm_only)
+  if (prm.dm_only || !has_type<ParticleT>::value)
     return true;
-  if constexpr (has_type<ParticleT>::value)
-    return ((1 << (int)p.Type) & prm.tracer_mask) != 0;
-  else
-    return true;
+  return ((1 << (int)particle_type(p, has_type<ParticleT>())) & prm.tracer_mask) != 0;
 }
 } // namespace detail
 
@@ -743,24 +862,16 @@ void CountParticleTypes(SubhaloT &S, const hbtx_params &prm)
     S.TracerIndex = 0;
     return;
   }
-  if constexpr (detail::has_NboundType<SubhaloT>::value)
-    for (auto &x : S.NboundType)
-      x = 0;
-  if constexpr (detail::has_MboundType<SubhaloT>::value)
-    for (auto &x : S.MboundType)
-      x = 0.f;
+  detail::set_nbound_type(S, nullptr, detail::has_NboundType<SubhaloT>());
+  detail::set_mbound_type(S, nullptr, detail::has_MboundType<SubhaloT>());
   int64_t tracer = -1;
   const int64_t nb = std::min<int64_t>((int64_t)S.Nbound, (int64_t)S.Particles.size());
   for (int64_t i = 0; i < nb; i++)
   {
     const auto &p = S.Particles[i];
-    if constexpr (detail::has_type<ParticleT>::value)
-    {
-      if constexpr (detail::has_NboundType<SubhaloT>::value)
-        S.NboundType[(int)p.Type]++;
-      if constexpr (detail::has_MboundType<SubhaloT>::value)
-        S.MboundType[(int)p.Type] += p.Mass;
-    }
+    detail::add_type(S, p,
+                     detail::bool_c<detail::has_type<ParticleT>::value && detail::has_NboundType<SubhaloT>::value>(),
+                     detail::bool_c<detail::has_type<ParticleT>::value && detail::has_MboundType<SubhaloT>::value>());
     if (tracer < 0 && detail::IsTracer(p, prm))
       tracer = i;
   }
