@@ -75,30 +75,21 @@ def timed_steps(ctx, steps, warmup, step):
     return times
 
 
-def plain_gpu_estimate(hbtx, pop, prm, share):
-    counts = pop.counts()
-    rng = np.random.default_rng(123)
-    big = np.where(counts > 10000)[0]
-    small = np.where(counts <= 10000)[0]
-    pick = rng.choice(small, max(1, int(share * len(small))), replace=False) if len(small) else []
-    sample = np.concatenate([big, np.sort(pick)]).astype(np.int64)
-    weight_small = len(small) / max(1, len(pick))
+def plain_gpu_full(hbtx, pop, prm):
+    """The plain GPU loop on the whole evaluated snapshot: Subhalo_t::Unbind one
+    subhalo at a time (HBTX_PLAIN, host-driven iterations, the same kernels and
+    results as the batched path), every subhalo timed, nothing sampled.  Inputs
+    are resident before the clock starts."""
     c2 = hbtx.Context(0)
-    c2.load(pop.subset(sample))
-    c2.run(prm, hbtx.PLAIN)  # warm-up
+    c2.load(pop)
+    c2.sync()
     w0 = time.perf_counter()
     c2.run(prm, hbtx.PLAIN)
     c2.sync()
-    t_all = time.perf_counter() - w0
-    c2.load(pop.subset(big))
-    w0 = time.perf_counter()
-    c2.run(prm, hbtx.PLAIN)
-    c2.sync()
-    t_big = time.perf_counter() - w0
+    secs = time.perf_counter() - w0
+    out = c2.fetch()
     c2.close()
-    est = t_big + max(0.0, t_all - t_big) * weight_small
-    return est, (f"all {len(big)} subhaloes with >1e4 particles plus {len(pick)} of {len(small)} smaller "
-                 f"ones (weight {weight_small:.1f})")
+    return secs, out
 
 
 def other_configs(hbtx, synth, ctx, log):
@@ -182,12 +173,14 @@ def cpu_baseline(pop, args, log):
         sampled_n += sub.n
     rate = n_total / (cpu_total / cores)
     left = int((bins > CPU_TOP_BIN).sum())
+    left_n = int(sizes[bins > CPU_TOP_BIN].sum())
     log(f"cpu baseline {rate:.3e} particles/s on {cores} cores ({sampled} subhaloes sampled, {wall:.1f}s)")
     return {"value": rate, "unit": "particles/s", "cores": cores, "kind": "port",
             "sample": f"{sampled} subhaloes ({sampled_n} particles) of snapshot {args.snapshot}, stratified by "
                       f"log2 size up to 2^{CPU_TOP_BIN + 1} particles, plain method, {wall:.1f}s; the rate is the "
-                      f"sampled bins' population ({int(n_total)} particles, {left} larger subhaloes left out) "
-                      f"over its estimated CPU time on {cores} cores"}
+                      f"sampled bins' population ({int(n_total)} particles) over its estimated CPU time on "
+                      f"{cores} cores; the {left} subhaloes above 2^{CPU_TOP_BIN + 1} particles ({left_n} "
+                      f"particles) are neither timed nor extrapolated: the rate covers the other bins only"}
 
 
 def main():
@@ -199,15 +192,21 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--snapshot", type=int, default=6, help="evaluated snapshot (trained on 0..5)")
     ap.add_argument("--threshold", type=float, default=0.1, help="proposal: P(bound) > threshold")
-    ap.add_argument("--warm-min", type=int, default=1000,
-                    help="subhaloes of fewer particles run the plain iteration (no proposal): a flipped "
-                         "particle there would exceed the per-subhalo mismatch tolerance on its own")
+    ap.add_argument("--warm-min", type=int, default=30000,
+                    help="subhaloes of fewer particles run the plain iteration (no proposal).  The warm "
+                         "start and the plain iteration take different paths, so particles within the tree "
+                         "approximation of E = 0 can fall either way; below ~1e4 bound particles a single "
+                         "one exceeds the 1e-3 per-subhalo tolerance.  30000 is the smallest threshold of "
+                         "--warm-min-sweep at which every subhalo of the headline workload is within it at "
+                         "BoundMassPrecision 0.995 and 1.0")
+    ap.add_argument("--warm-min-sweep", default="",
+                    help="comma-separated warm-start thresholds: per threshold, the learned step's time and "
+                         "its per-subhalo agreement with the plain iteration at both BoundMassPrecision "
+                         "settings (an extra line, 'warm_min_sweep')")
     ap.add_argument("--train-per-snapshot", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--direct-cutoff", type=int, default=-1,
                     help="sources <= cutoff use the exact sum; -1 = tuned value, 0 = tree everywhere")
-    ap.add_argument("--plain-sample", type=float, default=0.01,
-                    help="fraction of subhaloes below 1e4 particles in the plain-loop sample")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -321,6 +320,9 @@ def main():
         return
     mismatch = learned.bound_mass_mismatch(pop, warm, ref)
     mism = learned.mismatch_summary(pop, warm, ref)
+    if mism["subhalos_within"] < 1.0:
+        log(f"PER-SUBHALO GATE FAILED: {mism['subhalos_within']:.5f} of the subhaloes within "
+            f"{mism['tolerance']} of the plain iteration's bound mass")
     # the same comparison at BoundMassPrecision 1.0 (iterate until no particle leaves)
     prm1 = hbtx.workload_params(direct_cutoff=args.direct_cutoff, bound_mass_precision=1.0)
     ms_plain1 = timed_steps(ctx, args.steps, args.warmup, lambda: ctx.run(prm1, gflag))
@@ -339,7 +341,38 @@ def main():
              "mismatch": learned.mismatch_summary(pop, warm1, ref1)}
     log(f"BoundMassPrecision 1.0: batched {prec1['batched_ms_per_step']:.2f} ms, learned "
         f"{prec1['learned_ms_per_step']:.2f} ms, mismatch {prec1['mismatch']['summed']:.2e}")
-    # work counters: one extra, untimed run of the counting instantiation (same results, deterministic work)
+    gate = {"tolerance": 1e-3, "rule": "bound-mass symmetric difference over the plain iteration's bound mass, "
+                                       "per subhalo and summed, at BoundMassPrecision 0.995 and 1.0",
+            "passed": bool(mism["subhalos_within"] >= 1.0 and mism["summed"] <= 1e-3
+                           and prec1["mismatch"]["subhalos_within"] >= 1.0
+                           and prec1["mismatch"]["summed"] <= 1e-3)}
+    if not gate["passed"]:
+        log("PER-SUBHALO GATE FAILED at BoundMassPrecision 1.0 or summed")
+    sweep = None
+    if args.warm_min_sweep:
+        sweep = {}
+        for wm in [int(x) for x in args.warm_min_sweep.split(",")]:
+            row = {}
+            for tag, p, rf, ms_b in (("p0.995", prm, ref, ms_plain_b), ("p1.0", prm1, ref1, ms_plain1)):
+                def step_wm():
+                    ctx.features(p, fetch=False)
+                    ctx.propose(net, args.threshold, count=False, min_particles=wm)
+                    ctx.run(p, hbtx.WARMSTART | gflag)
+                ms_wm = timed_steps(ctx, args.steps, args.warmup, step_wm)
+                mm = learned.mismatch_summary(pop, ctx.fetch(), rf)
+                row[tag] = {"learned_ms_per_step": float(np.mean(ms_wm)),
+                            "learned_gain": float(np.mean(ms_b) / np.mean(ms_wm)),
+                            "subhalos_within": mm["subhalos_within"],
+                            "bound_mass_in_subhalos_within": mm["bound_mass_in_subhalos_within"],
+                            "summed": mm["summed"]}
+            sweep[str(wm)] = row
+            log(f"warm_min {wm}: " + ", ".join(f"{t} {r['learned_ms_per_step']:.1f} ms gain "
+                                                f"{r['learned_gain']:.2f} within {r['subhalos_within']:.5f}"
+                                                for t, r in row.items()))
+    # work counters: one extra, untimed run of the counting instantiation (same results, deterministic work),
+    # from the headline's proposal
+    ctx.features(prm, fetch=False)
+    ctx.propose(net, args.threshold, count=False, min_particles=args.warm_min)
     ctx.run(prm, hbtx.WARMSTART | hbtx.COUNT)
     st = ctx.stats()
     ctx.run(prm, hbtx.COUNT)
@@ -388,6 +421,7 @@ def main():
                      "kernel_avg_launch_ms": walk_ms / max(1, rounds), "flop_per_step": flops},
         "bound_mass_mismatch_vs_plain": mismatch,
         "mismatch_detail": mism,
+        "per_subhalo_gate": gate,
         "profiles_and_shape": {"ms": prof_ms, "subhalos_with_r200": int(prof["has_r200"].sum()),
                                "note": "host wall time of hbtx_batch_profiles, results copied to the host"},
         "bound_mass_precision_1": prec1,
@@ -403,13 +437,19 @@ def main():
         "walk_stats": {k: v for k, v in st.items() if not k.startswith("unused")},
     }
 
+    if sweep is not None:
+        line["warm_min_sweep"] = sweep
+
     if not args.no_plain:
-        est, sample = plain_gpu_estimate(hbtx, pop, prm, args.plain_sample)
-        plain_rate = pop.n / est
-        line["plain_gpu"] = {"value": plain_rate, "unit": "particles/s", "seconds_estimated": est,
-                             "sample": sample}
+        secs, plain_out = plain_gpu_full(hbtx, pop, prm)
+        plain_rate = pop.n / secs
+        same = bool(np.array_equal(plain_out["order"], ref["order"])
+                    and np.array_equal(plain_out["result"]["nbound"], ref["result"]["nbound"]))
+        line["plain_gpu"] = {"value": plain_rate, "unit": "particles/s", "seconds": secs,
+                             "sample": f"all {pop.nsub} subhaloes of snapshot {args.snapshot}, one at a time",
+                             "identical_to_batched": same}
         line["speedup_vs_plain"] = value / plain_rate
-        log(f"plain GPU estimate {est:.2f}s -> {plain_rate:.3e} particles/s")
+        log(f"plain GPU loop {secs:.2f}s -> {plain_rate:.3e} particles/s (identical to batched: {same})")
 
     if not args.no_tree:
         # one tree over a 2e6-particle NFW host with 50 embedded subhaloes, potential at every particle

@@ -31,7 +31,18 @@ enum : int32_t
   MODE_IDLE = 0,
   MODE_FULL = 1,  // all current candidates are targets, sources = candidates (or subsample)
   MODE_CORR = 2,  // correction: sources = particles removed in the previous round
-  MODE_REFINE = 3 // centre refinement: self-binding energies of the most bound particles
+  MODE_REFINE = 3, // centre refinement: self-binding energies of the most bound particles
+  MODE_ADD = 4     // after an accepted warm round: the potential of the bound particles from
+                   // outside the proposal is added to every candidate's energy
+};
+// Learned warm start, per subhalo
+enum : int32_t
+{
+  WARM_OFF = 0,     // plain iteration
+  WARM_PENDING = 1, // the warm round (proposed set as sources, every particle a target) comes next
+  WARM_CORR = 2,    // accepted: the first correction (sources: proposed particles the warm round removed)
+  WARM_DONE = 3,    // the correction sequence continues as in the plain iteration
+  WARM_ADD = 4      // accepted with bound particles outside the proposal: a MODE_ADD round first
 };
 
 // Device copy of the unbinding parameters (derived values precomputed on the host
@@ -137,6 +148,9 @@ HBTX_DEV float4 const_load_f4u(const uint2 *p)
   const v4i a = *(const c4 *)(uintptr_t)p;
   return make_float4(__int_as_float(a[0]), __int_as_float(a[1]), __int_as_float(a[2]), __int_as_float(a[3]));
 }
+// Longest walk-record span of one tree: the walk keeps its position as an int32
+// byte offset and steps at most 24 bytes past the span's end.
+#define WALK_MAX_SPAN_UNITS ((0x7fffffff - 64) / 8)
 HBTX_HD int32_t wrec_unit(int32_t j, int32_t cscan_j)
 {
   return 2 * j + 3 * cscan_j;
@@ -180,6 +194,19 @@ HBTX_HD float ordered_to_float(uint32_t u)
 HBTX_DEV float nearest_f(float x, float box, float half)
 {
   return (x > half) ? (x - box) : ((x < -half) ? (x + box) : x);
+}
+// NEAREST(a - b) rounded once, as the reference gets it from float operands in
+// double arithmetic (src/gravity_tree.cpp:108-133).  The float difference s and
+// its rounding error e (Knuth's two-sum: s + e = a - b exactly) are formed first;
+// the box shift of s is exact (|s| lies within a factor of two of the box when it
+// applies), and e is added last.  A plain float a - b loses up to half an ulp of
+// the box size when the operands sit on opposite sides of the box edge.
+HBTX_DEV float nearest_diff_f(float a, float b, float box, float half)
+{
+  const float s = a - b;
+  const float bv = s - a;
+  const float e = (a - (s - bv)) + (-b - bv);
+  return nearest_f(s, box, half) + e;
 }
 HBTX_DEV double nearest_d(double x, double box, double half)
 {

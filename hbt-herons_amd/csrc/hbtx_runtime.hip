@@ -149,7 +149,7 @@ struct Batch
   DArr<int32_t> status, corr, iters, mode, treemode, nlast, nbound, nrem, nuns, ntgt, nsrc, src_off, ent_begin,
     ent_end, nitems, bnd_cnt, bnd_tr, first_idx, seg_begin, seg_len, rbegin, rlen, type_cnt, first_tracer, aux32;
   DArr<float> mlast, mbound, ref, oldref, mb, upscale, ref_in, mb_in, minit;
-  DArr<int32_t> warm, npro, reset_;
+  DArr<int32_t> warm, npro, nadd, reset_;
   DArr<float> feat, logit;
   // active-slot tiles of the current round and their prefix sums
   DArr<int4> stiles;
@@ -238,6 +238,7 @@ struct hbtx_ctx
   int walk_chunks = 8;          // item chunks of the walk (one per XCD)
   int64_t direct_cutoff = 1024; // sources at or below use the exact sum when params ask for the tuned value
   int walk_waves = 32;          // resident walk waves per CU (grid of the persistent walk)
+  int walk_lanes = 0;           // 1: every lane walks its own target (lane_walk), 0: wave-union walk
   std::string tune_source = "built-in defaults";
   // scratch
   DArr<int32_t> part32;
@@ -588,7 +589,7 @@ void batch_reserve(hbtx_ctx *c, Batch &B)
   for (auto *a : i32s)
     a->alloc(ns + 1);
   B.type_cnt.alloc(6 * (int64_t)ns + 6);
-  DArr<int32_t> *w3[] = {&B.warm, &B.npro, &B.reset_};
+  DArr<int32_t> *w3[] = {&B.warm, &B.npro, &B.nadd, &B.reset_};
   for (auto *a : w3)
     a->alloc(ns + 1);
   DArr<float> *f1[] = {&B.mlast, &B.mbound, &B.minit};
@@ -665,6 +666,7 @@ View make_view(Batch &B, int64_t p0, int64_t n, int32_t s0, int32_t nsub, bool w
   S.max_active_n = B.misc.p + 6;
   S.warm = B.warm.p + s0;
   S.npro = B.npro.p + s0;
+  S.nadd = B.nadd.p + s0;
   S.reset = B.reset_.p + s0;
   S.minit = B.minit.p + s0;
   PartArrays &Q = V.Q;
@@ -851,7 +853,7 @@ void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, cons
   // (a captured round cannot wait for the count: its buffers are sized for the
   // largest trees the batch can make, and the moments pass covers every depth)
   k_tree_emit<<<blocks_for(n, 256, 8 * 1024), 256, 0, st>>>(T);
-  k_sub_entries<<<gs, 256, 0, st>>>(V.S, B.cscan.p, B.ent_cap);
+  k_sub_entries<<<gs, 256, 0, st>>>(V.S, B.cscan.p, B.ent_cap, B.misc.p + 2);
   k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, B.ent_cap, B.misc.p + 2);
   if (level >= 2)
   {
@@ -926,19 +928,17 @@ void round_walk_launch(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   const int64_t maxitems = std::min<int64_t>(V.n / HBTX_WAVE + V.nsub + 1, c->walk_items_hint);
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxitems, (int64_t)c->walk_waves * c->num_cu));
   HIPCHK(hipEventRecord(c->wev0, st));
-  if (P.periodic)
+  const int variant = (P.periodic ? 4 : 0) | (c->count_work ? 2 : 0) | (c->walk_lanes ? 1 : 0);
+  switch (variant)
   {
-    if (c->count_work)
-      k_walk_energy<true, true><<<grid, 64, 0, st>>>(W);
-    else
-      k_walk_energy<true, false><<<grid, 64, 0, st>>>(W);
-  }
-  else
-  {
-    if (c->count_work)
-      k_walk_energy<false, true><<<grid, 64, 0, st>>>(W);
-    else
-      k_walk_energy<false, false><<<grid, 64, 0, st>>>(W);
+  case 0: k_walk_energy<false, false, false><<<grid, 64, 0, st>>>(W); break;
+  case 1: k_walk_energy<false, false, true><<<grid, 64, 0, st>>>(W); break;
+  case 2: k_walk_energy<false, true, false><<<grid, 64, 0, st>>>(W); break;
+  case 3: k_walk_energy<false, true, true><<<grid, 64, 0, st>>>(W); break;
+  case 4: k_walk_energy<true, false, false><<<grid, 64, 0, st>>>(W); break;
+  case 5: k_walk_energy<true, false, true><<<grid, 64, 0, st>>>(W); break;
+  case 6: k_walk_energy<true, true, false><<<grid, 64, 0, st>>>(W); break;
+  default: k_walk_energy<true, true, true><<<grid, 64, 0, st>>>(W); break;
   }
   HIPCHK(hipEventRecord(c->wev1, st));
   HIPCHK(hipGetLastError());
@@ -1296,6 +1296,15 @@ bool ensure_round_graphs(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int
   return true;
 }
 
+// The overflow word of a round: bit 1 the entry buffers were too small, bit 2 a
+// tree's records exceed the walk's int32 byte offsets.
+void check_overflow(int32_t ov)
+{
+  REQUIRE(!(ov & 2), HBTX_E_CAPACITY,
+          "a subhalo's octree exceeds the walk's 2 GiB record span (int32 byte offsets)");
+  REQUIRE(ov == 0, HBTX_E_CAPACITY, "tree entry capacity exceeded");
+}
+
 // Iterate one view to convergence. Returns rounds used.  graphs: rounds after
 // the first replay the captured head and tail around the walk.
 int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, bool warm, bool graphs)
@@ -1341,7 +1350,7 @@ int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, 
     // the next round's level: subhaloes only leave the active set
     const int64_t maxn = h_misc[6];
     level = (maxn <= P.direct_cutoff) ? 2 : (maxn <= SORT_SMALL_MAX ? 1 : 0);
-    REQUIRE(h_misc[2] == 0, HBTX_E_CAPACITY, "tree entry capacity exceeded");
+    check_overflow(h_misc[2]);
     if (h_misc[0] == 0)
       break;
   }
@@ -1349,15 +1358,35 @@ int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, 
   return round;
 }
 
-void upload_batch(hbtx_ctx *c, Batch &B, int64_t n, const float *pos, const float *vel, const float *mass,
-                  const float *u, const int32_t *type, const int64_t *id, int64_t nsub, const int64_t *offsets,
-                  const float *ref_pos, const float *ref_vel, const float *mb_pos, const float *mb_vel)
+// The batch layout limits, checked before anything touches the device:
+// particle slots and subhalo indices are int32, and the walk addresses one
+// subhalo's records by int32 byte offsets, so a subhalo's tree (16 bytes a
+// particle, 24 a cell) must stay below 2 GiB.  Particles alone fix the bound
+// below; the cells are counted on the device once the tree exists (the span
+// check of k_sub_entries).
+void check_layout(int64_t n, int64_t nsub, const int64_t *offsets)
 {
   REQUIRE(n >= 0 && nsub >= 1, HBTX_E_INVALID, "need n >= 0 and nsub >= 1");
   REQUIRE(n < (int64_t)0x20000000, HBTX_E_INVALID, "at most 2^29 particles per batch");
   REQUIRE(nsub < (int64_t)0x7fffffff, HBTX_E_INVALID, "too many subhaloes");
-  REQUIRE(pos && vel && mass && offsets && ref_pos && ref_vel, HBTX_E_INVALID, "null input array");
+  REQUIRE(offsets, HBTX_E_INVALID, "null offsets");
   REQUIRE(offsets[0] == 0 && offsets[nsub] == n, HBTX_E_INVALID, "offsets must start at 0 and end at n");
+  for (int64_t s = 0; s < nsub; s++)
+  {
+    REQUIRE(offsets[s + 1] >= offsets[s], HBTX_E_INVALID, "offsets must be non-decreasing");
+    REQUIRE(offsets[s + 1] - offsets[s] <= HBTX_MAX_SUBHALO_PARTICLES, HBTX_E_CAPACITY,
+            "subhalo " + std::to_string(s) + " has " + std::to_string(offsets[s + 1] - offsets[s]) +
+              " particles: its octree would exceed the walk's 2 GiB record span (at most " +
+              std::to_string((long long)HBTX_MAX_SUBHALO_PARTICLES) + " particles per subhalo)");
+  }
+}
+
+void upload_batch(hbtx_ctx *c, Batch &B, int64_t n, const float *pos, const float *vel, const float *mass,
+                  const float *u, const int32_t *type, const int64_t *id, int64_t nsub, const int64_t *offsets,
+                  const float *ref_pos, const float *ref_vel, const float *mb_pos, const float *mb_vel)
+{
+  check_layout(n, nsub, offsets);
+  REQUIRE(pos && vel && mass && ref_pos && ref_vel, HBTX_E_INVALID, "null input array");
   B.n = n;
   B.nsub = (int32_t)nsub;
   B.has_type = type != nullptr;
@@ -1369,7 +1398,6 @@ void upload_batch(hbtx_ctx *c, Batch &B, int64_t n, const float *pos, const floa
   B.max_count = 0;
   for (int64_t s = 0; s < nsub; s++)
   {
-    REQUIRE(offsets[s + 1] >= offsets[s], HBTX_E_INVALID, "offsets must be non-decreasing");
     B.h_count[s] = (int32_t)(offsets[s + 1] - offsets[s]);
     B.max_count = std::max(B.max_count, B.h_count[s]);
   }
@@ -1546,7 +1574,7 @@ void build_tree(hbtx_ctx *c, const hbtx_params &prm, int64_t n, const float *pos
   int32_t h_misc[4];
   HIPCHK(hipMemcpyAsync(h_misc, B.misc.p, sizeof(h_misc), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  REQUIRE(h_misc[2] == 0, HBTX_E_CAPACITY, "tree entry capacity exceeded");
+  check_overflow(h_misc[2]);
   const int32_t ne = h_misc[3];
   T->ctx = c;
   T->n = n;
@@ -1713,6 +1741,8 @@ void load_tuning(hbtx_ctx *c)
       c->walk_chunks = (int)std::min<int64_t>(WALK_MAX_CHUNKS, v);
     if (json_int(txt, "walk_waves", v) && v >= 1)
       c->walk_waves = (int)std::min<int64_t>(64, v);
+    if (json_int(txt, "walk_lanes", v))
+      c->walk_lanes = v ? 1 : 0;
     c->tune_source = path;
   }
   if (const char *e = getenv("HBTX_DIRECT_CUTOFF"))
@@ -1728,6 +1758,11 @@ void load_tuning(hbtx_ctx *c)
   if (const char *e = getenv("HBTX_WALK_WAVES"))
   {
     c->walk_waves = std::min(64, std::max(1, atoi(e)));
+    c->tune_source = "environment";
+  }
+  if (const char *e = getenv("HBTX_WALK_LANES"))
+  {
+    c->walk_lanes = atoi(e) ? 1 : 0;
     c->tune_source = "environment";
   }
 }
@@ -1802,6 +1837,28 @@ int hbtx_device_sync(hbtx_ctx *c)
   if (!c)
     return HBTX_E_INVALID;
   API_GUARD(c, HIPCHK(hipStreamSynchronize(c->stream)));
+}
+
+int hbtx_check_layout(int64_t n, int64_t nsub, const int64_t *offsets, char *msg, int32_t msg_len)
+{
+  std::string m;
+  int rc = HBTX_OK;
+  try
+  {
+    check_layout(n, nsub, offsets);
+  }
+  catch (const Error &e)
+  {
+    rc = e.code;
+    m = e.what();
+  }
+  if (msg && msg_len > 0)
+  {
+    const size_t k = std::min<size_t>(m.size(), (size_t)msg_len - 1);
+    std::memcpy(msg, m.data(), k);
+    msg[k] = 0;
+  }
+  return rc;
 }
 
 int hbtx_batch_load(hbtx_ctx *c, int64_t n, const float *pos, const float *vel, const float *mass, const float *u,

@@ -14,7 +14,7 @@ struct PointArgs
   int32_t eb, ee;          // walk-record unit range
   int32_t treemode;
   const uint2 *wt;         // walk records
-  float *phi;              // specific potential, scattered to the target's input index
+  float *phi;              // specific potential, scattered to the target's input index, or null
   double *phi_d;           // optional double output
   int32_t *counter;
   unsigned long long *stats;
@@ -66,7 +66,8 @@ __global__ __launch_bounds__(64) void k_points_potential(PointArgs A)
     {
       const float self = x.w / A.P.eps;
       const double phi = ((double)self + (double)acc.s - (double)acc.c) * A.P.G_over_a;
-      A.phi[ti] = (float)phi;
+      if (A.phi)
+        A.phi[ti] = (float)phi;
       if (A.phi_d)
         A.phi_d[ti] = phi;
     }

@@ -88,7 +88,7 @@ __global__ void k_init_mass_finish(SubArrays S, DevParams P, const double *red, 
     // warm round in the proposed set's own centre-of-mass frame; subhaloes whose
     // proposal is near the disruption limit (MinNumPartOfSub) iterate plainly, since
     // whether they survive depends on the iteration's path
-    S.warm[s] = 1;
+    S.warm[s] = WARM_PENDING;
     S.npro[s] = (int)v[1];
     if (!P.periodic)
       for (int k = 0; k < 3; k++)

@@ -46,7 +46,7 @@ struct TreeArgs
   uint2 *wt;                 // walk records (units of 8 bytes)
   int32_t *cell_u;           // per entry index: unit of a cell's walk record
   int32_t cap;               // entry capacity (walk records: 3 cap units)
-  int32_t *overflow;         // set to 1 if entries would exceed cap
+  int32_t *overflow;         // bit 1 set if entries would exceed cap
   DevParams P;
 };
 
@@ -125,7 +125,7 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
       A.wt[uj + 1] = make_uint2(__float_as_uint(x.z), __float_as_uint(m));
     }
     else
-      *A.overflow = 1;
+      atomicOr(A.overflow, 1);
     return;
   }
   const char2 lv = A.lev[j];
@@ -134,7 +134,7 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
   const int q = P + nn;
   if (q >= A.cap)
   {
-    *A.overflow = 1;
+    atomicOr(A.overflow, 1);
     return;
   }
   A.ent[q] = Entry{x.x, x.y, x.z, m, 0.f, (q + 1 - t0) * (int)sizeof(Entry), 0.f, q + 1};

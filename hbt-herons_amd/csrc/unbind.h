@@ -32,9 +32,10 @@ struct SubArrays
   int32_t *first_idx;       // smallest candidate index (periodic reference particle)
   int32_t *active_count;    // [1]
   int32_t *max_active_n;    // [1] particles of the largest subhalo still active after the round
-  // learned warm start: warm = 0 off, 1 first (proposal-sourced) round pending, 2 done;
-  // npro = proposed particles; reset = proposal rejected, restart from all particles
-  int32_t *warm, *npro, *reset;
+  // learned warm start: warm = WARM_* state; npro = proposed particles; nadd = bound
+  // particles outside the proposal (re-admitted by the warm round); reset = proposal
+  // rejected, restart from all particles
+  int32_t *warm, *npro, *nadd, *reset;
   float *minit;             // initial bound mass (all particles)
 };
 
@@ -91,8 +92,9 @@ __global__ void k_init_subs(SubArrays S, const float *ref_in, const float *mb_in
   const int n = S.off[s + 1] - S.off[s];
   S.status[s] = (n == 0) ? ST_CONVERGED : ST_ACTIVE;
   S.corr[s] = 0;
-  S.warm[s] = 0;
+  S.warm[s] = WARM_OFF;
   S.npro[s] = 0;
+  S.nadd[s] = 0;
   S.reset[s] = 0;
   S.iters[s] = 0;
   S.nbound[s] = n;
@@ -137,7 +139,14 @@ __global__ void k_plan(SubArrays S, DevParams P, const int32_t *round_p)
   }
   const int nb = S.nbound[s];
   int ns;
-  if (S.corr[s])
+  if (S.warm[s] == WARM_ADD)
+  {
+    // after an accepted warm round: the bound particles from outside the proposal
+    // are the sources; frames, nlast and the correction state stay as they are
+    S.mode[s] = MODE_ADD;
+    ns = S.nadd[s];
+  }
+  else if (S.corr[s])
   {
     S.mode[s] = MODE_CORR;
     ns = S.nrem[s];
@@ -152,8 +161,9 @@ __global__ void k_plan(SubArrays S, DevParams P, const int32_t *round_p)
     else
       ns = (int)(P.mss + S.nuns[s]);
   }
-  S.nlast[s] = nb;
-  if (S.warm[s] == 1)
+  if (S.warm[s] != WARM_ADD)
+    S.nlast[s] = nb;
+  if (S.warm[s] == WARM_PENDING)
   {
     // warm round: every particle is a target, the proposed set is the source set,
     // and convergence is measured against the proposed set
@@ -197,9 +207,19 @@ __global__ void k_mark_sources(SubArrays S, PartArrays Q, DevParams P, const int
       int f = 0;
       float sm = 0.f;
       const int st = Q.state[p];
-      if (md == MODE_FULL && st == 0 && warm == 1)
+      if (md == MODE_FULL && st == 0 && warm == WARM_PENDING)
       {
         if (Q.proposal[p])
+        {
+          f = 1;
+          sm = Q.posm[p].w;
+        }
+      }
+      else if (md == MODE_ADD)
+      {
+        // the bound particles the warm round found outside the proposal (each
+        // also a target: its self term is cancelled)
+        if (st == 0 && !Q.proposal[p])
         {
           f = 1;
           sm = Q.posm[p].w;
@@ -218,9 +238,13 @@ __global__ void k_mark_sources(SubArrays S, PartArrays Q, DevParams P, const int
           sm = P.dm_only ? m * up[1] : (donot_subsample(P, t) ? m : m * up[t]);
         }
       }
+      else if (md == MODE_CORR && warm == WARM_CORR)
+        // first correction after a warm round: the proposed particles the warm round
+        // removed (the proposal was its source set), and any the MODE_ADD round
+        // (round 2, if it ran) left unbound
+        f = ((st == 1 && Q.proposal[p]) || (st == 2 && round == 3)) ? 1 : 0;
       else if (md == MODE_CORR && st == round - 1)
-        // after a warm round only the proposed particles were sources
-        f = (round == 2 && warm == 2 && Q.proposal) ? (int)Q.proposal[p] : 1;
+        f = 1;
       flag[p] = f;
       Q.selfm[p] = sm;
     }
@@ -443,17 +467,26 @@ __global__ void k_targets_from_sources(SubArrays S, PartArrays Q, const int32_t 
   }
 }
 
-__global__ void k_sub_entries(SubArrays S, const int32_t *cscan, int32_t cap)
+// Per subhalo, the walk-record units of its tree, clamped to the record buffer (3
+// cap units) so that an overflowing round, reported to the host, never reads past
+// the buffer.  The walk addresses a tree's records by int32 byte offsets: a tree
+// over WALK_MAX_SPAN_UNITS units raises overflow bit 2 (reported as a capacity
+// error) and walks nothing.
+__global__ void k_sub_entries(SubArrays S, const int32_t *cscan, int32_t cap, int32_t *overflow)
 {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S.nsub)
     return;
   const int b = S.src_off[s], e = b + S.nsrc[s];
-  // clamped so that an overflowing tree (reported to the host) is never read past the buffer
-  // walk-record units, clamped to the record buffer (3 cap units): an
-  // overflowing round, reported to the host, never reads past the buffer
-  S.ent_begin[s] = min(wrec_unit(b, cscan[b]), 3 * cap);
-  S.ent_end[s] = min(wrec_unit(e, cscan[e]), 3 * cap);
+  const int ub = min(wrec_unit(b, cscan[b]), 3 * cap);
+  int ue = min(wrec_unit(e, cscan[e]), 3 * cap);
+  if (ue - ub > WALK_MAX_SPAN_UNITS)
+  {
+    atomicOr(overflow, 2);
+    ue = ub;
+  }
+  S.ent_begin[s] = ub;
+  S.ent_end[s] = ue;
 }
 
 // Entries of the round's trees: e_total = min(E, cap), overflow raised when E
@@ -467,7 +500,7 @@ __global__ void k_entry_total(const int32_t *s_total, const int32_t *cscan, int3
     const int E = S + cscan[S];
     *e_total = min(E, cap);
     if (E > cap)
-      *overflow = 1;
+      atomicOr(overflow, 1);
   }
 }
 
@@ -493,8 +526,9 @@ __global__ void k_items_emit(SubArrays S, const int32_t *order, const int32_t *i
 }
 
 // Partition flags over slots: low 32 bits = bound, high 32 bits = bound tracer.
-// In a warm round only proposed particles may stay; bound particles outside the
-// proposal are removed with the unbound ones and counted in warm_np[s].
+// In a warm round every particle with E < 0 stays, proposed or not; the bound
+// particles outside the proposal are counted in warm_np[s] (their potential is
+// added to the others' energies by the MODE_ADD round that follows).
 __global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, const int4 *tiles, const int32_t *ntiles_p,
                                   int64_t *flag, int32_t *warm_np)
 {
@@ -503,7 +537,7 @@ __global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, const 
     const int4 tl = tiles[t_];
     const int s = tl.x;
     const int end = min(tl.z, S.off[s] + S.ntgt[s]);
-    const bool warm = warm_np && S.warm[s] == 1;
+    const bool warm = warm_np && S.warm[s] == WARM_PENDING;
     for (int q = tl.y + threadIdx.x; q < tl.z; q += blockDim.x)
     {
       int64_t f = 0;
@@ -512,10 +546,8 @@ __global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, const 
       if (q < end && Q.esl[q] < 0.f)
       {
         const int p = (warm || Q.ptype) ? Q.cur[q] : 0;
-        if (warm && !Q.proposal[p])
-          outside = true;
-        else
-          f = 1 | ((int64_t)(is_tracer(P, ptype_of(Q, p)) ? 1 : 0) << 32);
+        outside = warm && !Q.proposal[p];
+        f = 1 | ((int64_t)(is_tracer(P, ptype_of(Q, p)) ? 1 : 0) << 32);
       }
       flag[q] = f;
       // the tile is one subhalo's: one counter update per wave
@@ -576,37 +608,56 @@ __global__ void k_round_decide(SubArrays S, DevParams P, const int64_t *sub_tota
   const int64_t d = sub_total[s];
   const int nb = (int)(d & 0xffffffffll);
   const int ntr = P.dm_only ? nb : (int)(d >> 32);
+  if (S.mode[s] == MODE_ADD)
+  {
+    // The added potential only deepens the energies, so every candidate normally
+    // stays; any that did not are sources of the correction that follows, with the
+    // proposed particles the warm round removed.  Frames and the correction state
+    // are left as the warm round set them.
+    S.nrem[s] += S.nbound[s] - nb;
+    S.nbound[s] = nb;
+    S.bnd_cnt[s] = nb;
+    S.bnd_tr[s] = ntr;
+    S.warm[s] = WARM_CORR; // (the disruption test follows the correction)
+    return;
+  }
   const int nlast = S.nlast[s];
   S.nbound[s] = nb;
   S.nrem[s] = nlast - nb;
   S.bnd_cnt[s] = nb;
   S.bnd_tr[s] = ntr;
-  if (S.warm[s] == 1)
+  if (S.warm[s] == WARM_CORR)
+    S.warm[s] = WARM_DONE; // this was the first correction after the warm round
+  if (S.warm[s] == WARM_PENDING)
   {
     // Proposal verification: the proposed set's exact potential is evaluated at
-    // every particle.  Accepted when at most 0.1% as many particles outside the
-    // proposal as inside it come out bound (those are removed, as the plain
-    // iteration removes the particles it finds unbound on its way), at least half
-    // of the proposal is bound and the subhalo is well above the disruption limit
-    // (twice MinNumPartOfSub and MinNumTracerPartOfSub); the correction sequence
-    // then continues from the proposed set.  Otherwise the plain iteration restarts
-    // from every particle.
-    const int np = warm_np ? warm_np[s] : 0;
+    // every particle, and every particle with E < 0 stays, proposed or not.
+    // Accepted when the bound particles from outside the proposal number at most
+    // 0.1% of the bound set, at least half of the proposal is bound and the
+    // subhalo is well above the disruption limit (twice MinNumPartOfSub and
+    // MinNumTracerPartOfSub).  The outside ones' potential is then added to every
+    // candidate's energy (a MODE_ADD round) and the correction sequence continues
+    // from the bound set.  Otherwise the plain iteration restarts from every
+    // particle.
+    const int np = warm_np ? warm_np[s] : 0; // bound, outside the proposal
+    const int nin = nb - np;                 // bound and proposed
     const int n = S.off[s + 1] - S.off[s];
     const int npro = S.npro[s];
-    S.warm[s] = 2;
-    const bool accept = (int64_t)nb >= 2 * P.min_part && (int64_t)ntr >= 2 * P.min_tracer &&
-                        2 * (int64_t)nb >= (int64_t)npro && (int64_t)np * 1000 <= (int64_t)nb;
+    const bool accept = (int64_t)nin >= 2 * P.min_part && (int64_t)ntr >= 2 * P.min_tracer &&
+                        2 * (int64_t)nin >= (int64_t)npro && (int64_t)np * 1000 <= (int64_t)nb;
+    S.warm[s] = WARM_DONE;
     if (accept)
     {
       // the warm round's frame is the proposal's: at least one more round follows so
       // that the final energies are taken in the bound set's frame
-      const int nrem = npro - nb; // proposed but unbound: the next correction's sources
+      const int nrem = npro - nin; // proposed but unbound: the first correction's sources
       S.nrem[s] = nrem;
       S.nlast[s] = n;
+      S.nadd[s] = np;
       if (nrem < nb)
       {
         S.corr[s] = 1;
+        S.warm[s] = (np > 0) ? WARM_ADD : WARM_CORR;
         for (int k = 0; k < 6; k++)
           S.oldref[6 * s + k] = S.ref[6 * s + k];
       }
@@ -674,7 +725,7 @@ __global__ void k_warm_keys(SubArrays S, PartArrays Q, int64_t n, uint64_t *keys
   GRID_STRIDE(q, n)
   {
     const int s = S.sub_of[q];
-    if (S.warm[s] == 1 && S.mode[s] == MODE_FULL && S.treemode[s] && S.nsrc[s] < S.ntgt[s])
+    if (S.warm[s] == WARM_PENDING && S.mode[s] == MODE_FULL && S.treemode[s] && S.nsrc[s] < S.ntgt[s])
     {
       const float4 x = Q.posm[Q.cur[q]];
       const double *c = S.cube + 4 * s;
@@ -691,7 +742,7 @@ __global__ void k_warm_segments(SubArrays S, int32_t *seg_begin, int32_t *seg_le
   if (s >= S.nsub)
     return;
   seg_begin[s] = S.off[s];
-  seg_len[s] = (S.warm[s] == 1 && S.mode[s] == MODE_FULL && S.treemode[s] && S.nsrc[s] < S.ntgt[s])
+  seg_len[s] = (S.warm[s] == WARM_PENDING && S.mode[s] == MODE_FULL && S.treemode[s] && S.nsrc[s] < S.ntgt[s])
                  ? (S.off[s + 1] - S.off[s])
                  : 0;
 }
@@ -929,7 +980,8 @@ __global__ void k_round_finish(SubArrays S, DevParams P, const double *red)
   S.mbound[s] = (float)M;
   const int64_t nb = S.nbound[s];
   const int64_t nlast = S.nlast[s];
-  if (!S.reset[s] && nb >= nlast * P.prec)
+  // (no convergence before the MODE_ADD round has added the outside particles' potential)
+  if (!S.reset[s] && S.mode[s] != MODE_ADD && S.warm[s] != WARM_ADD && nb >= nlast * P.prec)
     S.status[s] = ST_CONVERGED;
   else
   {

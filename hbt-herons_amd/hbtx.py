@@ -145,6 +145,7 @@ def lib():
         L.hbtx_tree_free.restype = None
         L.hbtx_direct_potential.argtypes = [P, C.POINTER(Params), i64, P, P, i64, P, P, P]
         L.hbtx_segmented_sort.argtypes = [P, i64, P, P, i64, P]
+        L.hbtx_check_layout.argtypes = [i64, i64, P, C.c_char_p, C.c_int32]
         L.hbtx_batch_profiles.argtypes = [P, C.POINTER(Params), P]
         L.hbtx_profiles.argtypes = [P, C.POINTER(Params), i64, P, P, i64, P, P, P, P]
         L.hbtx_reassign_targets.argtypes = [P, C.POINTER(Params), i64, P, P, P, i64, P, C.c_int32, C.c_float, P,
@@ -166,6 +167,15 @@ def lib():
         L.hbtx_set_tuning.argtypes = [P, i64, C.c_int32]
         _lib = L
     return _lib
+
+
+def check_layout(n, offsets):
+    """hbtx_check_layout: (status, message) for a batch of n particles in
+    len(offsets) - 1 subhaloes; needs no device."""
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    buf = C.create_string_buffer(512)
+    rc = lib().hbtx_check_layout(int(n), len(off) - 1, off.ctypes.data_as(C.c_void_p), buf, 512)
+    return rc, buf.value.decode()
 
 
 def header_symbols():

@@ -32,6 +32,13 @@ extern "C" {
 #define HBTX_E_CAPACITY (-4)
 #define HBTX_E_NOT_IMPLEMENTED (-5)
 
+/* Largest subhalo a batch accepts: the walk addresses one subhalo's octree
+ * records (16 bytes a particle, 24 a cell) by int32 byte offsets, so a tree must
+ * stay under 2 GiB.  Larger subhaloes are rejected with HBTX_E_CAPACITY before
+ * upload; a tree whose cells push it over the span is reported as HBTX_E_CAPACITY
+ * by the round that builds it. */
+#define HBTX_MAX_SUBHALO_PARTICLES ((int64_t)(1 << 27) - 64)
+
 /* Flags for hbtx_unbind_batch / hbtx_batch_run */
 #define HBTX_BATCHED 0u          /* all subhaloes advance together, device-side control (default) */
 #define HBTX_PLAIN 1u            /* one subhalo at a time, host-driven iteration (Subhalo_t::Unbind loop) */
@@ -123,6 +130,14 @@ int hbtx_unbind_batch(hbtx_ctx *ctx, const hbtx_params *prm, int64_t n, const fl
                       const int64_t *offsets, const float *ref_pos, const float *ref_vel, const float *mb_pos,
                       const float *mb_vel, uint32_t flags, int64_t *out_order, float *out_energy,
                       float *out_potential, hbtx_sub_result *out_res);
+
+/* Host-only check of a batch layout (no device needed), the checks
+ * hbtx_batch_load and hbtx_unbind_batch make before touching the device:
+ * offsets[0] = 0, non-decreasing, offsets[nsub] = n, n < 2^29 and every subhalo
+ * at most HBTX_MAX_SUBHALO_PARTICLES.  Returns HBTX_OK, HBTX_E_INVALID or
+ * HBTX_E_CAPACITY; msg (may be NULL) receives the reason.  No reference
+ * counterpart: the reference's Unbind has no size limit. */
+int hbtx_check_layout(int64_t n, int64_t nsub, const int64_t *offsets, char *msg, int32_t msg_len);
 
 /* Device-resident batch: upload once, run (timed on the context stream with HIP
  * events, inputs already in HBM), fetch.  Same array meanings as above. */

@@ -772,16 +772,18 @@ void unbind(const Cfg &c, SubState &S, std::vector<int64_t> &final_order, bool w
       if (c.p.thermal)
         E[i].E += q.u;
     }
+    // every particle with E < 0 stays, proposed or not (np_out: bound, outside the proposal)
     int64_t np_out = 0;
     for (int64_t i = 0; i < N; i++)
       if (E[i].E < 0 && !prop[E[i].idx])
         np_out++;
-    auto it = std::stable_partition(E.begin(), E.end(), [&](const PE &a) { return a.E < 0 && prop[a.idx]; });
+    auto it = std::stable_partition(E.begin(), E.end(), [&](const PE &a) { return a.E < 0; });
     const int64_t nb = it - E.begin();
+    const int64_t nin = nb - np_out;
     int64_t ntr = 0;
     for (int64_t i = 0; i < nb; i++)
       ntr += (c.p.dm_only || is_tracer(c, P[E[i].idx])) ? 1 : 0;
-    const bool accept = nb >= 2 * c.p.min_num_part && ntr >= 2 * c.p.min_num_tracer && 2 * nb >= npro &&
+    const bool accept = nin >= 2 * c.p.min_num_part && ntr >= 2 * c.p.min_num_tracer && 2 * nin >= npro &&
                         np_out * 1000 <= nb;
     if (accept)
     {
@@ -789,7 +791,7 @@ void unbind(const Cfg &c, SubState &S, std::vector<int64_t> &final_order, bool w
       for (int64_t i = nb; i < N; i++)
         if (prop[E[i].idx])
           warm_src.push_back(E[i]);
-      const int64_t nrem = npro - nb;
+      const int64_t nrem = npro - nin;
       nbound = nb;
       nlast = N;
       if (nrem < nb)
@@ -799,9 +801,37 @@ void unbind(const Cfg &c, SubState &S, std::vector<int64_t> &final_order, bool w
         old_vel = ref_vel;
       }
       mbound = es.com_frame(ref_pos, ref_vel, nbound);
+      if (correction && np_out > 0)
+      {
+        // the outside particles' potential is added to every candidate's energy
+        // (one more round; frames and the correction state stay)
+        iters++;
+        std::vector<PE> EA;
+        for (int64_t i = 0; i < nb; i++)
+          if (!prop[E[i].idx])
+            EA.push_back(E[i]);
+        ESnap esa(EA.data(), (int64_t)EA.size(), P, c);
+        tree.build(c, esa, (int64_t)EA.size());
+        for (int64_t i = 0; i < nb; i++)
+        {
+          const Particle &q = P[E[i].idx];
+          E[i].E = (float)((double)E[i].E + tree.potential(q.x, prop[E[i].idx] ? 0.f : q.m));
+        }
+        // any candidate the added potential left unbound joins the first correction's sources
+        auto ia = std::stable_partition(E.begin(), E.begin() + nb, [](const PE &a) { return a.E < 0; });
+        const int64_t nb2 = ia - E.begin();
+        if (nb2 < nb)
+        {
+          std::sort(E.begin() + nb2, E.begin() + nb, [](const PE &a, const PE &b) { return a.E < b.E; });
+          for (int64_t i = nb2; i < nb; i++)
+            warm_src.push_back(E[i]);
+          nbound = nb2;
+          mbound = es.com_frame(ref_pos, ref_vel, nbound);
+        }
+      }
       // convergence against every particle: the warm round ends the iteration only
-      // when nearly all particles are bound
-      if (nbound >= nlast * prec)
+      // when nearly all particles are bound and no outside particle was added
+      if (!(correction && np_out > 0) && nbound >= nlast * prec)
       {
         finalize();
         done = true;
@@ -1456,4 +1486,25 @@ extern "C" void oracle_reassign_targets(const ounb_params *prm, int64_t n, const
       }
     }
   }
+}
+
+// SetMassUpscaleFactor (src/subhalo_unbind.cpp:374-391, with GetMassUpscaleFactor
+// :290-317 or GetMassUpscaleFactorPerParticleType :323-361) over particles in
+// E-list order: returns the number of gravity sources and the per-type factors.
+extern "C" int64_t oracle_mass_upscale(const ounb_params *prm, int64_t n, const float *mass, const int32_t *type,
+                                       int64_t nlast, float mlast, int64_t mss, int64_t nuns, float *out_upscale)
+{
+  Cfg c(*prm);
+  std::vector<Particle> P(n);
+  std::vector<PE> E(n);
+  for (int64_t i = 0; i < n; i++)
+  {
+    P[i] = Particle{i, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, mass[i], 0.f, type ? type[i] : 1};
+    E[i] = PE{0.f, i};
+  }
+  ESnap es(E.data(), n, P, c);
+  const int64_t ns = es.set_upscale(nlast, mlast, mss, nuns);
+  for (int k = 0; k < TYPE_MAX; k++)
+    out_upscale[k] = es.upscale[k];
+  return ns;
 }

@@ -96,6 +96,11 @@ int64_t oracle_tree_potential(const ounb_params *prm, int64_t n_src, const float
                               int64_t n_tgt, const float *tgt_pos, const float *tgt_mass, int nthreads,
                               double *out_pot);
 
+/* SetMassUpscaleFactor over n particles in E-list order (mass, type or NULL):
+ * returns the number of gravity sources; out_upscale[6] = per-type factors. */
+int64_t oracle_mass_upscale(const ounb_params *prm, int64_t n, const float *mass, const int32_t *type,
+                            int64_t nlast, float mlast, int64_t mss, int64_t nuns, float *out_upscale);
+
 /* FP64 all-pairs sum with the reference spline kernel (theta -> 0 limit of the walk). */
 void oracle_direct_potential(const ounb_params *prm, int64_t n_src, const float *src_pos, const float *src_mass,
                              int64_t n_tgt, const float *tgt_pos, const float *tgt_mass, int nthreads,

@@ -121,6 +121,9 @@ def lib():
         L.oracle_reassign_targets.restype = None
         L.oracle_reassign_targets.argtypes = [C.POINTER(Params), C.c_int64, P, P, P, C.c_int64, P, C.c_int32,
                                               C.c_float, P]
+        L.oracle_mass_upscale.restype = C.c_int64
+        L.oracle_mass_upscale.argtypes = [C.POINTER(Params), C.c_int64, P, P, C.c_int64, C.c_float, C.c_int64,
+                                          C.c_int64, P]
         L.oracle_profiles.restype = None
         L.oracle_profiles.argtypes = [C.POINTER(Params), C.c_int64, P, P, C.c_int64, P, P, P, P]
         _lib = L
@@ -248,3 +251,14 @@ def reassign_targets(pos, is_tracer, sub, group_offsets, k=10, rguess=None, para
     lib().oracle_reassign_targets(C.byref(prm), len(tr), _p(pos), _p(tr), _p(sb), len(go) - 1, _p(go), int(k),
                                   float(rguess), _p(out))
     return out
+
+
+def mass_upscale(mass, ptype, nlast, mlast, mss, nuns, params=None):
+    """SetMassUpscaleFactor over particles in E-list order: (sources, factors[6])."""
+    prm = params if isinstance(params, Params) else make_params(**(params or {}))
+    mass = _f32(mass)
+    pt = None if ptype is None else np.ascontiguousarray(ptype, dtype=np.int32)
+    out = np.zeros(6, np.float32)
+    ns = lib().oracle_mass_upscale(C.byref(prm), len(mass), _p(mass), _p(pt), int(nlast), float(mlast), int(mss),
+                                   int(nuns), _p(out))
+    return int(ns), out

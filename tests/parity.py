@@ -77,3 +77,33 @@ def order_is_reference_layout(out, pop, max_sample_size=0, refine_fraction=0.1):
         if sorted(out["order"][a:b].tolist()) != list(range(b - a)):
             return False
     return True
+
+
+def order_differences_are_ties(a, b, pop, rtol=2e-5):
+    """Compare the final orders of a and b slot by slot.  Where they differ, the
+    two particles must be energy ties in b (within rtol of the subhalo's largest
+    |E|): the reference sorts with the unstable std::sort
+    (src/subhalo_unbind.cpp:673,710), so the order of ties is unpinned, and the
+    GPU's FP32 energies can swap particles closer than its rounding.  Disrupted
+    subhaloes keep their input order on both sides.  Returns the number of slots
+    that differ; raises AssertionError on any other difference."""
+    ndiff = 0
+    for s in range(pop.nsub):
+        lo, hi = pop.offsets[s], pop.offsets[s + 1]
+        if hi == lo:
+            continue  # an orphan: no particles on either side
+        oa, ob = a["order"][lo:hi], b["order"][lo:hi]
+        da, db = bool(a["result"][s]["disrupted"]), bool(b["result"][s]["disrupted"])
+        assert da == db, f"subhalo {s}: disrupted {da} vs {db}"
+        if da:
+            assert np.array_equal(oa, ob), f"subhalo {s}: disrupted order differs"
+            continue
+        eb = np.empty(hi - lo, np.float64)
+        eb[ob] = b["energy"][lo:hi]
+        scale = rtol * max(np.abs(eb).max(), 1e-30)
+        k = np.where(oa != ob)[0]
+        ndiff += len(k)
+        bad = k[np.abs(eb[oa[k]] - eb[ob[k]]) > scale]
+        assert len(bad) == 0, (f"subhalo {s}: slots {bad[:5]} hold {oa[bad[:5]]} vs {ob[bad[:5]]} with energies "
+                               f"{eb[oa[bad[:5]]]} vs {eb[ob[bad[:5]]]}")
+    return ndiff

@@ -4,8 +4,10 @@ and warm-started unbinding vs the plain iteration and the CPU oracle.
 Gates: gradients within 2e-3 of the largest gradient component of the FP64 oracle
 evaluated at the same BF16-rounded weights (FP32 accumulation over <= 4096
 samples); warm-started bound sets within 0.1% of bound mass of the plain GPU
-unbinding and of the oracle, whatever the proposal; the trained network lowers
-the rounds and the walk work against the plain iteration and an untrained network.
+unbinding and of the oracle, summed, whatever the proposal; with the headline's
+warm-start threshold (subhaloes of at least 30,000 particles), within 0.1% per
+subhalo as well; the trained network lowers the rounds and the walk work against
+the plain iteration and an untrained network.
 """
 import numpy as np
 import pytest
@@ -209,3 +211,44 @@ def test_trained_network_reduces_work(ctx, evolved, trained):
     assert w_warm < 0.9 * w_plain
     assert w_warm < 0.9 * w_un
     assert i_warm < i_plain
+
+
+WARM_MIN = 30000  # the headline's warm-start threshold (bench.py --warm-min)
+
+
+@pytest.fixture(scope="module")
+def evolved_large():
+    # subhaloes at and above the warm-start threshold, evolved like the headline population
+    pop = synth.population(seed=23, nsub=36, nmin=20000, nmax=120000)
+    return synth.evolve(pop, nsnap=8, seed=6)
+
+
+@pytest.fixture(scope="module")
+def trained_large(ctx, evolved_large):
+    prm = hbtx.workload_params()
+    X, y, _ = learned.collect_training_set(ctx, evolved_large, prm, train_snaps=range(6))
+    net, loss = learned.train(ctx, X, y, epochs=4)
+    return net
+
+
+@pytest.mark.parametrize("k,prec", [(6, 0.995), (7, 0.995), (6, 1.0), (7, 1.0)])
+def test_warm_start_per_subhalo_gate(ctx, evolved_large, trained_large, k, prec):
+    """The plain iteration judges every subhalo: with the headline's threshold the
+    warm start's bound mass is within 0.1% of the plain iteration's in each
+    subhalo (the symmetric difference over the plain bound mass), not only summed.
+    Subhaloes below the threshold run the plain iteration and match exactly."""
+    net = trained_large
+    prm = hbtx.workload_params(bound_mass_precision=prec)
+    pop = evolved_large[k]
+    prev = plain_and_prev(ctx, evolved_large, k, prm)
+    warm, _, _, npro = learned.warm_unbind(ctx, pop, prev, net, prm, min_particles=WARM_MIN)
+    plain = ctx.unbind(pop, prm)
+    per = learned.mismatch_per_subhalo(pop, warm, plain)
+    counts = pop.counts()
+    warmed = counts >= WARM_MIN
+    print(f"prec {prec} snapshot {k}: {int(warmed.sum())} warm-started subhaloes, {npro} proposed; "
+          f"per-subhalo mismatch max {per.max():.2e}, summed {learned.bound_mass_mismatch(pop, warm, plain):.2e}")
+    assert warmed.sum() >= 10 and npro > 0
+    assert np.all(per <= 1e-3), (np.where(per > 1e-3)[0], per[per > 1e-3])
+    assert np.all(per[~warmed] == 0.0)
+    assert np.array_equal(warm["result"]["disrupted"], plain["result"]["disrupted"])

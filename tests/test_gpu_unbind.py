@@ -11,7 +11,7 @@ import pytest
 import hbtx
 import pyoracle as O
 import synth
-from parity import bound_mass_agreement, energy_errors, order_is_reference_layout
+from parity import bound_mass_agreement, energy_errors, order_differences_are_ties, order_is_reference_layout
 
 pytestmark = pytest.mark.gpu
 
@@ -132,7 +132,8 @@ def test_edge_cases(ctx):
     g, r = gpu["result"], ref["result"]
     assert np.array_equal(g["nbound"], r["nbound"])
     assert np.array_equal(g["disrupted"], r["disrupted"])
-    assert np.array_equal(gpu["order"], ref["order"]) or True  # ties in energy are unpinned
+    # the same order slot by slot, up to energy ties (std::sort is unstable)
+    order_differences_are_ties(gpu, ref, pop)
     # all-unbound subhalo: huge velocities -> disrupted, frame = most-bound input
     v2 = vel * 1e4
     pop2 = synth.Population(pos, v2, pop.mass, off, pop.pid)
@@ -170,7 +171,26 @@ def test_subsampling_and_refinement_match_oracle(ctx, hydro):
     frac, err = check_parity(gpu, ref, pop, e_tol=1e-4, max_sample_size=1000)
     g, r = gpu["result"], ref["result"]
     assert np.array_equal(g["disrupted"], r["disrupted"])
-    # the refined most bound particle (slot 0) agrees for nearly all subhaloes
+    # the refined most bound particle (slot 0) is the oracle's, or ties with it in
+    # the self-binding energy the refinement sorts by (RefineBindingEnergyOrder,
+    # src/subhalo_unbind.cpp:395-425: the tree over the first max(MaxSampleSize,
+    # 0.1 Nbound) particles, true masses, the final frame)
     alive = np.where((g["nbound"] > 0) & (r["nbound"] > 0))[0]
-    same = [gpu["order"][pop.offsets[s]] == ref["order"][pop.offsets[s]] for s in alive]
-    assert np.mean(same) >= 0.95, np.mean(same)
+    opr = oracle_params(prm)
+    differ = 0
+    for s in alive:
+        a = pop.offsets[s]
+        pg, pr = gpu["order"][a], ref["order"][a]
+        if pg == pr:
+            continue
+        differ += 1
+        nb = int(r[s]["nbound"])
+        size = min(nb, max(1000, int(np.float32(0.1) * np.float32(nb))))
+        top = a + ref["order"][a:a + size]
+        both = a + np.array([pg, pr])
+        assert set(both.tolist()) <= set(top.tolist()), f"subhalo {s}: refined centre outside the refined set"
+        phi, _ = O.tree_potential(pop.pos[top], pop.mass[top], pop.pos[both], pop.mass[both], params=opr)
+        dv = pop.vel[both] - r[s]["com_vel"]
+        e = 0.5 * (dv.astype(np.float64) ** 2).sum(1) + phi
+        assert abs(e[0] - e[1]) <= 2e-5 * abs(e[1]), (s, e)
+    print(f"refined centres: {differ} of {len(alive)} differ from the oracle's, all energy ties")

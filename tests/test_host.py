@@ -74,3 +74,28 @@ def test_population_target_size():
     pop = synth.population(seed=3, target_particles=300000, nmax=20000)
     assert 300000 <= pop.n < 330000
     assert np.all(np.diff(pop.offsets) >= 20)
+
+
+def test_layout_check_rejects_subhaloes_beyond_the_walk_span():
+    # the walk keeps a tree's position as an int32 byte offset (16 bytes a particle
+    # record): a subhalo above HBTX_MAX_SUBHALO_PARTICLES is a capacity error before
+    # any upload, the largest accepted one passes
+    txt = open(hbtx.HEADER).read()
+    assert "HBTX_MAX_SUBHALO_PARTICLES ((int64_t)(1 << 27) - 64)" in txt
+    nmax = (1 << 27) - 64
+    assert 16 * nmax < 2 ** 31
+    rc, msg = hbtx.check_layout(nmax, [0, nmax])
+    assert rc == hbtx.OK and msg == ""
+    rc, msg = hbtx.check_layout(nmax + 11, [0, 10, nmax + 11])
+    assert rc == hbtx.E_CAPACITY and "subhalo 1" in msg and "2 GiB" in msg
+    rc, _ = hbtx.check_layout(2 * nmax, [0, nmax, 2 * nmax])
+    assert rc == hbtx.OK
+
+
+def test_layout_check_rejects_malformed_offsets():
+    assert hbtx.check_layout(10, [0, 5, 10])[0] == hbtx.OK
+    assert hbtx.check_layout(10, [0, 6, 5, 10])[0] == hbtx.E_INVALID   # decreasing
+    assert hbtx.check_layout(10, [1, 5, 10])[0] == hbtx.E_INVALID      # not starting at 0
+    assert hbtx.check_layout(10, [0, 5, 9])[0] == hbtx.E_INVALID       # not ending at n
+    assert hbtx.check_layout(1 << 29, [0, 1 << 29])[0] == hbtx.E_INVALID  # batch limit
+    assert hbtx.check_layout(0, [0])[0] == hbtx.E_INVALID              # no subhalo

@@ -33,11 +33,14 @@ def frame(x, v, m, idx):
 
 def brute_force_warm(pop, prec, prop):
     """Warm start rules: the first iteration = proposed sources, every particle a target, in
-    the proposal's frame; bound = E < 0 and proposed; accepted if at most 0.1% as
-    many outsiders as insiders are bound, at least half the proposal is bound and
-    the bound count is at least 2 MinNumPartOfSub; the correction sources are then
-    the proposed, removed particles and the convergence test runs against every
-    particle.  Rejected or ineligible proposals run the plain loop.
+    the proposal's frame; bound = E < 0, proposed or not; accepted if the bound
+    particles from outside the proposal are at most 0.1% of the bound set, at least
+    half the proposal is bound and the bound proposed count is at least
+    2 MinNumPartOfSub.  Then, when a correction follows and outside particles are
+    bound, one more iteration adds their exact potential to every candidate's
+    energy (frames unchanged); the correction sources are the proposed, removed
+    particles (and any the added potential left unbound), and the convergence test
+    runs against every particle.  Rejected or ineligible proposals run the plain loop.
     Returns (bound set, energies, iterations, accepted)."""
     x = pop.pos.astype(np.float64)
     v = pop.vel.astype(np.float32)
@@ -57,17 +60,28 @@ def brute_force_warm(pop, prec, prop):
         dv = v - rv
         k2 = (dv * dv).sum(1).astype(np.float32)
         E[:] = (k2.astype(np.float64) * 0.5 + phi_exact(x, m, allp, pidx, np.where(prop, m, 0))).astype(np.float32)
-        bound = allp[(E < 0) & prop]
+        bound = allp[E < 0]
         outside = int(((E < 0) & ~prop).sum())
-        if (len(bound) >= 2 * MIN_PART and 2 * len(bound) >= len(pidx) and outside * 1000 <= len(bound)):
+        nin = len(bound) - outside
+        if (nin >= 2 * MIN_PART and 2 * nin >= len(pidx) and outside * 1000 <= len(bound)):
             accepted = True
             removed = pidx[E[pidx] >= 0]
             if len(removed) < len(bound):
                 corr = True
                 old = (rp.copy(), rv.copy())
             rp, rv = frame(x, v, m, bound)
+            if corr and outside > 0:
+                iters += 1
+                ext = bound[~prop[bound]]
+                E[bound] = (E[bound].astype(np.float64)
+                            + phi_exact(x, m, bound, ext, np.where(prop[bound], 0, m[bound]))).astype(np.float32)
+                left = bound[E[bound] >= 0]
+                if len(left):
+                    removed = np.concatenate([removed, left])
+                    bound = bound[E[bound] < 0]
+                    rp, rv = frame(x, v, m, bound)
             cand = bound
-            if np.float32(len(bound)) >= np.float32(n) * np.float32(prec):
+            if not (corr and outside > 0) and np.float32(len(bound)) >= np.float32(n) * np.float32(prec):
                 return set(bound.tolist()), E, iters, accepted
     if not accepted:
         rp, rv = pop.ref_pos[0].copy(), pop.ref_vel[0].copy()
@@ -151,3 +165,40 @@ def test_ineligible_proposal_is_the_plain_iteration():
             assert np.array_equal(warm["order"][a:b], plain["order"][a:b])
             assert np.array_equal(warm["energy"][a:b], plain["energy"][a:b])
             assert warm["result"][s] == plain["result"][s]
+
+
+@pytest.fixture(scope="module")
+def larger_halo():
+    pop = synth.single_nfw(7, n_halo=1500, n_inter=300)
+    plain = O.unbind_batch(pop, O.default_params(theta=0.0), nthreads=1)
+    nb = int(plain["result"][0]["nbound"])
+    bound = np.zeros(pop.n, bool)
+    bound[plain["order"][:nb]] = True
+    return pop, bound, plain["order"][:nb]
+
+
+@pytest.mark.parametrize("prec", [0.995, 1.0])
+def test_warm_oracle_readmits_bound_particles_outside_the_proposal(larger_halo, prec):
+    # one bound particle left out of the proposal (0.07% of the bound set): the
+    # warm round keeps it, its potential is added in one more iteration, and the
+    # result matches the brute-force rules (the particle stays bound)
+    pop, bound, order = larger_halo
+    assert len(order) >= 1000
+    prop = bound.copy()
+    missing = int(order[len(order) // 3])
+    prop[missing] = False
+    r = O.unbind_batch(pop, O.default_params(theta=0.0, bound_mass_precision=prec), nthreads=1,
+                       proposal=prop.astype(np.uint8))
+    ref, E, iters, accepted = brute_force_warm(pop, prec, prop)
+    assert accepted
+    assert missing in ref
+    nb = int(r["result"][0]["nbound"])
+    assert set(r["order"][:nb].tolist()) == ref
+    assert int(r["result"][0]["iterations"]) == iters
+    np.testing.assert_allclose(r["energy"][:nb], E[r["order"][:nb]], rtol=2e-5, atol=1e-3)
+    # the same with the exact proposal: one iteration fewer (no outside particle to add)
+    r2 = O.unbind_batch(pop, O.default_params(theta=0.0, bound_mass_precision=prec), nthreads=1,
+                        proposal=bound.astype(np.uint8))
+    ref2, _, iters2, _ = brute_force_warm(pop, prec, bound)
+    assert int(r2["result"][0]["iterations"]) == iters2 == iters - 1
+    assert set(r2["order"][:int(r2["result"][0]["nbound"])].tolist()) == ref2
