@@ -7,15 +7,17 @@
 // must give identical subhaloes; the level-batched result is written for
 // tests/test_gpu_hierarchy.py to compare with the CPU oracle.
 //
-// usage: hierarchy_test <input.bin> <output.bin>
+// usage: hierarchy_test <input.bin> <output.bin> [<snapshot_output.bin>]
 //   input : int64 nsub, n, nroots, nedges, merge_trapped, direct_cutoff;
 //           int64 offsets[nsub+1]; float pos[n][3], vel[n][3], mass[n];
 //           int64 id[n]; float ref_pos[nsub][3], ref_vel[nsub][3];
 //           int64 track[nsub], sink[nsub]; int64 child_off[nsub+1],
-//           child[nedges]; int64 roots[nroots]
+//           child[nedges]; int64 roots[nroots]; optionally int64 num_sub_old
+//           (subhaloes from there on were born this snapshot; default nsub)
 //   output: per subhalo int64 nbound, sink, nparticles, mostbound_id;
 //           int32 snapshot_of_sink, snapshot_of_death; float core[8];
-//           then every subhalo's particle ids in order
+//           then every subhalo's particle ids in order; snapshot_output: the
+//           same for the subhaloes of hbtx::RefineSnapshot
 //   stdout: "levels L merged M identical 1 truncate_ok 1 ... snapshot_identical 1"
 //   (the last: hbtx::RefineSnapshot against SubhaloSnapshot_t::RefineParticles'
 //   control flow restated with the same per-subhalo recursion)
@@ -150,9 +152,35 @@ static bool same(const Subhalo_t &a, const Subhalo_t &b, int64_t s)
   return ok;
 }
 
+// per subhalo fields, then every subhalo's particle ids (the output format above)
+static void write_subs(const char *path, const std::vector<Subhalo_t> &subs)
+{
+  FILE *o = fopen(path, "wb");
+  if (!o)
+    return;
+  for (auto &S : subs)
+  {
+    const int64_t np = (int64_t)S.Particles.size();
+    wr(o, &S.Nbound, 1);
+    wr(o, &S.SinkTrackId, 1);
+    wr(o, &np, 1);
+    wr(o, &S.MostBoundParticleId, 1);
+    wr(o, &S.SnapshotOfSink, 1);
+    wr(o, &S.SnapshotOfDeath, 1);
+    wr(o, S.CoreComovingPosition.data(), 3);
+    wr(o, S.CorePhysicalVelocity.data(), 3);
+    wr(o, &S.CoreComovingSigmaR, 1);
+    wr(o, &S.CorePhysicalSigmaV, 1);
+  }
+  for (auto &S : subs)
+    for (auto &p : S.Particles)
+      wr(o, &p.Id, 1);
+  fclose(o);
+}
+
 int main(int argc, char **argv)
 {
-  if (argc != 3)
+  if (argc != 3 && argc != 4)
     return 2;
   FILE *f = fopen(argv[1], "rb");
   if (!f)
@@ -169,6 +197,9 @@ int main(int argc, char **argv)
       !rd(f, track.data(), nsub) || !rd(f, sink.data(), nsub) || !rd(f, coff.data(), nsub + 1) ||
       !rd(f, child.data(), nedges) || !rd(f, roots.data(), nroots))
     return 2;
+  int64_t num_sub_old = nsub;
+  if (!rd(f, &num_sub_old, 1))
+    num_sub_old = nsub;
   fclose(f);
   auto make = [&](std::vector<Subhalo_t> &subs) {
     subs.assign(nsub, Subhalo_t());
@@ -240,25 +271,7 @@ int main(int argc, char **argv)
       for (size_t i = 0; trunc_ok && i < T.Particles.size(); i++)
         trunc_ok = T.Particles[i].Id == A[s].Particles[i].Id;
     }
-    FILE *o = fopen(argv[2], "wb");
-    for (auto &S : A)
-    {
-      const int64_t np = (int64_t)S.Particles.size();
-      wr(o, &S.Nbound, 1);
-      wr(o, &S.SinkTrackId, 1);
-      wr(o, &np, 1);
-      wr(o, &S.MostBoundParticleId, 1);
-      wr(o, &S.SnapshotOfSink, 1);
-      wr(o, &S.SnapshotOfDeath, 1);
-      wr(o, S.CoreComovingPosition.data(), 3);
-      wr(o, S.CorePhysicalVelocity.data(), 3);
-      wr(o, &S.CoreComovingSigmaR, 1);
-      wr(o, &S.CorePhysicalSigmaV, 1);
-    }
-    for (auto &S : A)
-      for (auto &p : S.Particles)
-        wr(o, &p.Id, 1);
-    fclose(o);
+    write_subs(argv[2], A);
     // PrintSubhaloStatistics' counts; subhalo 0 born this snapshot
     A[0].SnapshotOfBirth = g_snap;
     const hbtx::SubhaloCounts sc = hbtx::SubhaloStatistics(A, g_snap);
@@ -296,7 +309,9 @@ int main(int argc, char **argv)
     make(C);
     make(D);
     double secs = 0.0;
-    hbtx::RefineSnapshot(ctx, g_prm, C, groups, heads, field, nsub, g_snap, g_merge, 3.0, &secs);
+    hbtx::RefineSnapshot(ctx, g_prm, C, groups, heads, field, num_sub_old, g_snap, g_merge, 3.0, &secs);
+    if (argc == 4)
+      write_subs(argv[3], C);
     for (size_t h = 0; h < groups.size(); h++)
     {
       auto &nests = D[groups[h][0]].NestedSubhalos;
@@ -306,6 +321,8 @@ int main(int argc, char **argv)
       nests.resize(old);
     }
     for (int64_t x : field)
+      hbtx::Unbind(ctx, g_prm, D[x], g_snap);
+    for (int64_t x = num_sub_old; x < nsub; x++)
       hbtx::Unbind(ctx, g_prm, D[x], g_snap);
     for (auto &S : D)
     {

@@ -140,3 +140,45 @@ def run(H, params, snapshot_id, merge_trapped):
     for r in H.roots:
         recursive_unbind(nodes, r, params, snapshot_id, merge_trapped)
     return nodes
+
+
+def truncate_source(nd, relax=3.0):
+    """Subhalo_t::TruncateSource (src/subhalo_unbind.cpp:842-852)."""
+    ns = nd.nbound if nd.nbound <= 1 else int(nd.nbound * relax)
+    ns = min(ns, len(nd.mass))
+    nd.pos, nd.vel, nd.mass, nd.pid = nd.pos[:ns], nd.vel[:ns], nd.mass[:ns], nd.pid[:ns]
+
+
+def refine_snapshot(H, params, snapshot_id, merge_trapped, groups, heads, field, num_sub_old, relax=3.0):
+    """SubhaloSnapshot_t::RefineParticles (src/subhalo_unbind.cpp:855-919, the
+    exclusive-mass build) restated sequentially: per host halo with members, the
+    central's nested list gets the other heads for its RecursiveUnbind; then every
+    field subhalo and every subhalo born this snapshot (index >= num_sub_old) is
+    unbound on its own; then every source is truncated."""
+    nodes = nodes_from(H)
+    for h, members in enumerate(groups):
+        if len(members) == 0:
+            continue
+        central = nodes[members[0]]
+        old = len(central.children)
+        central.children = central.children + list(heads[h][1:])
+        recursive_unbind(nodes, members[0], params, snapshot_id, merge_trapped)
+        central.children = central.children[:old]
+    for s in field:
+        unbind(nodes[s], params, snapshot_id)
+    for s in range(num_sub_old, len(nodes)):
+        unbind(nodes[s], params, snapshot_id)
+    for nd in nodes:
+        truncate_source(nd, relax)
+    return nodes
+
+
+def subhalo_statistics(snap_death, snap_sink, snap_birth, snapshot_id):
+    """SubhaloSnapshot_t::PrintSubhaloStatistics' counts (src/subhalo_unbind.cpp:940-980):
+    (sunk, disrupted, new, fake)."""
+    d, k, b = np.asarray(snap_death), np.asarray(snap_sink), np.asarray(snap_birth)
+    sunk = int(np.sum((d == snapshot_id) & (k == snapshot_id)))
+    disrupted = int(np.sum((b < snapshot_id) & (d == snapshot_id) & (k == -1)))
+    fake = int(np.sum((b == snapshot_id) & (d == snapshot_id)))
+    new = int(np.sum((b == snapshot_id) & (d == -1)))
+    return sunk, disrupted, new, fake
