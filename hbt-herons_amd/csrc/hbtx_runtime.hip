@@ -28,6 +28,7 @@
 #include <sstream>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -59,6 +60,33 @@ struct Error : public std::runtime_error
     if (!(cond))                                                                                                     \
       throw Error(code, msg);                                                                                        \
   } while (0)
+
+// The array of hbtx_direct_potential_noalloc: packed source records, packed
+// targets and the work counter, HBTX_NOALLOC_MAX_POINTS of each.  Every GPU holds
+// its own copy at its own address until the process ends; all calls on one GPU
+// share it (calls that overlap on one GPU race).
+constexpr size_t kScratchTgt = 16 * (size_t)HBTX_NOALLOC_MAX_POINTS;
+constexpr size_t kScratchCtr = 32 * (size_t)HBTX_NOALLOC_MAX_POINTS;
+constexpr size_t kScratchBytes = kScratchCtr + 256;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on the current device, looked up once per device
+// (host code never uses the array's name as an address).
+void *scratch_address(int device)
+{
+  static std::mutex mu;
+  static void *addr[256] = {};
+  REQUIRE(device >= 0 && device < 256, HBTX_E_INVALID, "device index beyond 255");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device])
+  {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    REQUIRE(p != nullptr, HBTX_E_HIP, "no device address for the scratch array");
+    addr[device] = p;
+  }
+  return addr[device];
+}
 
 // Advanced by every device allocation and release: kernels captured in a graph
 // hold buffer addresses, so a graph is valid only for the generation it was
@@ -238,6 +266,7 @@ struct hbtx_ctx
   int walk_chunks = 8;          // item chunks of the walk (one per XCD)
   int64_t direct_cutoff = 1024; // sources at or below use the exact sum when params ask for the tuned value
   int walk_waves = 32;          // resident walk waves per CU (grid of the persistent walk)
+  void *scratch = nullptr;      // this device's address of the no-allocation entry point's array
   int walk_lanes = 0;           // 1: every lane walks its own target (lane_walk), 0: wave-union walk
   std::string tune_source = "built-in defaults";
   // scratch
@@ -1787,6 +1816,7 @@ int hbtx_create(hbtx_ctx **out, int device)
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     c->num_cu = std::max(1, prop.multiProcessorCount);
+    c->scratch = scratch_address(device);
     load_tuning(c);
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));
@@ -2073,6 +2103,59 @@ int hbtx_direct_potential(hbtx_ctx *c, const hbtx_params *prm, int64_t n_src, co
     HIPCHK(hipSetDevice(c->device));
     build_tree(c, *prm, n_src, src_pos, src_mass, &T, 0);
     tree_points(c, &T, n_tgt, tgt_pos, tgt_mass, out_pot);
+  });
+}
+
+int hbtx_direct_potential_noalloc(hbtx_ctx *c, const hbtx_params *prm, int64_t n_src, const float *d_src_pos,
+                                  const float *d_src_mass, int64_t n_tgt, const float *d_tgt_pos,
+                                  const float *d_tgt_mass, double *d_out_pot)
+{
+  if (!c || !prm)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    REQUIRE(n_src >= 1 && n_tgt >= 0, HBTX_E_INVALID, "need at least one source and n_tgt >= 0");
+    REQUIRE(n_src <= HBTX_NOALLOC_MAX_POINTS && n_tgt <= HBTX_NOALLOC_MAX_POINTS, HBTX_E_CAPACITY,
+            "hbtx_direct_potential_noalloc takes at most HBTX_NOALLOC_MAX_POINTS (" +
+              std::to_string(HBTX_NOALLOC_MAX_POINTS) + ") sources and targets; got " + std::to_string(n_src) +
+              " and " + std::to_string(n_tgt));
+    REQUIRE(d_src_pos && d_src_mass, HBTX_E_INVALID, "null source array");
+    REQUIRE(n_tgt == 0 || (d_tgt_pos && d_out_pot), HBTX_E_INVALID, "null target or output array");
+    REQUIRE(c->scratch, HBTX_E_HIP, "the scratch array has no device address");
+    HIPCHK(hipSetDevice(c->device));
+    hbtx_params p = *prm;
+    p.direct_cutoff = (int64_t)1 << 40;
+    p.max_sample_size = 0;
+    const DevParams P = make_dev_params(p);
+    hipStream_t st = c->stream;
+    unsigned char *base = static_cast<unsigned char *>(c->scratch);
+    uint2 *wt = reinterpret_cast<uint2 *>(base);
+    float4 *tgt = reinterpret_cast<float4 *>(base + kScratchTgt);
+    int32_t *counter = reinterpret_cast<int32_t *>(base + kScratchCtr);
+    k_pack_records<<<blocks_for(n_src), 256, 0, st>>>(d_src_pos, d_src_mass, (int32_t)n_src, wt);
+    if (n_tgt > 0)
+    {
+      k_pack_points<<<blocks_for(n_tgt), 256, 0, st>>>(d_tgt_pos, d_tgt_mass, (int32_t)n_tgt, tgt);
+      k_fill_i32<<<1, 64, 0, st>>>(counter, 1, 0); // the previous call's count stays in the array
+      PointArgs A;
+      A.tgt = tgt;
+      A.tgt_order = nullptr;
+      A.ntgt = (int32_t)n_tgt;
+      A.eb = 0;
+      A.ee = (int32_t)(2 * n_src);
+      A.treemode = 0;
+      A.wt = wt;
+      A.phi = nullptr;
+      A.phi_d = d_out_pot;
+      A.counter = counter;
+      A.stats = nullptr;
+      A.P = P;
+      const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(n_tgt / HBTX_WAVE + 1, 32 * c->num_cu));
+      if (P.periodic)
+        k_points_potential<true, false><<<grid, 64, 0, st>>>(A);
+      else
+        k_points_potential<false, false><<<grid, 64, 0, st>>>(A);
+    }
+    HIPCHK(hipGetLastError());
   });
 }
 

@@ -93,4 +93,22 @@ __global__ void k_point_keys(const float4 *tgt, int32_t n, const double *cube, u
   }
 }
 
+// Caller-owned device arrays packed into the direct sum's records {x, y, z, m}
+// (the layout the tree build emits for a direct-sum source set) and into the
+// walk's target points {x, y, z, target mass}.
+__global__ void k_pack_records(const float *pos, const float *mass, int32_t n, uint2 *wt)
+{
+  GRID_STRIDE(i, n)
+  {
+    wt[2 * i] = make_uint2(__float_as_uint(pos[3 * i]), __float_as_uint(pos[3 * i + 1]));
+    wt[2 * i + 1] = make_uint2(__float_as_uint(pos[3 * i + 2]), __float_as_uint(mass[i]));
+  }
+}
+
+__global__ void k_pack_points(const float *pos, const float *mass, int32_t n, float4 *out)
+{
+  GRID_STRIDE(i, n)
+  out[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], mass ? mass[i] : 0.f);
+}
+
 } // namespace hbtx

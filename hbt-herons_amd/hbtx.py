@@ -144,6 +144,7 @@ def lib():
         L.hbtx_tree_free.argtypes = [P]
         L.hbtx_tree_free.restype = None
         L.hbtx_direct_potential.argtypes = [P, C.POINTER(Params), i64, P, P, i64, P, P, P]
+        L.hbtx_direct_potential_noalloc.argtypes = [P, C.POINTER(Params), i64, P, P, i64, P, P, P]
         L.hbtx_segmented_sort.argtypes = [P, i64, P, P, i64, P]
         L.hbtx_check_layout.argtypes = [i64, i64, P, C.c_char_p, C.c_int32]
         L.hbtx_batch_profiles.argtypes = [P, C.POINTER(Params), P]

@@ -223,6 +223,25 @@ int hbtx_direct_potential(hbtx_ctx *ctx, const hbtx_params *prm, int64_t n_src, 
                           const float *src_mass, int64_t n_tgt, const float *tgt_pos, const float *tgt_mass,
                           double *out_pot);
 
+/* Exact direct-sum potential without allocation: the same results as
+ * hbtx_direct_potential, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Inputs and the output are
+ * DEVICE arrays of the context's device: d_src_pos float[n_src][3], d_src_mass
+ * float[n_src] (non-negative), d_tgt_pos float[n_tgt][3], d_tgt_mass float[n_tgt]
+ * or NULL (targetMass: its self term is removed), d_out_pot double[n_tgt].
+ * Staging (the packed sources and targets, the work counter) lives in one
+ * file-scope device array of the library (about 32 MiB a GPU, held until the
+ * process ends), so n_src and n_tgt are at most HBTX_NOALLOC_MAX_POINTS
+ * (HBTX_E_CAPACITY beyond).  The call only enqueues work on the context's
+ * stream: synchronize it (hbtx_device_sync) before reading d_out_pot.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (several
+ * contexts, several host threads) race. */
+#define HBTX_NOALLOC_MAX_POINTS (1 << 20)
+int hbtx_direct_potential_noalloc(hbtx_ctx *ctx, const hbtx_params *prm, int64_t n_src, const float *d_src_pos,
+                                  const float *d_src_mass, int64_t n_tgt, const float *d_tgt_pos,
+                                  const float *d_tgt_mass, double *d_out_pot);
+
 /* ---------------------------------------------------------------------------
  * Learned warm start (no counterpart in the reference; it only proposes the
  * starting set: the proposal is verified by an exact potential computation and
