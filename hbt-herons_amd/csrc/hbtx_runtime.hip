@@ -267,7 +267,6 @@ struct hbtx_ctx
   int64_t direct_cutoff = 1024; // sources at or below use the exact sum when params ask for the tuned value
   int walk_waves = 32;          // resident walk waves per CU (grid of the persistent walk)
   void *scratch = nullptr;      // this device's address of the no-allocation entry point's array
-  int walk_lanes = 0;           // 1: every lane walks its own target (lane_walk), 0: wave-union walk
   std::string tune_source = "built-in defaults";
   // scratch
   DArr<int32_t> part32;
@@ -957,17 +956,19 @@ void round_walk_launch(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   const int64_t maxitems = std::min<int64_t>(V.n / HBTX_WAVE + V.nsub + 1, c->walk_items_hint);
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxitems, (int64_t)c->walk_waves * c->num_cu));
   HIPCHK(hipEventRecord(c->wev0, st));
-  const int variant = (P.periodic ? 4 : 0) | (c->count_work ? 2 : 0) | (c->walk_lanes ? 1 : 0);
-  switch (variant)
+  if (P.periodic)
   {
-  case 0: k_walk_energy<false, false, false><<<grid, 64, 0, st>>>(W); break;
-  case 1: k_walk_energy<false, false, true><<<grid, 64, 0, st>>>(W); break;
-  case 2: k_walk_energy<false, true, false><<<grid, 64, 0, st>>>(W); break;
-  case 3: k_walk_energy<false, true, true><<<grid, 64, 0, st>>>(W); break;
-  case 4: k_walk_energy<true, false, false><<<grid, 64, 0, st>>>(W); break;
-  case 5: k_walk_energy<true, false, true><<<grid, 64, 0, st>>>(W); break;
-  case 6: k_walk_energy<true, true, false><<<grid, 64, 0, st>>>(W); break;
-  default: k_walk_energy<true, true, true><<<grid, 64, 0, st>>>(W); break;
+    if (c->count_work)
+      k_walk_energy<true, true><<<grid, 64, 0, st>>>(W);
+    else
+      k_walk_energy<true, false><<<grid, 64, 0, st>>>(W);
+  }
+  else
+  {
+    if (c->count_work)
+      k_walk_energy<false, true><<<grid, 64, 0, st>>>(W);
+    else
+      k_walk_energy<false, false><<<grid, 64, 0, st>>>(W);
   }
   HIPCHK(hipEventRecord(c->wev1, st));
   HIPCHK(hipGetLastError());
@@ -1770,8 +1771,6 @@ void load_tuning(hbtx_ctx *c)
       c->walk_chunks = (int)std::min<int64_t>(WALK_MAX_CHUNKS, v);
     if (json_int(txt, "walk_waves", v) && v >= 1)
       c->walk_waves = (int)std::min<int64_t>(64, v);
-    if (json_int(txt, "walk_lanes", v))
-      c->walk_lanes = v ? 1 : 0;
     c->tune_source = path;
   }
   if (const char *e = getenv("HBTX_DIRECT_CUTOFF"))
@@ -1787,11 +1786,6 @@ void load_tuning(hbtx_ctx *c)
   if (const char *e = getenv("HBTX_WALK_WAVES"))
   {
     c->walk_waves = std::min(64, std::max(1, atoi(e)));
-    c->tune_source = "environment";
-  }
-  if (const char *e = getenv("HBTX_WALK_LANES"))
-  {
-    c->walk_lanes = atoi(e) ? 1 : 0;
     c->tune_source = "environment";
   }
 }

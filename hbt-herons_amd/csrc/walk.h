@@ -154,67 +154,6 @@ HBTX_DEV void tree_walk(const uint2 *wt, int ub, int ue, bool valid, float px, f
   }
 }
 
-// Lane walk: every lane runs its own stackless walk of the same records, at its
-// own position (a unit index in a VGPR), with vector loads.  A lane visits
-// exactly the records of its own preorder walk (the same terms, in the same
-// order, as the wave-union walk gives it), and a wave step serves every active
-// lane, so the wave takes as many steps as its longest lane walk instead of the
-// length of the union of its lanes' walks.  The loads of one step are
-// independent of the opening tests of the other lanes, which the union walk
-// must wait for before its next scalar load.
-template <bool PERIODIC, bool STATS>
-HBTX_DEV void lane_walk(const uint2 *wt, int ub, int ue, bool valid, float px, float py, float pz,
-                        const DevParams &P, Kahan &acc, WalkCounts &wc)
-{
-  const uint2 *tree = wt + ub;
-  const int32_t nu = ue - ub; // units of this tree
-  int32_t u = valid ? 0 : nu; // this lane's record
-  bool more = wave_mask(u < nu) != 0ull;
-  while (more)
-  {
-    float blk = 0.f;
-#pragma unroll
-    for (int k = 0; k < 16; k++)
-    {
-      const bool active = u < nu;
-      const int32_t ul = active ? u : 0; // (idle lanes read the first record; the buffer has spare units)
-      const uint2 w0 = tree[ul], w1 = tree[ul + 1], w2 = tree[ul + 2];
-      const float ex = __uint_as_float(w0.x), ey = __uint_as_float(w0.y), ez = __uint_as_float(w1.x);
-      const bool cell = (int32_t)w1.y < 0; // sign bit of m
-      const float m = __uint_as_float(w1.y & 0x7fffffffu);
-      const float crit2 = cell ? __uint_as_float(w2.x) : 0.f;
-      const float r2 = sep2<PERIODIC>(ex, ey, ez, px, py, pz, P);
-      const bool open = r2 < crit2;
-      const bool accept = active && !open;
-      const float rinv = __builtin_amdgcn_rsqf(r2);
-      float term = -m * rinv;
-      if (wave_mask(accept && r2 < P.h2))
-      {
-        if (STATS)
-          wc.nnear++;
-        const float tn = spline_near(r2, rinv, m, P.hinv);
-        term = (r2 < P.h2) ? tn : term;
-      }
-      blk += accept ? term : 0.f;
-      if (STATS)
-      {
-        wc.steps++;
-        wc.visits += __popcll(wave_mask(active));
-        wc.accepts += __popcll(wave_mask(accept));
-      }
-      // an opened cell's subtree follows its 3 units; an accepted record jumps to its
-      // skip (bytes from the tree's first record; a particle's is the next record)
-      const int32_t next = ul + (cell ? 3 : 2);
-      const int32_t skip = cell ? max((int32_t)w2.y >> 3, next) : next;
-      u = active ? (open ? next : skip) : u;
-      more = wave_mask(u < nu) != 0ull;
-      if (!more)
-        break;
-    }
-    acc.add(blk);
-  }
-}
-
 template <bool PERIODIC>
 HBTX_DEV float direct_term(const uint2 *wt, int i, float px, float py, float pz, const DevParams &P, bool &near)
 {
@@ -274,7 +213,7 @@ HBTX_DEV void rel_vel(const DevParams &P, float x0, float x1, float x2, float v0
   }
 }
 
-template <bool PERIODIC, bool STATS, bool LANE>
+template <bool PERIODIC, bool STATS>
 __global__ __launch_bounds__(64) void k_walk_energy(WalkArgs A)
 {
   const DevParams &P = A.P;
@@ -327,12 +266,7 @@ __global__ __launch_bounds__(64) void k_walk_energy(WalkArgs A)
     if (eb < ee)
     {
       if (A.treemode[s])
-      {
-        if (LANE)
-          lane_walk<PERIODIC, STATS>(A.wt, eb, ee, valid, x.x, x.y, x.z, P, acc, wc);
-        else
-          tree_walk<PERIODIC, STATS>(A.wt, eb, ee, valid, x.x, x.y, x.z, P, acc, wc);
-      }
+        tree_walk<PERIODIC, STATS>(A.wt, eb, ee, valid, x.x, x.y, x.z, P, acc, wc);
       else
       {
         direct_sum<PERIODIC>(A.wt, eb, ee, x.x, x.y, x.z, P, acc, wc);
