@@ -301,8 +301,7 @@ def main():
         log(f"batched (no warm start): {np.mean(ms_plain_b):.2f} ms/step")
 
     def learned_step():
-        ctx.features(prm, fetch=False)
-        ctx.propose(net, args.threshold, count=False, min_particles=args.warm_min)
+        ctx.warm_proposal(net, prm, args.threshold, args.warm_min, count=False)
         ctx.run(prm, hbtx.WARMSTART | gflag)
 
     ms_learned = timed_steps(ctx, args.steps, args.warmup, learned_step)
@@ -310,7 +309,7 @@ def main():
     walk_ms = st_time["walk_ms"]
     rounds = int(st_time["rounds"])
     warm = ctx.fetch()
-    npro, inf_ms = ctx.propose(net, args.threshold, min_particles=args.warm_min)
+    npro, inf_ms = ctx.warm_proposal(net, prm, args.threshold, args.warm_min)
     log(f"batched + learned: {np.mean(ms_learned):.2f} ms/step (walk {walk_ms:.2f} ms, {rounds} rounds, "
         f"{npro} proposed, inference {inf_ms:.2f} ms)")
     if args.learned_only:
@@ -329,8 +328,7 @@ def main():
     ref1 = ctx.fetch()
 
     def learned_step1():
-        ctx.features(prm1, fetch=False)
-        ctx.propose(net, args.threshold, count=False, min_particles=args.warm_min)
+        ctx.warm_proposal(net, prm1, args.threshold, args.warm_min, count=False)
         ctx.run(prm1, hbtx.WARMSTART | gflag)
 
     ms_learned1 = timed_steps(ctx, args.steps, args.warmup, learned_step1)
@@ -355,8 +353,7 @@ def main():
             row = {}
             for tag, p, rf, ms_b in (("p0.995", prm, ref, ms_plain_b), ("p1.0", prm1, ref1, ms_plain1)):
                 def step_wm():
-                    ctx.features(p, fetch=False)
-                    ctx.propose(net, args.threshold, count=False, min_particles=wm)
+                    ctx.warm_proposal(net, p, args.threshold, wm, count=False)
                     ctx.run(p, hbtx.WARMSTART | gflag)
                 ms_wm = timed_steps(ctx, args.steps, args.warmup, step_wm)
                 mm = learned.mismatch_summary(pop, ctx.fetch(), rf)
@@ -371,8 +368,7 @@ def main():
                                                 for t, r in row.items()))
     # work counters: one extra, untimed run of the counting instantiation (same results, deterministic work),
     # from the headline's proposal
-    ctx.features(prm, fetch=False)
-    ctx.propose(net, args.threshold, count=False, min_particles=args.warm_min)
+    ctx.warm_proposal(net, prm, args.threshold, args.warm_min, count=False)
     ctx.run(prm, hbtx.WARMSTART | hbtx.COUNT)
     st = ctx.stats()
     ctx.run(prm, hbtx.COUNT)

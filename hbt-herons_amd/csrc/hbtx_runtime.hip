@@ -179,6 +179,10 @@ struct Batch
   DArr<float> mlast, mbound, ref, oldref, mb, upscale, ref_in, mb_in, minit;
   DArr<int32_t> warm, npro, nadd, reset_;
   DArr<float> feat, logit;
+  // 32-particle groups of the subhaloes of at least egroups_min particles (-1: none built)
+  DArr<int32_t> egroups;
+  int64_t egroups_n = 0, egroups_min = -1;
+  bool feat_full = false;  // feat holds every particle's features (hbtx_batch_features)
   // active-slot tiles of the current round and their prefix sums
   DArr<int4> stiles;
   DArr<int32_t> stile_n, stile_off, tsum32, tpre32, sub_first32, sub_total32;
@@ -1423,6 +1427,8 @@ void upload_batch(hbtx_ctx *c, Batch &B, int64_t n, const float *pos, const floa
   B.has_u = u != nullptr;
   B.has_proposal = false;
   B.has_prev = false;
+  B.egroups_min = -1;
+  B.feat_full = false;
   B.h_off.assign(offsets, offsets + nsub + 1);
   B.h_count.resize(nsub);
   B.max_count = 0;
@@ -2648,7 +2654,7 @@ int hbtx_mlp_predict(hbtx_ctx *c, hbtx_mlp *M, int64_t n, const float *X, float 
     dX.alloc(n * MLP_IN);
     dl.alloc(n);
     HIPCHK(hipMemcpyAsync(dX.p, X, n * MLP_IN * sizeof(float), hipMemcpyHostToDevice, st));
-    k_mlp_forward<<<mlp_forward_blocks(n), 256, 0, st>>>(M->wf.p, M->wb.p, dX.p, n, dl.p);
+    k_mlp_forward<<<mlp_forward_blocks(n), 256, 0, st>>>(M->wf.p, M->wb.p, dX.p, n, dl.p, nullptr, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(logit, dl.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -2689,6 +2695,7 @@ int hbtx_batch_set_previous(hbtx_ctx *c, const int32_t *prev_rank, const uint8_t
     HIPCHK(hipMemcpyAsync(B.prev_rs.p, prev_rscale, ns * sizeof(float), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
     B.has_prev = true;
+    B.feat_full = false;  // the features depend on the previous outputs
   });
 }
 
@@ -2706,8 +2713,9 @@ int hbtx_batch_features(hbtx_ctx *c, const hbtx_params *prm, float *out_features
     B.feat.alloc(n * MLP_IN);
     View V = make_view(B, 0, n, 0, B.nsub, true);
     PrevArrays A{B.prev_rank.p, B.prev_bound.p, B.prev_energy.p, B.prev_com.p, B.prev_mb.p, B.prev_rs.p};
-    k_features<<<blocks_for(n), 256, 0, st>>>(V.S, V.Q, P, A, n, B.feat.p);
+    k_features<<<blocks_for(n), 256, 0, st>>>(V.S, V.Q, P, A, n, B.feat.p, nullptr, 0);
     HIPCHK(hipGetLastError());
+    B.feat_full = true;
     if (out_features)
     {
       HIPCHK(hipMemcpyAsync(out_features, B.feat.p, n * MLP_IN * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -2737,12 +2745,12 @@ int hbtx_batch_propose(hbtx_ctx *c, hbtx_mlp *M, float threshold, int64_t min_pa
   API_GUARD(c, {
     HIPCHK(hipSetDevice(c->device));
     Batch &B = c->B;
-    REQUIRE(B.n > 0 && B.feat.p, HBTX_E_INVALID, "compute the batch features first");
+    REQUIRE(B.n > 0 && B.feat.p && B.feat_full, HBTX_E_INVALID, "compute the batch features first");
     hipStream_t st = c->stream;
     const int64_t n = B.n;
     B.logit.alloc(n);
     HIPCHK(hipEventRecord(c->wev0, st));
-    k_mlp_forward<<<mlp_forward_blocks(n), 256, 0, st>>>(M->wf.p, M->wb.p, B.feat.p, n, B.logit.p);
+    k_mlp_forward<<<mlp_forward_blocks(n), 256, 0, st>>>(M->wf.p, M->wb.p, B.feat.p, n, B.logit.p, nullptr, 0);
     k_proposal_from_logit<<<blocks_for(n), 256, 0, st>>>(B.logit.p, n, std::log(threshold / (1.f - threshold)),
                                                           B.sub_of.p, B.off.p, min_particles, B.proposal.p);
     HIPCHK(hipEventRecord(c->wev1, st));
@@ -2753,6 +2761,79 @@ int hbtx_batch_propose(hbtx_ctx *c, hbtx_mlp *M, float threshold, int64_t min_pa
     if (elapsed_ms)
       *elapsed_ms = ms;
     B.has_proposal = true;
+    if (nproposed)
+    {
+      std::vector<uint8_t> h(n);
+      HIPCHK(hipMemcpy(h.data(), B.proposal.p, n, hipMemcpyDeviceToHost));
+      int64_t cnt = 0;
+      for (int64_t i = 0; i < n; i++)
+        cnt += h[i];
+      *nproposed = cnt;
+    }
+  });
+}
+
+int hbtx_batch_warm_proposal(hbtx_ctx *c, const hbtx_params *prm, hbtx_mlp *M, float threshold,
+                             int64_t min_particles, int64_t *nproposed, double *elapsed_ms)
+{
+  if (!c || !prm || !M || !(threshold > 0.f && threshold < 1.f) || min_particles < 0)
+    return HBTX_E_INVALID;
+  API_GUARD(c, {
+    HIPCHK(hipSetDevice(c->device));
+    Batch &B = c->B;
+    REQUIRE(B.n > 0 && B.has_prev, HBTX_E_INVALID, "load a batch and its previous-snapshot outputs first");
+    const DevParams P = make_dev_params(*prm);
+    hipStream_t st = c->stream;
+    const int64_t n = B.n;
+    if (B.egroups_min != min_particles)
+    {
+      // the 32-particle groups that hold a particle of a subhalo of at least
+      // min_particles particles (subhaloes are contiguous, so the list ascends)
+      std::vector<int32_t> g;
+      int64_t last = -1;
+      for (int32_t s = 0; s < B.nsub; s++)
+      {
+        const int64_t a = B.h_off[s], e = B.h_off[s + 1];
+        if (e - a < std::max<int64_t>(min_particles, 1))
+          continue;
+        for (int64_t q = std::max<int64_t>(a / 32, last + 1); q <= (e - 1) / 32; q++)
+          g.push_back((int32_t)q);
+        last = (e - 1) / 32;
+      }
+      B.egroups.alloc(std::max<size_t>(g.size(), 1));
+      if (!g.empty())
+        HIPCHK(hipMemcpyAsync(B.egroups.p, g.data(), g.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+      B.egroups_n = (int64_t)g.size();
+      B.egroups_min = min_particles;
+    }
+    B.feat.alloc(n * MLP_IN);
+    B.logit.alloc(n);
+    B.feat_full = false;  // only the listed groups' features are written
+    View V = make_view(B, 0, n, 0, B.nsub, true);
+    PrevArrays A{B.prev_rank.p, B.prev_bound.p, B.prev_energy.p, B.prev_com.p, B.prev_mb.p, B.prev_rs.p};
+    HIPCHK(hipEventRecord(c->wev0, st));
+    if (B.egroups_n > 0)
+    {
+      k_features<<<blocks_for(32 * B.egroups_n), 256, 0, st>>>(V.S, V.Q, P, A, n, B.feat.p, B.egroups.p,
+                                                               B.egroups_n);
+      k_mlp_forward<<<mlp_forward_blocks(32 * B.egroups_n), 256, 0, st>>>(M->wf.p, M->wb.p, B.feat.p, n,
+                                                                          B.logit.p, B.egroups.p, B.egroups_n);
+    }
+    k_proposal_from_logit<<<blocks_for(n), 256, 0, st>>>(B.logit.p, n, std::log(threshold / (1.f - threshold)),
+                                                          B.sub_of.p, B.off.p, std::max<int64_t>(min_particles, 1),
+                                                          B.proposal.p);
+    HIPCHK(hipEventRecord(c->wev1, st));
+    HIPCHK(hipGetLastError());
+    B.has_proposal = true;
+    if (elapsed_ms || nproposed)
+      HIPCHK(hipEventSynchronize(c->wev1));
+    if (elapsed_ms)
+    {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, c->wev0, c->wev1));
+      *elapsed_ms = ms;
+    }
     if (nproposed)
     {
       std::vector<uint8_t> h(n);

@@ -27,10 +27,16 @@ struct PrevArrays
   const float *rscale;  // per subhalo: previous bound rms radius (comoving)
 };
 
-__global__ void k_features(SubArrays S, PartArrays Q, DevParams P, PrevArrays V, int64_t n, float *X)
+// groups (may be null): only the particles of these 32-particle groups.
+__global__ void k_features(SubArrays S, PartArrays Q, DevParams P, PrevArrays V, int64_t n, float *X,
+                           const int32_t *groups, int64_t ngroups)
 {
-  GRID_STRIDE(p, n)
+  const int64_t cnt = groups ? 32 * ngroups : n;
+  GRID_STRIDE(i, cnt)
   {
+    const int64_t p = groups ? 32 * (int64_t)groups[i >> 5] + (i & 31) : i;
+    if (p >= n)
+      continue;
     const int s = S.sub_of[p];
     const int ns = S.off[s + 1] - S.off[s];
     const float *c = V.com + 6 * s;

@@ -58,9 +58,12 @@ HBTX_DEV __bf16 bf16_bits(uint16_t b)
 // Lane l holds output column l % 32 and rows 8*(r/4) + 4*(l/32) + r%4 of a 32 x 32
 // accumulator block (r = 0..15); a lane's layer-1 rows are exactly the K entries
 // it supplies to layer 2, with the K order of both operands permuted alike.
+// groups (may be null): the 32-particle groups to evaluate (particles 32 g .. 32 g + 31
+// for g = groups[0 .. ngroups)); null evaluates every particle.
 __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ w, const uint16_t *__restrict__ wb,
                                                      const float *__restrict__ X, int64_t n,
-                                                     float *__restrict__ logit)
+                                                     float *__restrict__ logit, const int32_t *__restrict__ groups,
+                                                     int64_t ngroups)
 {
   const int lane = threadIdx.x & 63, h = lane >> 5, c = lane & 31;
   bf16x8 a1[2];
@@ -90,8 +93,10 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ w
   const float b3 = w[MLP_OFF_B3];
   const int64_t wave = (int64_t)blockIdx.x * (blockDim.x / HBTX_WAVE) + (threadIdx.x >> 6);
   const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / HBTX_WAVE);
-  for (int64_t g = wave; g * 32 < n; g += nwaves)
+  const int64_t G = groups ? ngroups : (n + 31) / 32;
+  for (int64_t gi = wave; gi < G; gi += nwaves)
   {
+    const int64_t g = groups ? (int64_t)groups[gi] : gi;
     const int64_t j = g * 32 + c;
     float x[8];
     if (j < n)

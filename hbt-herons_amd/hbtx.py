@@ -163,6 +163,8 @@ def lib():
         L.hbtx_batch_set_previous.argtypes = [P, P, P, P, P, P, P, P]
         L.hbtx_batch_features.argtypes = [P, C.POINTER(Params), P]
         L.hbtx_batch_propose.argtypes = [P, P, C.c_float, i64, C.POINTER(i64), C.POINTER(C.c_double)]
+        L.hbtx_batch_warm_proposal.argtypes = [P, C.POINTER(Params), P, C.c_float, i64, C.POINTER(i64),
+                                               C.POINTER(C.c_double)]
         L.hbtx_batch_get_proposal.argtypes = [P, P]
         L.hbtx_get_tuning.argtypes = [P, C.POINTER(i64), C.POINTER(C.c_int32)]
         L.hbtx_set_tuning.argtypes = [P, i64, C.c_int32]
@@ -295,6 +297,18 @@ class Context:
         ms = C.c_double(0.0)
         self._check(lib().hbtx_batch_propose(self._h, mlp._h, threshold, int(min_particles),
                                              C.byref(npro) if count else None, C.byref(ms)))
+        return (npro.value if count else None), ms.value
+
+    def warm_proposal(self, mlp, params=None, threshold=0.5, min_particles=0, count=True):
+        """Features + inference + proposal in one call, evaluated only on the
+        particles of subhaloes of at least min_particles particles (the proposal
+        of features() followed by propose()); returns (number proposed or None,
+        milliseconds of the feature and inference kernels)."""
+        prm = params if isinstance(params, Params) else make_params(**(params or {}))
+        npro = C.c_int64(0)
+        ms = C.c_double(0.0)
+        self._check(lib().hbtx_batch_warm_proposal(self._h, C.byref(prm), mlp._h, threshold, int(min_particles),
+                                                   C.byref(npro) if count else None, C.byref(ms)))
         return (npro.value if count else None), ms.value
 
     def get_proposal(self):

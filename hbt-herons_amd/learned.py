@@ -138,8 +138,7 @@ def warm_unbind(ctx, pop, prev, net, params, threshold=0.1, flags=hbtx.BATCHED, 
     Returns (output, run milliseconds, inference milliseconds, proposed count)."""
     ctx.load(pop)
     ctx.set_previous(prev)
-    ctx.features(params, fetch=False)
-    npro, inf_ms = ctx.propose(net, threshold, min_particles=min_particles)
+    npro, inf_ms = ctx.warm_proposal(net, params, threshold, min_particles)
     ms = ctx.run(params, flags | hbtx.WARMSTART)
     out = ctx.fetch()
     return out, ms, inf_ms, npro

@@ -275,9 +275,18 @@ int hbtx_batch_set_previous(hbtx_ctx *ctx, const int32_t *prev_rank, const uint8
 int hbtx_batch_features(hbtx_ctx *ctx, const hbtx_params *prm, float *out_features);
 /* Inference on the batch features: proposal = P(bound) > threshold, kept on the device
  * for HBTX_WARMSTART; subhaloes of fewer than min_particles particles get no proposal
- * (they run the plain iteration); *elapsed_ms times the inference kernels. */
+ * (they run the plain iteration); *elapsed_ms times the inference kernels. Needs
+ * hbtx_batch_features after the last hbtx_batch_set_previous (HBTX_E_INVALID otherwise). */
 int hbtx_batch_propose(hbtx_ctx *ctx, hbtx_mlp *mlp, float threshold, int64_t min_particles, int64_t *nproposed,
                        double *elapsed_ms);
+/* Features and inference for the subhaloes of at least min_particles particles
+ * only, in one call (the others get no proposal and run the plain iteration):
+ * the same proposal as hbtx_batch_features followed by hbtx_batch_propose, with
+ * the features and the classifier evaluated on those subhaloes' particles alone.
+ * *elapsed_ms times the feature and inference kernels. Leaves the other particles'
+ * features unset, so hbtx_batch_propose needs hbtx_batch_features again. */
+int hbtx_batch_warm_proposal(hbtx_ctx *ctx, const hbtx_params *prm, hbtx_mlp *mlp, float threshold,
+                             int64_t min_particles, int64_t *nproposed, double *elapsed_ms);
 /* Copy of the current proposal (uint8[n]). */
 int hbtx_batch_get_proposal(hbtx_ctx *ctx, uint8_t *proposal);
 

@@ -124,6 +124,39 @@ def test_warm_start_matches_oracle_warm_start(ctx, evolved, trained, k, prec):
     assert order_is_reference_layout(warm, pop)
 
 
+def test_warm_proposal_equals_features_then_propose(ctx, evolved, trained):
+    """hbtx_batch_warm_proposal evaluates the features and the network on the
+    eligible subhaloes' particles only: its proposal is bit for bit the one of
+    hbtx_batch_features followed by hbtx_batch_propose, for every threshold of
+    subhalo size (the cached group list is rebuilt when the size changes and when
+    another batch is loaded)."""
+    net, _ = trained
+    prm = hbtx.workload_params()
+    counts = evolved[6].counts()
+    sizes = [0, 1, int(np.median(counts)), int(counts.max()), int(counts.max()) + 1, 0]
+    for k in (6, 7):
+        pop = evolved[k]
+        prev = plain_and_prev(ctx, evolved, k, prm)
+        ctx.load(pop)
+        ctx.set_previous(prev)
+        with pytest.raises(hbtx.HbtxError):
+            ctx.propose(net, 0.1)  # no features since set_previous
+        for m in sizes:
+            npro, _ = ctx.warm_proposal(net, prm, 0.1, m)
+            a = ctx.get_proposal()
+            with pytest.raises(hbtx.HbtxError):
+                ctx.propose(net, 0.1)  # the restricted call leaves partial features
+            ctx.features(prm, fetch=False)
+            npro2, _ = ctx.propose(net, 0.1, min_particles=m)
+            b = ctx.get_proposal()
+            assert npro == npro2 == int(a.sum()), (k, m)
+            assert np.array_equal(a, b), (k, m)
+            eligible = np.repeat(pop.counts() >= max(m, 1), pop.counts())
+            assert not np.any(a[~eligible])
+            if m <= 1:
+                assert npro > 0
+
+
 @pytest.mark.parametrize("k,prec", [(6, 0.995), (7, 0.995), (6, 1.0), (7, 1.0)])
 def test_warm_start_matches_plain_iteration(ctx, evolved, trained, k, prec):
     """The plain iteration judges the warm start: bound mass within 0.1%."""
