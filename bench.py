@@ -314,7 +314,8 @@ def main():
         f"{npro} proposed, inference {inf_ms:.2f} ms)")
     if args.learned_only:
         log(json.dumps({"ms_per_step": float(np.mean(ms_learned)), "walk_ms": walk_ms, "rounds": rounds,
-                        "walk_avg_launch_ms": walk_ms / max(1, rounds)}))
+                        "walk_avg_launch_ms": walk_ms / max(1, rounds), "tree_entries": st_time["tree_entries"],
+                        "tree_entries_first": st_time["tree_entries_first"]}))
         ctx.close()
         return
     mismatch = learned.bound_mass_mismatch(pop, warm, ref)

@@ -77,19 +77,11 @@ struct DevParams
   int64_t direct_cutoff;
 };
 
-// One depth-first tree entry (32 bytes, one scalar load).  Cells carry the centre
-// of mass and mass; particles their position and (scaled) mass.  crit2 is the
-// squared opening radius len^2 / theta^2 (0 for particles, which are never
-// opened); len2 keeps len^2 (float bits) for export; skip is the first entry
-// after this entry's subtree.
-struct __attribute__((aligned(32))) Entry
-{
-  float x, y, z, m;
-  float crit2;
-  int32_t rskip; // skip in bytes relative to the first entry of the entry's tree (the walk's stride)
-  float len2;
-  int32_t skip;
-};
+// Depth-first tree entries are kept as parallel arrays: float4 (x, y, z, m) -
+// a cell's centre of mass and mass, a particle's position and (scaled) mass -,
+// the depth (int8, -1 for a particle) and, for cells only, skip (int32: the
+// first entry after the cell's subtree; a particle's is its own index + 1).
+// The walk reads its own compact records, built alongside.
 
 // Loads through the constant address space: a wave-uniform index then becomes a
 // scalar (s_load) access.  Only for buffers that no kernel writes while it reads them.

@@ -165,7 +165,8 @@ struct Batch
   DArr<int32_t> flag, fscan, crank, cnt, cscan;
   DArr<int64_t> flag64, fscan64;
   DArr<uint64_t> keys;
-  DArr<Entry> ent; // ent_cap + 1 entries (the walk may prefetch one past the end)
+  DArr<float4> ent;      // per entry (x, y, z, m): ent_cap + 1
+  DArr<int32_t> eskip;   // per cell entry: skip
   DArr<int8_t> ent_depth;
   DArr<uint2> wt;        // walk records: 3 ent_cap + 4 units of 8 bytes
   DArr<int32_t> cell_u;  // per entry index: walk-record unit of a cell
@@ -197,7 +198,7 @@ struct Batch
   DArr<int32_t> misc; // [0] active, [1] (unused), [2] overflow, [3] entry total, [4] max depth
   DArr<int32_t> wctr; // walk work counters, 16 ints apart
   DArr<char2> lev;                // per dense source: the level pair of its tree
-  DArr<int32_t> cells, depth_cnt; // tree cells bucketed by depth; counts, offsets, cursors
+  DArr<int32_t> mom_slots;        // moment passes: per window / group, the cells that end beyond it
   DArr<hbtx_sub_result> dres;
   DArr<float> prof_r, prof_m; // profiles: sorted radii and cumulative masses
   DArr<hbtx_sub_profile> dprof;
@@ -228,7 +229,8 @@ struct hbtx_tree
   int32_t ne = 0;
   int64_t ncells = 0;
   DevParams P;
-  DArr<Entry> ent;
+  DArr<float4> ent;
+  DArr<int32_t> eskip;
   DArr<int8_t> ent_depth;
   DArr<uint2> wt;   // walk records of the tree
   int32_t nw = 0;   // their units
@@ -293,6 +295,7 @@ struct hbtx_ctx
   hipGraphExec_t g_head[3] = {nullptr, nullptr, nullptr}, g_tail = nullptr; // head per round level
   GraphKey g_key;
   int graph_rounds = 0, graph_captures = 0;
+  int64_t tree_entries = 0, tree_entries_first = 0; // octree entries built, all rounds / first round
   Batch B;
   // particle reassignment buffers (grow-only, reused across calls)
   struct
@@ -608,6 +611,7 @@ void batch_reserve(hbtx_ctx *c, Batch &B)
   B.lev.alloc(n + 1);
   B.ent_cap = (int32_t)std::min<int64_t>(3 * n + 64, WREC_MAX_ENTRIES);
   B.ent.alloc((int64_t)B.ent_cap + 1);
+  B.eskip.alloc((int64_t)B.ent_cap + 1);
   B.ent_depth.alloc(((int64_t)B.ent_cap + 16) & ~(int64_t)15);
   B.wt.alloc(3 * (int64_t)B.ent_cap + 4);
   B.cell_u.alloc((int64_t)B.ent_cap + 1);
@@ -743,6 +747,7 @@ TreeArgs tree_args(Batch &B, View &V, const DevParams &P)
   T.cscan = B.cscan.p;
   T.s_total = B.tpre32.p + (V.n / SLOT_TILE + V.nsub + 1); // grand total of the source tile scan
   T.ent = B.ent.p;
+  T.eskip = B.eskip.p;
   T.ent_depth = B.ent_depth.p;
   T.wt = B.wt.p;
   T.cell_u = B.cell_u.p;
@@ -767,8 +772,10 @@ void grow_entries(Batch &B, int64_t need, bool slack = true)
   const int64_t cap = std::min<int64_t>(slack ? need + need / 4 + 64 : need, WREC_MAX_ENTRIES);
   REQUIRE(cap >= need, HBTX_E_CAPACITY, "tree entry count exceeds the walk-record range");
   B.ent.release();
+  B.eskip.release();
   B.ent_depth.release();
   B.ent.alloc(cap + 1);
+  B.eskip.alloc(cap + 1);
   B.ent_depth.alloc((cap + 16) & ~(int64_t)15);
   B.wt.release();
   B.cell_u.release();
@@ -862,19 +869,15 @@ void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, cons
   seg_sort(c, B.keys.p, B.src.p, V.nsub, B.seg_begin.p, B.seg_len.p, n, 63, level >= 1);
   k_targets_from_sources<<<gb, 256, 0, st>>>(V.S, V.Q, B.src.p, B.src_sub.p, s_total);
   TreeArgs T = tree_args(B, V, P);
-  k_fill_i32<<<1, 64, 0, st>>>(B.misc.p + 4, 1, 0);
-  k_tree_count<<<blocks_for(n, 256, 8 * 1024), 256, 0, st>>>(T, B.cnt.p, B.misc.p + 4);
+  k_tree_count<<<blocks_for(n, 256, 8 * 1024), 256, 0, st>>>(T, B.cnt.p);
   scan32_dev(c, B.cnt.p, B.cscan.p, n, s_total);
-  int maxdepth = HBTX_KEY_LEVELS;
   if (!graph)
   {
     // exact entry count and depth of this round's trees; grow the entry buffers if needed
     k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, 0x7fffffff, B.misc.p + 2);
-    int32_t hm[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(hm, B.misc.p + 3, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    int32_t etot = 0;
+    HIPCHK(hipMemcpyAsync(&etot, B.misc.p + 3, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    const int32_t etot = hm[0];
-    maxdepth = std::min(hm[1], HBTX_KEY_LEVELS);
     REQUIRE(etot >= 0, HBTX_E_CAPACITY, "tree entry count overflows int32");
     if (etot > B.ent_cap)
     {
@@ -883,7 +886,7 @@ void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, cons
     }
   }
   // (a captured round cannot wait for the count: its buffers are sized for the
-  // largest trees the batch can make, and the moments pass covers every depth)
+  // largest trees the batch can make)
   k_tree_emit<<<blocks_for(n, 256, 8 * 1024), 256, 0, st>>>(T);
   k_sub_entries<<<gs, 256, 0, st>>>(V.S, B.cscan.p, B.ent_cap, B.misc.p + 2);
   k_entry_total<<<1, 1, 0, st>>>(s_total, B.cscan.p, B.misc.p + 3, B.ent_cap, B.misc.p + 2);
@@ -892,18 +895,25 @@ void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, cons
     HIPCHK(hipGetLastError());
     return; // direct sums only: no cells
   }
-  // moments bottom-up, one pass per depth over that depth's cells
-  B.cells.alloc(std::max<int64_t>(B.ent_cap, 1));
-  B.depth_cnt.alloc(3 * DEPTH_BINS + 1);
-  int32_t *dcnt = B.depth_cnt.p, *doff = dcnt + DEPTH_BINS, *dcur = doff + DEPTH_BINS + 1;
-  const int eg = blocks_for(B.ent_cap, 256, 8192);
-  k_fill_i32<<<1, 64, 0, st>>>(dcnt, DEPTH_BINS, 0);
-  k_depth_count<<<eg, 256, 0, st>>>(B.ent_depth.p, B.misc.p + 3, dcnt);
-  k_depth_offsets<<<1, 64, 0, st>>>(dcnt, doff, dcur);
-  k_depth_scatter<<<blocks_for(B.ent_cap, 256 * DEPTH_ITEMS, 8192), 256, 0, st>>>(B.ent_depth.p, B.misc.p + 3, doff,
-                                                                                   dcur, B.cells.p);
-  for (int d = maxdepth; d >= 0; d--)
-    k_tree_moments<<<blocks_for(n / 4 + 1, 256, 8192), 256, 0, st>>>(T, B.cells.p, doff, B.misc.p + 3, d);
+  // moments bottom-up: LDS-staged windows, then one pass per level of window
+  // groups (levels sized for the entry capacity; each pass reads the entry total)
+  std::vector<int64_t> units{(std::max<int64_t>(B.ent_cap, 1) + MOM_WIN - 1) / MOM_WIN};
+  while (units.back() > 1)
+    units.push_back((units.back() + MOM_GROUP - 1) / MOM_GROUP);
+  int64_t nslots = 0;
+  for (int64_t u : units)
+    nslots += u;
+  B.mom_slots.alloc(nslots * MOM_SLOTS);
+  int32_t *sl = B.mom_slots.p;
+  k_moments_windows<<<(int)std::min<int64_t>(units[0], 4096), MOM_THREADS, 0, st>>>(T, B.misc.p + 3, sl);
+  int64_t span = MOM_WIN;
+  for (size_t k = 1; k < units.size(); k++)
+  {
+    int32_t *up = sl + units[k - 1] * MOM_SLOTS;
+    k_moments_groups<<<(int)std::min<int64_t>(units[k], 4096), MOM_GROUP, 0, st>>>(T, B.misc.p + 3, span, sl, up);
+    sl = up;
+    span *= MOM_GROUP;
+  }
   HIPCHK(hipGetLastError());
 }
 
@@ -1381,6 +1391,9 @@ int run_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot, 
     HIPCHK(hipEventElapsedTime(&wms, c->wev0, c->wev1));
     c->walk_ms += wms;
     c->walk_items_hint = std::max(h_misc[7], 1);
+    c->tree_entries += h_misc[3];
+    if (round == 1)
+      c->tree_entries_first += h_misc[3];
     // the next round's level: subhaloes only leave the active set
     const int64_t maxn = h_misc[6];
     level = (maxn <= P.direct_cutoff) ? 2 : (maxn <= SORT_SMALL_MAX ? 1 : 0);
@@ -1523,6 +1536,7 @@ void run_batch(hbtx_ctx *c, const hbtx_params &prm, uint32_t flags, double *elap
   c->rounds = 0;
   c->graph_rounds = 0;
   c->graph_captures = 0;
+  c->tree_entries = c->tree_entries_first = 0;
   HIPCHK(hipEventRecord(c->ev0, st));
   const bool want_pot = (flags & HBTX_SAVE_POTENTIAL) != 0;
   c->count_work = (flags & HBTX_COUNT) != 0;
@@ -1619,6 +1633,7 @@ void build_tree(hbtx_ctx *c, const hbtx_params &prm, int64_t n, const float *pos
   T->P = P;
   T->treemode = treemode;
   T->ent.alloc((int64_t)ne + 1);
+  T->eskip.alloc((int64_t)ne + 1);
   T->ent_depth.alloc(ne);
   T->src_of_entry.alloc(n);
   T->cube.alloc(4);
@@ -1626,7 +1641,8 @@ void build_tree(hbtx_ctx *c, const hbtx_params &prm, int64_t n, const float *pos
   T->nw = (int32_t)(2 * n + 3 * ((int64_t)ne - n));
   T->wt.alloc((int64_t)T->nw + 2);
   HIPCHK(hipMemcpyAsync(T->wt.p, B.wt.p, ((int64_t)T->nw + 2) * sizeof(uint2), hipMemcpyDeviceToDevice, st));
-  HIPCHK(hipMemcpyAsync(T->ent.p, B.ent.p, ((int64_t)ne + 1) * sizeof(Entry), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(T->ent.p, B.ent.p, ((int64_t)ne + 1) * sizeof(float4), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(T->eskip.p, B.eskip.p, ((int64_t)ne + 1) * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(T->ent_depth.p, B.ent_depth.p, ne, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(T->src_of_entry.p, B.src.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(T->cube.p, B.cube.p, 4 * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -1935,7 +1951,8 @@ int hbtx_batch_fetch(hbtx_ctx *c, int64_t *out_order, float *out_energy, float *
 
 static const char *kStatNames[NSTATS] = {"tree_wave_steps", "tree_lane_visits", "tree_lane_accepts",
                                          "direct_lane_pairs", "targets", "work_items", "near_interactions",
-                                         "unused7", "unused8", "unused9", "unused10", "unused11", "walk_ms",
+                                         "tree_entries", "tree_entries_first", "unused9", "unused10", "unused11",
+                                         "walk_ms",
                                          "rounds", "graph_rounds", "graph_captures"};
 
 int hbtx_batch_stats(hbtx_ctx *c, double *stats, int nstats)
@@ -1947,6 +1964,10 @@ int hbtx_batch_stats(hbtx_ctx *c, double *stats, int nstats)
     HIPCHK(hipMemcpy(h, c->stats.p, sizeof(h), hipMemcpyDeviceToHost));
     for (int i = 0; i < nstats && i < NSTATS; i++)
       stats[i] = (double)h[i];
+    if (nstats > 7)
+      stats[7] = (double)c->tree_entries;
+    if (nstats > 8)
+      stats[8] = (double)c->tree_entries_first;
     if (nstats > 12)
       stats[12] = c->walk_ms;
     if (nstats > 13)
@@ -2053,10 +2074,14 @@ int hbtx_tree_export(hbtx_ctx *c, hbtx_tree *T, int64_t cap, int32_t *kind, int6
     if (nentries)
       *nentries = ne;
     REQUIRE(cap >= ne, HBTX_E_CAPACITY, "export capacity too small");
-    std::vector<Entry> en(ne);
+    std::vector<float4> en(ne);
+    std::vector<int32_t> esk(ne);
     std::vector<int8_t> dep(ne);
     std::vector<int32_t> src(T->n);
-    HIPCHK(hipMemcpy(en.data(), T->ent.p, ne * sizeof(Entry), hipMemcpyDeviceToHost));
+    double cube[4];
+    HIPCHK(hipMemcpy(en.data(), T->ent.p, ne * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(esk.data(), T->eskip.p, ne * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cube, T->cube.p, sizeof(cube), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(dep.data(), T->ent_depth.p, ne, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(src.data(), T->src_of_entry.p, T->n * sizeof(int32_t), hipMemcpyDeviceToHost));
     int64_t pi = 0;
@@ -2070,7 +2095,7 @@ int hbtx_tree_export(hbtx_ctx *c, hbtx_tree *T, int64_t cap, int32_t *kind, int6
       if (part)
         pi++;
       if (mass)
-        mass[q] = en[q].m;
+        mass[q] = en[q].w;
       if (com)
       {
         com[3 * q] = en[q].x;
@@ -2078,9 +2103,17 @@ int hbtx_tree_export(hbtx_ctx *c, hbtx_tree *T, int64_t cap, int32_t *kind, int6
         com[3 * q + 2] = en[q].z;
       }
       if (len)
-        len[q] = part ? 0.f : std::sqrt(en[q].len2);
+      {
+        // the cell's side: the root cube's, halved per level while not below the
+        // resolution (the same rule and float rounding as the build)
+        double L = 2.0 * cube[3];
+        for (int l = 0; l < dep[q]; l++)
+          L = (L >= T->P.res) ? L * 0.5 : L;
+        const float lf = (float)L;
+        len[q] = part ? 0.f : std::sqrt(lf * lf);
+      }
       if (skip)
-        skip[q] = en[q].skip;
+        skip[q] = part ? q + 1 : esk[q];
       if (depth)
         depth[q] = dep[q];
     }

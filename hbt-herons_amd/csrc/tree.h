@@ -14,10 +14,11 @@
 // sibling link, src/gravity_tree.cpp:50-80) is the position of the entry group
 // that follows the last source of its key range.
 //
-// Entry records (struct Entry): centre of mass / position, mass, squared
-// opening radius len^2/theta^2, len^2, skip; ent_depth = depth (cells) or -1
-// (particles).  A particle is an entry with opening radius 0 (never opened), so
-// the walk treats both uniformly.
+// Entries: ent = centre of mass / position and mass, ent_depth = depth (cells)
+// or -1 (particles), eskip = skip (cells).  The walk records (wt) carry, per
+// cell, the squared opening radius len^2/theta^2 and the skip as a byte offset;
+// a particle record has no opening radius (never opened), so the walk treats
+// both uniformly.
 #pragma once
 #include "common.h"
 
@@ -41,7 +42,8 @@ struct TreeArgs
   const int32_t *cnt;        // cells starting at each dense source
   const int32_t *cscan;      // exclusive scan of cnt (size >= S+1)
   const int32_t *s_total;    // number of dense sources
-  Entry *ent;
+  float4 *ent;               // per entry: (x, y, z, m)
+  int32_t *eskip;            // per cell entry: first entry after its subtree
   int8_t *ent_depth;
   uint2 *wt;                 // walk records (units of 8 bytes)
   int32_t *cell_u;           // per entry index: unit of a cell's walk record
@@ -73,11 +75,9 @@ HBTX_DEV void pair_levels(const TreeArgs &A, int j, int b, int e, int &cprev, in
   cprev = (j > b) ? common_levels(A.keys[j - 1], A.keys[j]) : -1;
 }
 
-// (also records the deepest cell level in *maxdepth)
-__global__ void k_tree_count(TreeArgs A, int32_t *__restrict__ cnt, int32_t *maxdepth)
+__global__ void k_tree_count(TreeArgs A, int32_t *__restrict__ cnt)
 {
   const int S = *A.s_total;
-  int depth = 0;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < S; j += gridDim.x * blockDim.x)
   {
     int c = 0;
@@ -89,20 +89,11 @@ __global__ void k_tree_count(TreeArgs A, int32_t *__restrict__ cnt, int32_t *max
       int cprev, cj;
       pair_levels(A, j, b, e, cprev, cj);
       c = max(0, cj - cprev);
-      if (c > 0)
-        depth = max(depth, cj);
       lv = make_char2((signed char)cprev, (signed char)cj);
     }
     A.lev[j] = lv;
     cnt[j] = c;
   }
-  // deepest cell level: wave maximum, and an atomic only from waves that raise the
-  // current value (one shared word: an atomic from every wave serialises)
-#pragma unroll
-  for (int d = 1; d < HBTX_WAVE; d <<= 1)
-    depth = max(depth, __shfl_xor(depth, d, HBTX_WAVE));
-  if (lane_id() == 0 && depth > 0 && depth > __hip_atomic_load(maxdepth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-    atomicMax(maxdepth, depth);
 }
 
 HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
@@ -113,13 +104,12 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
   const float4 x = A.posm[p];
   const float m = source_mass(A, p, s);
   const int b = A.src_off[s], e = b + A.nsrc[s];
-  const int t0 = b + A.cscan[b]; // first entry of this subhalo's tree
   const int u0 = wrec_unit(b, A.cscan[b]), uj = wrec_unit(j, A.cscan[j]);
   if (!A.treemode[s])
   {
     if (P < A.cap)
     {
-      A.ent[P] = Entry{x.x, x.y, x.z, m, 0.f, (P + 1 - t0) * (int)sizeof(Entry), 0.f, P + 1};
+      A.ent[P] = make_float4(x.x, x.y, x.z, m);
       A.ent_depth[P] = -1;
       A.wt[uj] = make_uint2(__float_as_uint(x.x), __float_as_uint(x.y));
       A.wt[uj + 1] = make_uint2(__float_as_uint(x.z), __float_as_uint(m));
@@ -137,7 +127,7 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
     atomicOr(A.overflow, 1);
     return;
   }
-  A.ent[q] = Entry{x.x, x.y, x.z, m, 0.f, (q + 1 - t0) * (int)sizeof(Entry), 0.f, q + 1};
+  A.ent[q] = make_float4(x.x, x.y, x.z, m);
   A.ent_depth[q] = -1;
   const int up = uj + 3 * nn;
   A.wt[up] = make_uint2(__float_as_uint(x.x), __float_as_uint(x.y));
@@ -187,7 +177,7 @@ HBTX_DEV void tree_emit_one(const TreeArgs &A, int j)
     const float len2 = lf * lf;
     // open if len^2 > r^2 theta^2 (src/gravity_tree.cpp:139)  <=>  r^2 < len^2 / theta^2
     const float crit2 = (A.P.theta2 > 0.f) ? len2 / A.P.theta2 : __int_as_float(0x7f800000);
-    A.ent[pos] = Entry{0.f, 0.f, 0.f, 0.f, crit2, (skip - t0) * (int)sizeof(Entry), len2, skip};
+    A.eskip[pos] = skip; // the moments pass writes the cell's ent
     A.ent_depth[pos] = (int8_t)l;
     // walk record: position and mass come with the moments
     const int uc = uj + 3 * (l - cprev - 1);
@@ -205,131 +195,182 @@ __global__ void k_tree_emit(TreeArgs A)
     tree_emit_one(A, j);
 }
 
-// Cells bucketed by depth, so that each depth's moment pass touches only its own
-// cells.  Pass 1 counts cells per depth (block histograms in LDS); after the
-// offsets, pass 2 writes every cell's index into its depth's bucket: a wave
-// takes one LDS place per distinct depth among its lanes (ballot match), the
-// block reserves one range per depth, and each lane writes at its rank.
-#define DEPTH_BINS 32
-#define DEPTH_ITEMS 4
-__global__ __launch_bounds__(256) void k_depth_count(const int8_t *__restrict__ ent_depth,
-                                                    const int32_t *__restrict__ e_total_p, int32_t *__restrict__ cnt)
+// Cell moments (mass and centre of mass) bottom-up, in child (sub-index) order
+// with double accumulation of the float child values, as UpdateInternalNodes /
+// ProcessNode do (src/gravity_tree.cpp:25-80), without one launch per depth.
+//
+// The entries are cut into windows of MOM_WIN consecutive entries.
+// k_moments_windows stages one window's entries in LDS and forms, depth by depth
+// with block barriers, the moments of every cell whose subtree [q, skip) lies in
+// the window.  A cell that starts in a window and ends beyond it contains the
+// window's last entry, so such cells are nested, at most one per depth: the
+// window records them in its MOM_SLOTS slots (-1 where none).
+// k_moments_groups takes MOM_GROUP consecutive units of the level below (windows,
+// then groups of windows) per block and forms the recorded cells that end in its
+// range, depth by depth: their children are particles, cells of one window, or
+// recorded cells one depth deeper that start in the same range.  Cells that end
+// beyond the range are again nested and go up one level.  The last level covers
+// every entry, so every cell is formed exactly once.
+#define MOM_WIN 1024
+#define MOM_GROUP 256
+#define MOM_SLOTS (HBTX_KEY_LEVELS + 1)
+
+HBTX_DEV void store_cell(const TreeArgs &A, int q, float fx, float fy, float fz, float fm)
 {
-  __shared__ int32_t h[DEPTH_BINS];
-  if (threadIdx.x < DEPTH_BINS)
-    h[threadIdx.x] = 0;
-  __syncthreads();
-  const int E = *e_total_p;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < E; q += gridDim.x * blockDim.x)
-  {
-    const int d = ent_depth[q];
-    if (d >= 0)
-      atomicAdd(&h[d], 1);
-  }
-  __syncthreads();
-  if (threadIdx.x < DEPTH_BINS && h[threadIdx.x])
-    atomicAdd(cnt + threadIdx.x, h[threadIdx.x]);
+  A.ent[q] = make_float4(fx, fy, fz, fm);
+  const int u = A.cell_u[q];
+  A.wt[u] = make_uint2(__float_as_uint(fx), __float_as_uint(fy));
+  A.wt[u + 1] = make_uint2(__float_as_uint(fz), __float_as_uint(fm) | 0x80000000u); // sign bit: a cell
 }
 
-// off[d] = first slot of depth d (exclusive prefix of cnt), cursor zeroed; one thread.
-__global__ void k_depth_offsets(const int32_t *__restrict__ cnt, int32_t *__restrict__ off,
-                                int32_t *__restrict__ cursor)
+#define MOM_THREADS 256
+#define MOM_PER (MOM_WIN / MOM_THREADS)
+__global__ __launch_bounds__(MOM_THREADS) void k_moments_windows(TreeArgs A, const int32_t *__restrict__ e_total_p,
+                                                                 int32_t *__restrict__ slots)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0)
-    return;
-  int run = 0;
-  for (int d = 0; d < DEPTH_BINS; d++)
-  {
-    off[d] = run;
-    run += cnt[d];
-    cursor[d] = 0;
-  }
-  off[DEPTH_BINS] = run;
-}
-
-__global__ __launch_bounds__(256) void k_depth_scatter(const int8_t *__restrict__ ent_depth,
-                                                      const int32_t *__restrict__ e_total_p,
-                                                      const int32_t *__restrict__ off, int32_t *__restrict__ cursor,
-                                                      int32_t *__restrict__ cells)
-{
-  __shared__ int32_t h[DEPTH_BINS], base[DEPTH_BINS];
+  __shared__ float4 xm[MOM_WIN];
+  __shared__ int32_t sk[MOM_WIN];
+  __shared__ int32_t ws[MOM_SLOTS];
+  __shared__ uint32_t depths_s;
   const int E = *e_total_p;
-  const int lane = lane_id();
-  const int per_block = DEPTH_ITEMS * 256;
+  const int nwin = (E + MOM_WIN - 1) / MOM_WIN;
+  const int t = threadIdx.x;
   // every block runs the same number of iterations (the barriers are uniform)
-  for (int q0 = blockIdx.x * per_block; q0 < E; q0 += gridDim.x * per_block)
+  for (int w = blockIdx.x; w < nwin; w += gridDim.x)
   {
-    if (threadIdx.x < DEPTH_BINS)
-      h[threadIdx.x] = 0;
+    const int a = w * MOM_WIN, nw = min(MOM_WIN, E - a);
+    if (t < MOM_SLOTS)
+      ws[t] = -1;
+    if (t == 0)
+      depths_s = 0u;
     __syncthreads();
-    int d[DEPTH_ITEMS], slot[DEPTH_ITEMS];
+    // stage the window: this thread's entries i = r * MOM_THREADS + t; the
+    // depth and subtree end of each stay in registers (-1: a particle, or a cell
+    // that ends beyond the window)
+    int dep[MOM_PER], end[MOM_PER];
+    uint32_t depths = 0u;
 #pragma unroll
-    for (int r = 0; r < DEPTH_ITEMS; r++)
+    for (int r = 0; r < MOM_PER; r++)
     {
-      const int q = q0 + r * 256 + (int)threadIdx.x;
-      d[r] = (q < E) ? ent_depth[q] : -1;
-      slot[r] = 0;
-      uint64_t todo = ballot(d[r] >= 0);
-      while (todo)
+      const int i = r * MOM_THREADS + t;
+      dep[r] = -1;
+      end[r] = 0;
+      if (i < nw)
       {
-        const int leader = __ffsll((long long)todo) - 1;
-        const int d0 = __shfl(d[r], leader, HBTX_WAVE);
-        const uint64_t grp = ballot(d[r] == d0);
-        int b = 0;
-        if (lane == leader)
-          b = atomicAdd(&h[d0], __popcll(grp));
-        b = __shfl(b, leader, HBTX_WAVE);
-        if (d[r] == d0)
-          slot[r] = b + __popcll(grp & lanemask_lt());
-        todo &= ~grp;
+        const float4 v = A.ent[a + i];
+        const int ks = A.eskip[a + i]; // (unset for a particle)
+        const int d = A.ent_depth[a + i];
+        const int k = (d >= 0) ? min(ks, E) - a : i + 1;
+        xm[i] = v;
+        sk[i] = k;
+        end[r] = k;
+        if (d >= 0)
+        {
+          if (k <= nw)
+          {
+            dep[r] = d;
+            depths |= 1u << d;
+          }
+          else
+            ws[d] = a + i;
+        }
       }
     }
-    __syncthreads();
-    if (threadIdx.x < DEPTH_BINS)
-      base[threadIdx.x] = h[threadIdx.x] ? atomicAdd(cursor + threadIdx.x, h[threadIdx.x]) : 0;
-    __syncthreads();
 #pragma unroll
-    for (int r = 0; r < DEPTH_ITEMS; r++)
-      if (d[r] >= 0)
-        cells[off[d[r]] + base[d[r]] + slot[r]] = q0 + r * 256 + (int)threadIdx.x;
+    for (int o = 1; o < HBTX_WAVE; o <<= 1)
+      depths |= (uint32_t)__shfl_xor((int)depths, o, HBTX_WAVE);
+    if (lane_id() == 0 && depths)
+      atomicOr(&depths_s, depths);
     __syncthreads();
+    // the depths present, deepest first: a cell's child cells are one depth deeper
+    uint32_t todo = depths_s;
+    while (todo)
+    {
+      const int d = 31 - __clz((int)todo);
+      todo &= ~(1u << d);
+#pragma unroll
+      for (int r = 0; r < MOM_PER; r++)
+      {
+        if (dep[r] != d)
+          continue;
+        const int i = r * MOM_THREADS + t;
+        double m = 0., cx = 0., cy = 0., cz = 0.;
+        for (int c = i + 1; c < end[r]; c = max(sk[c], c + 1))
+        {
+          const float4 v = xm[c];
+          const double cm = v.w;
+          m += cm;
+          cx += (double)v.x * cm;
+          cy += (double)v.y * cm;
+          cz += (double)v.z * cm;
+        }
+        xm[i] = make_float4((float)(cx / m), (float)(cy / m), (float)(cz / m), (float)m);
+      }
+      __syncthreads();
+    }
+    // the window's cells to HBM, once
+#pragma unroll
+    for (int r = 0; r < MOM_PER; r++)
+      if (dep[r] >= 0)
+      {
+        const int i = r * MOM_THREADS + t;
+        const float4 v = xm[i];
+        store_cell(A, a + i, v.x, v.y, v.z, v.w);
+      }
+    if (t < MOM_SLOTS)
+      slots[(int64_t)w * MOM_SLOTS + t] = ws[t];
+    __syncthreads(); // the next window reuses the LDS
   }
 }
 
-// Mass and centre of mass of every cell at one depth from its children, in
-// child (sub-index) order with double accumulation of the float child values,
-// as UpdateInternalNodes/ProcessNode do (src/gravity_tree.cpp:25-80).  Also
-// fills the cell's walk record.
-__global__ void k_tree_moments(TreeArgs A, const int32_t *__restrict__ cells, const int32_t *__restrict__ off,
-                               const int32_t *__restrict__ e_total_p, int depth)
+// One level above the windows: units of unit_span entries below, MOM_GROUP units a block.
+__global__ __launch_bounds__(MOM_GROUP) void k_moments_groups(TreeArgs A, const int32_t *__restrict__ e_total_p,
+                                                              int64_t unit_span, const int32_t *__restrict__ slots_in,
+                                                              int32_t *__restrict__ slots_out)
 {
+  __shared__ int32_t ws[MOM_SLOTS];
   const int E = *e_total_p;
-  const int b = off[depth], e = off[depth + 1];
-  Entry *__restrict__ ent = A.ent;
-  for (int k = b + blockIdx.x * blockDim.x + threadIdx.x; k < e; k += gridDim.x * blockDim.x)
+  const int64_t gspan = unit_span * MOM_GROUP;
+  const int64_t nunits = (E + unit_span - 1) / unit_span;
+  const int64_t ngroups = (E + gspan - 1) / gspan;
+  for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x)
   {
-    const int q = cells[k];
-    const int end = min(ent[q].skip, E);
-    double m = 0., cx = 0., cy = 0., cz = 0.;
-    int c = q + 1;
-    while (c < end)
+    const int64_t gend = min((int64_t)E, (g + 1) * gspan);
+    if (threadIdx.x < MOM_SLOTS)
+      ws[threadIdx.x] = -1;
+    __syncthreads();
+    const int64_t u = g * MOM_GROUP + threadIdx.x;
+    for (int d = MOM_SLOTS - 1; d >= 0; d--)
     {
-      const Entry en = ent[c];
-      const double cm = en.m;
-      m += cm;
-      cx += (double)en.x * cm;
-      cy += (double)en.y * cm;
-      cz += (double)en.z * cm;
-      c = max(en.skip, c + 1);
+      const int q = (u < nunits) ? slots_in[u * MOM_SLOTS + d] : -1;
+      if (q >= 0)
+      {
+        const int end = min(A.eskip[q], E);
+        if (end > gend)
+          ws[d] = q;
+        else
+        {
+          double m = 0., cx = 0., cy = 0., cz = 0.;
+          int c = q + 1;
+          while (c < end)
+          {
+            const float4 en = A.ent[c];
+            const double cm = en.w;
+            m += cm;
+            cx += (double)en.x * cm;
+            cy += (double)en.y * cm;
+            cz += (double)en.z * cm;
+            const int dc = A.ent_depth[c], kc = A.eskip[c]; // (both loads in flight together)
+            c = (dc >= 0) ? max(kc, c + 1) : c + 1;
+          }
+          store_cell(A, q, (float)(cx / m), (float)(cy / m), (float)(cz / m), (float)m);
+        }
+      }
+      __syncthreads(); // this depth's cells are written before the next shallower depth reads them
     }
-    const float fx = (float)(cx / m), fy = (float)(cy / m), fz = (float)(cz / m), fm = (float)m;
-    ent[q].x = fx;
-    ent[q].y = fy;
-    ent[q].z = fz;
-    ent[q].m = fm;
-    const int u = A.cell_u[q];
-    A.wt[u] = make_uint2(__float_as_uint(fx), __float_as_uint(fy));
-    A.wt[u + 1] = make_uint2(__float_as_uint(fz), __float_as_uint(fm) | 0x80000000u); // sign bit: a cell
+    if (threadIdx.x < MOM_SLOTS)
+      slots_out[g * MOM_SLOTS + threadIdx.x] = ws[threadIdx.x];
+    __syncthreads();
   }
 }
 

@@ -79,7 +79,7 @@ RESULT_DTYPE = np.dtype([
     ("disrupted", np.int32)], align=True)
 
 STAT_NAMES = ["tree_wave_steps", "tree_lane_visits", "tree_lane_accepts", "direct_lane_pairs", "targets",
-              "work_items", "near_interactions", "unused7", "unused8", "unused9", "unused10", "unused11",
+              "work_items", "near_interactions", "tree_entries", "tree_entries_first", "unused9", "unused10", "unused11",
               "walk_ms", "rounds", "graph_rounds", "graph_captures"]
 
 
