@@ -837,7 +837,7 @@ void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, 
     const int32_t *nt = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, n);
     k_upscale_tiles<<<red_grid(c, n, V.nsub), 256, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, nt, B.crank.p,
                                                             c->red_partial.p);
-    k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red2);
+    k_red_finalize<<<red_finalize_blocks(V.nsub), 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red2);
     k_upscale_finish<<<gs, 256, 0, st>>>(V.S, P, V.red2);
   }
   // (without subsampling the candidate rank is irrelevant: every candidate is a source)
@@ -1028,7 +1028,7 @@ void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool wa
   const int32_t *ntiles = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, n);
   k_red_tile<ComOp><<<red_grid(c, n, V.nsub), RED_THREADS, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, ntiles,
                                                                      c->red_partial.p);
-  k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
+  k_red_finalize<<<red_finalize_blocks(V.nsub), 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
   k_round_finish<<<gs, 256, 0, st>>>(V.S, P, V.red);
   if (P.periodic)
     k_round_finish_periodic<<<gs, 256, 0, st>>>(V.S, V.Q, P, V.red);
@@ -1190,7 +1190,7 @@ void run_final(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot
   const int32_t *ntiles = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, n);
   k_final_tile<<<red_grid(c, n, V.nsub), RED_THREADS, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, ntiles,
                                                                 c->red_partial.p);
-  k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
+  k_red_finalize<<<red_finalize_blocks(V.nsub), 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
   const int gw = blocks_for((int64_t)V.nsub * HBTX_WAVE, 256, 1 << 24);
   k_first_tracer<<<gw, 256, 0, st>>>(V.S, V.Q, P, V.fvals, V.first_tracer);
   k_core_wave<<<gw, 256, 0, st>>>(V.S, V.Q, P, V.fvals, V.red2);
@@ -1231,7 +1231,7 @@ void init_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool warm)
   k_fill_i32<<<gs, 256, 0, st>>>(B.aux32.p, V.nsub, 0x7fffffff);
   k_init_mass_tiles<<<red_grid(c, V.n, V.nsub), 256, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, nt, c->red_partial.p,
                                                               B.aux32.p);
-  k_red_finalize<<<gs, 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
+  k_red_finalize<<<red_finalize_blocks(V.nsub), 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
   k_init_mass_finish<<<gs, 256, 0, st>>>(V.S, P, V.red, B.aux32.p, warm ? 1 : 0);
   HIPCHK(hipGetLastError());
 }

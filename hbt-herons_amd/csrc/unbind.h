@@ -919,21 +919,25 @@ __global__ __launch_bounds__(RED_THREADS) void k_red_tile(SubArrays S, PartArray
   }
 }
 
+// One thread per (subhalo, value): the tiles' partial sums in tile order, with
+// reads and writes contiguous across the RED_NV values of a subhalo.
 __global__ void k_red_finalize(const int32_t *toff, const int32_t *ntile, int32_t nsub, const double *partial,
                                double *out)
 {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsub)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nsub * RED_NV)
     return;
-  double v[RED_NV];
-  for (int k = 0; k < RED_NV; k++)
-    v[k] = 0.;
+  const int s = (int)(i / RED_NV), k = (int)(i - (int64_t)s * RED_NV);
   const int t0 = toff[s], n = ntile[s];
+  double v = 0.;
   for (int t = t0; t < t0 + n; t++)
-    for (int k = 0; k < RED_NV; k++)
-      v[k] += partial[(int64_t)t * RED_NV + k];
-  for (int k = 0; k < RED_NV; k++)
-    out[(int64_t)s * RED_NV + k] = v[k];
+    v += partial[(int64_t)t * RED_NV + k];
+  out[i] = v;
+}
+
+inline int red_finalize_blocks(int32_t nsub)
+{
+  return (int)(((int64_t)nsub * RED_NV + 255) / 256);
 }
 
 // Reduction range of the CoM: bound candidates of subhaloes still active after the decision.
