@@ -162,8 +162,8 @@ struct Batch
   DArr<int32_t> state, cur, cur2, src, src_sub, fvals, ford;
   DArr<float> energy, esl, selfm, fenergy, fpot, eref;
   DArr<uint8_t> proposal;
-  DArr<int32_t> flag, fscan, crank, cnt, cscan;
-  DArr<int64_t> flag64, fscan64;
+  DArr<int32_t> flag, crank, cnt, cscan;
+  DArr<int64_t> flag64;
   DArr<uint64_t> keys;
   DArr<float4> ent;      // per entry (x, y, z, m): ent_cap + 1
   DArr<int32_t> eskip;   // per cell entry: skip
@@ -186,8 +186,8 @@ struct Batch
   bool feat_full = false;  // feat holds every particle's features (hbtx_batch_features)
   // active-slot tiles of the current round and their prefix sums
   DArr<int4> stiles;
-  DArr<int32_t> stile_n, stile_off, tsum32, tpre32, sub_first32, sub_total32;
-  DArr<int64_t> tsum64, tpre64, sub_first64, sub_total64;
+  DArr<int32_t> stile_n, stile_off, tsum32, tpre32, sub_first32;
+  DArr<int64_t> tsum64, tpre64, sub_total64;
   // previous snapshot's outputs for the learned warm start (same slot = same particle)
   DArr<int32_t> prev_rank;
   DArr<uint8_t> prev_bound;
@@ -601,12 +601,10 @@ void batch_reserve(hbtx_ctx *c, Batch &B)
   B.eref.alloc(n);
   B.proposal.alloc(n);
   B.flag.alloc(n + 1);
-  B.fscan.alloc(n + 1);
   B.crank.alloc(n + 1);
   B.cnt.alloc(n + 1);
   B.cscan.alloc(n + 1);
   B.flag64.alloc(n + 1);
-  B.fscan64.alloc(n + 1);
   B.keys.alloc(n);
   B.lev.alloc(n + 1);
   B.ent_cap = (int32_t)std::min<int64_t>(3 * n + 64, WREC_MAX_ENTRIES);
@@ -653,8 +651,6 @@ void batch_reserve(hbtx_ctx *c, Batch &B)
     B.tsum64.alloc(maxt + 1);
     B.tpre64.alloc(maxt + 1);
     B.sub_first32.alloc(ns + 1);
-    B.sub_total32.alloc(ns + 1);
-    B.sub_first64.alloc(ns + 1);
     B.sub_total64.alloc(ns + 1);
   }
   reserve_sort(c, n, ns);
@@ -759,7 +755,7 @@ TreeArgs tree_args(Batch &B, View &V, const DevParams &P)
 }
 
 void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, const int32_t *nt, bool graph,
-                        int level);
+                        int level, bool tsum_ready);
 // device round counter (1 in the first round), advanced by k_round_begin
 int32_t *round_ptr(Batch &B)
 {
@@ -804,22 +800,6 @@ const int32_t *build_slot_tiles(hbtx_ctx *c, Batch &B, View &V)
   return B.stile_off.p + V.nsub;
 }
 
-// Exclusive prefix of flag over the active tiles' slots: out[q] (global over the
-// tiles), per subhalo the prefix at its first slot and its total; returns the
-// device pointer to the grand total.
-template <typename T>
-const T *tile_scan(hbtx_ctx *c, View &V, const int4 *tiles, const int32_t *nt, const T *flag, T *out, T *tsum,
-                   T *tpre, T *sub_first, T *sub_total, DArr<T> &part)
-{
-  const int64_t maxt = slot_tile_cap(V);
-  const int tg = tile_grid(c, V);
-  k_tile_sum<T><<<tg, SLOT_THREADS, 0, c->stream>>>(tiles, nt, (int32_t)maxt, flag, tsum);
-  scan_excl<T>(c, tsum, tpre, maxt, part);
-  k_tile_down<T><<<tg, SLOT_THREADS, 0, c->stream>>>(V.S, tiles, nt, flag, tpre, out, sub_first, sub_total);
-  HIPCHK(hipGetLastError());
-  return tpre + maxt;
-}
-
 // Sources of the round -> dense list, then (tree-mode subhaloes) the octree.
 void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool graph, int level)
 {
@@ -841,24 +821,30 @@ void round_sources_and_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, 
     k_upscale_finish<<<gs, 256, 0, st>>>(V.S, P, V.red2);
   }
   // (without subsampling the candidate rank is irrelevant: every candidate is a source)
-  k_mark_sources<<<tile_grid(c, V), 256, 0, st>>>(V.S, V.Q, P, B.stiles.p, nt, round_ptr(B),
-                                                  P.mss > 0 ? B.crank.p : nullptr, B.flag.p);
-  build_sources_tree(c, B, V, P, nt, graph, level);
+  k_mark_sources<<<tile_grid(c, V), SCAN_THREADS, 0, st>>>(V.S, V.Q, P, B.stiles.p, nt, round_ptr(B),
+                                                           P.mss > 0 ? B.crank.p : nullptr, B.flag.p, B.tsum32.p,
+                                                           (int32_t)slot_tile_cap(V));
+  build_sources_tree(c, B, V, P, nt, graph, level, true);
 }
 
 // Flags in B.flag (on the slots of the active tiles) -> dense source list, tree
-// cube, keys, sort, octree entries.
+// cube, keys, sort, octree entries.  tsum_ready: the flag producer already wrote
+// the tile totals into B.tsum32.
 void build_sources_tree(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, const int32_t *nt, bool graph,
-                        int level)
+                        int level, bool tsum_ready)
 {
   hipStream_t st = c->stream;
   const int64_t n = V.n;
   const int gb = blocks_for(n);
   const int gs = blocks_for(V.nsub, 256, 1 << 20);
-  const int32_t *s_total = tile_scan<int32_t>(c, V, B.stiles.p, nt, B.flag.p, B.fscan.p, B.tsum32.p, B.tpre32.p,
-                                              B.sub_first32.p, B.sub_total32.p, c->part32);
-  k_scatter_sources<<<tile_grid(c, V), 256, 0, st>>>(V.S, B.stiles.p, nt, B.flag.p, B.fscan.p, B.src.p,
-                                                     B.src_sub.p);
+  const int64_t maxt = slot_tile_cap(V);
+  if (!tsum_ready)
+    k_tile_sum<int32_t><<<tile_grid(c, V), SLOT_THREADS, 0, st>>>(B.stiles.p, nt, (int32_t)maxt, B.flag.p,
+                                                                  B.tsum32.p);
+  scan_excl<int32_t>(c, B.tsum32.p, B.tpre32.p, maxt, c->part32);
+  const int32_t *s_total = B.tpre32.p + maxt;
+  k_scatter_sources<<<tile_grid(c, V), SLOT_THREADS, 0, st>>>(V.S, B.stiles.p, nt, B.flag.p, B.tpre32.p, B.src.p,
+                                                              B.src_sub.p, B.sub_first32.p);
   k_sub_src<<<gs, 256, 0, st>>>(V.S, B.sub_first32.p);
   k_tree_src_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p);
   const int32_t *nbt = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, n);
@@ -1005,12 +991,12 @@ void round_partition(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool wa
   const int tg = tile_grid(c, V);
   if (warm_round)
     k_fill_i32<<<gs, 256, 0, st>>>(B.aux32.p, V.nsub, 0);
-  k_partition_flags<<<tg, 256, 0, st>>>(V.S, V.Q, P, B.stiles.p, nt, B.flag64.p,
-                                        warm_round ? B.aux32.p : nullptr);
-  tile_scan<int64_t>(c, V, B.stiles.p, nt, B.flag64.p, B.fscan64.p, B.tsum64.p, B.tpre64.p, B.sub_first64.p,
-                     B.sub_total64.p, c->part64);
-  k_partition_scatter<<<tg, 256, 0, st>>>(V.S, V.Q, B.stiles.p, nt, round_ptr(B), B.flag64.p, B.fscan64.p,
-                                          B.sub_first64.p);
+  const int64_t maxt = slot_tile_cap(V);
+  k_partition_flags<<<tg, SCAN_THREADS, 0, st>>>(V.S, V.Q, P, B.stiles.p, nt, B.flag64.p,
+                                                 warm_round ? B.aux32.p : nullptr, B.tsum64.p, (int32_t)maxt);
+  scan_excl<int64_t>(c, B.tsum64.p, B.tpre64.p, maxt, c->part64);
+  k_partition_scatter<<<tg, SLOT_THREADS, 0, st>>>(V.S, V.Q, B.stiles.p, nt, round_ptr(B), B.flag64.p, B.tpre64.p,
+                                                   B.sub_total64.p);
   k_copy_cur<<<tg, 256, 0, st>>>(V.S, V.Q, B.stiles.p, nt, B.sub_total64.p);
   if (P.mss > 0)
   {
@@ -1177,7 +1163,7 @@ void run_final(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot
     const int32_t *nt = build_slot_tiles(c, B, V);
     k_slot_of<<<gb, 256, 0, st>>>(V.S, n, V.fvals, V.Q.cur2);
     k_mark_refine<<<gb, 256, 0, st>>>(V.S, V.Q, n, V.Q.cur2, B.flag.p);
-    build_sources_tree(c, B, V, P, nt, false, 0);
+    build_sources_tree(c, B, V, P, nt, false, 0, false);
     c->walk_items_hint = INT64_MAX; // (other targets than the rounds')
     round_walk(c, B, V, P);
     k_refine_keys<<<gb, 256, 0, st>>>(V.S, n, V.fvals, B.eref.p + V.p0, B.keys.p, B.seg_begin.p, B.seg_len.p);

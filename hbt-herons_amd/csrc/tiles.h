@@ -62,46 +62,89 @@ __global__ __launch_bounds__(SLOT_THREADS) void k_tile_sum(const int4 *__restric
   }
 }
 
-// Pass 3 (pass 2 is the exclusive scan of tsum into tpre): per-slot global
-// exclusive prefix out[q] = tpre[t] + prefix inside the tile, and per subhalo the
-// prefix at its first slot (sub_first) and its total (sub_total).  Thread i owns
-// slots 4i..4i+3 of the tile.
+// The flags of one tile, SLOT_ITEMS consecutive slots a thread, and their
+// exclusive prefix across the block (pass 2, the exclusive scan of the tile totals
+// into tpre, gives the tile's base).
 template <typename T>
-__global__ __launch_bounds__(SLOT_THREADS) void k_tile_down(SubArrays S, const int4 *__restrict__ tiles,
-                                                           const int32_t *ntiles_p, const T *__restrict__ flag,
-                                                           const T *__restrict__ tpre, T *__restrict__ out,
-                                                           T *__restrict__ sub_first, T *__restrict__ sub_total)
+HBTX_DEV T tile_flags_prefix(const int4 &tl, const T *__restrict__ flag, T v[SLOT_ITEMS], T *lds, T *tot)
 {
-  __shared__ T lds[SLOT_THREADS / HBTX_WAVE];
+  const int q0 = tl.y + SLOT_ITEMS * threadIdx.x;
+  T sum = 0;
+#pragma unroll
+  for (int k = 0; k < SLOT_ITEMS; k++)
+  {
+    v[k] = (q0 + k < tl.z) ? flag[q0 + k] : (T)0;
+    sum += v[k];
+  }
+  return block_excl_scan(sum, lds, tot);
+}
+
+// Pass 3 of the source compaction: every flagged slot to its place in the dense
+// source list (the tile's base + the prefix inside the tile), and per subhalo the
+// position of its first source.
+__global__ __launch_bounds__(SLOT_THREADS) void k_scatter_sources(SubArrays S, const int4 *tiles,
+                                                                  const int32_t *ntiles_p, const int32_t *flag,
+                                                                  const int32_t *tpre, int32_t *src,
+                                                                  int32_t *src_sub, int32_t *sub_first)
+{
+  __shared__ int32_t lds[SLOT_THREADS / HBTX_WAVE];
   TILE_LOOP(tiles, ntiles_p)
   {
     const int4 tl = tiles[t_];
+    int32_t v[SLOT_ITEMS], tot;
+    int32_t ex = tile_flags_prefix(tl, flag, v, lds, &tot) + tpre[t_];
     const int q0 = tl.y + SLOT_ITEMS * threadIdx.x;
-    T v[SLOT_ITEMS];
-    T sum = 0;
 #pragma unroll
     for (int k = 0; k < SLOT_ITEMS; k++)
     {
-      v[k] = (q0 + k < tl.z) ? flag[q0 + k] : (T)0;
-      sum += v[k];
-    }
-    T tot;
-    T ex = block_excl_scan(sum, lds, &tot) + tpre[t_];
-#pragma unroll
-    for (int k = 0; k < SLOT_ITEMS; k++)
-    {
-      if (q0 + k < tl.z)
-        out[q0 + k] = ex;
+      if (v[k])
+      {
+        src[ex] = q0 + k;
+        src_sub[ex] = tl.x;
+      }
       ex += v[k];
     }
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0 && tl.w == t_)
+      sub_first[tl.x] = tpre[t_];
+  }
+}
+
+// RemoveUnboundParticles (:35-44): stable compaction of the bound candidates of
+// each subhalo (rank = prefix of the kept flags from the subhalo's first tile);
+// removed particles are tagged with the round.  Per subhalo the totals (kept in
+// the low word, tracers in the high word) go to sub_total.
+__global__ __launch_bounds__(SLOT_THREADS) void k_partition_scatter(SubArrays S, PartArrays Q, const int4 *tiles,
+                                                                    const int32_t *ntiles_p, const int32_t *round_p,
+                                                                    const int64_t *flag, const int64_t *tpre,
+                                                                    int64_t *sub_total)
+{
+  __shared__ int64_t lds[SLOT_THREADS / HBTX_WAVE];
+  const int round = *round_p;
+  TILE_LOOP(tiles, ntiles_p)
+  {
+    const int4 tl = tiles[t_];
+    const int s = tl.x;
+    const int end = min(tl.z, S.off[s] + S.ntgt[s]);
+    const int64_t first = tpre[tl.w];
+    int64_t v[SLOT_ITEMS], tot;
+    int64_t ex = tile_flags_prefix(tl, flag, v, lds, &tot) + tpre[t_];
+    const int q0 = tl.y + SLOT_ITEMS * threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < SLOT_ITEMS; k++)
     {
-      const int s = tl.x;
-      if (tl.w == t_)
-        sub_first[s] = tpre[t_];
-      if (tl.z == S.off[s + 1])
-        sub_total[s] = tpre[t_] + tot - tpre[tl.w];
+      const int q = q0 + k;
+      if (q < end)
+      {
+        const int p = Q.cur[q];
+        if (v[k] & 0xffffffffll)
+          Q.cur2[S.off[s] + (int)((ex - first) & 0xffffffffll)] = p;
+        else
+          Q.state[p] = round;
+      }
+      ex += v[k];
     }
+    if (threadIdx.x == 0 && tl.z == S.off[s + 1])
+      sub_total[s] = tpre[t_] + tot - first;
   }
 }
 

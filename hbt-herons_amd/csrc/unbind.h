@@ -190,12 +190,18 @@ __global__ void k_mark_candidates(SubArrays S, PartArrays Q, int64_t n, int32_t 
 
 // Source flags: full mode -> candidates with rank < nsrc; correction -> removed
 // in the previous round.  Also the self mass used to cancel the self term.
-__global__ void k_mark_sources(SubArrays S, PartArrays Q, DevParams P, const int4 *tiles, const int32_t *ntiles_p,
-                               const int32_t *round_p, const int32_t *crank, int32_t *flag)
+// (also each tile's flag total in tsum[t], and 0 for the tiles [ntiles, maxt), for
+// the scan over the tiles; launched with SCAN_THREADS threads a block)
+__global__ __launch_bounds__(SCAN_THREADS) void k_mark_sources(SubArrays S, PartArrays Q, DevParams P,
+                                                               const int4 *tiles, const int32_t *ntiles_p,
+                                                               const int32_t *round_p, const int32_t *crank,
+                                                               int32_t *flag, int32_t *tsum, int32_t maxt)
 {
+  __shared__ int32_t lds[SCAN_THREADS / HBTX_WAVE];
   const int round = *round_p;
   TILE_LOOP(tiles, ntiles_p)
   {
+    int32_t cnt = 0;
     const int4 tl = tiles[t_];
     const int s = tl.x;
     const int md = S.mode[s];
@@ -246,24 +252,16 @@ __global__ void k_mark_sources(SubArrays S, PartArrays Q, DevParams P, const int
       else if (md == MODE_CORR && st == round - 1)
         f = 1;
       flag[p] = f;
+      cnt += f;
       Q.selfm[p] = sm;
     }
+    int32_t tot;
+    block_excl_scan(cnt, lds, &tot);
+    if (threadIdx.x == 0)
+      tsum[t_] = tot;
   }
-}
-
-__global__ void k_scatter_sources(SubArrays S, const int4 *tiles, const int32_t *ntiles_p, const int32_t *flag,
-                                  const int32_t *fscan, int32_t *src, int32_t *src_sub)
-{
-  TILE_LOOP(tiles, ntiles_p)
-  {
-    const int4 tl = tiles[t_];
-    for (int p = tl.y + threadIdx.x; p < tl.z; p += blockDim.x)
-      if (flag[p])
-      {
-        src[fscan[p]] = (int32_t)p;
-        src_sub[fscan[p]] = tl.x;
-      }
-  }
+  for (int t = *ntiles_p + blockIdx.x * blockDim.x + threadIdx.x; t < maxt; t += gridDim.x * blockDim.x)
+    tsum[t] = 0;
 }
 
 __global__ void k_sub_src(SubArrays S, const int32_t *sub_first)
@@ -529,11 +527,17 @@ __global__ void k_items_emit(SubArrays S, const int32_t *order, const int32_t *i
 // In a warm round every particle with E < 0 stays, proposed or not; the bound
 // particles outside the proposal are counted in warm_np[s] (their potential is
 // added to the others' energies by the MODE_ADD round that follows).
-__global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, const int4 *tiles, const int32_t *ntiles_p,
-                                  int64_t *flag, int32_t *warm_np)
+// (also each tile's flag total in tsum[t], and 0 for the tiles [ntiles, maxt);
+// launched with SCAN_THREADS threads a block)
+__global__ __launch_bounds__(SCAN_THREADS) void k_partition_flags(SubArrays S, PartArrays Q, DevParams P,
+                                                                  const int4 *tiles, const int32_t *ntiles_p,
+                                                                  int64_t *flag, int32_t *warm_np, int64_t *tsum,
+                                                                  int32_t maxt)
 {
+  __shared__ int64_t lds[SCAN_THREADS / HBTX_WAVE];
   TILE_LOOP(tiles, ntiles_p)
   {
+    int64_t cnt = 0;
     const int4 tl = tiles[t_];
     const int s = tl.x;
     const int end = min(tl.z, S.off[s] + S.ntgt[s]);
@@ -550,39 +554,19 @@ __global__ void k_partition_flags(SubArrays S, PartArrays Q, DevParams P, const 
         f = 1 | ((int64_t)(is_tracer(P, ptype_of(Q, p)) ? 1 : 0) << 32);
       }
       flag[q] = f;
+      cnt += f;
       // the tile is one subhalo's: one counter update per wave
       const uint64_t m = ballot(outside);
       if (outside && (m & lanemask_lt()) == 0ull)
         atomicAdd(warm_np + s, (int32_t)__popcll(m));
     }
+    int64_t tot;
+    block_excl_scan(cnt, lds, &tot);
+    if (threadIdx.x == 0)
+      tsum[t_] = tot;
   }
-}
-
-// RemoveUnboundParticles (:35-44): stable compaction of the bound candidates;
-// removed particles are tagged with the round.
-__global__ void k_partition_scatter(SubArrays S, PartArrays Q, const int4 *tiles, const int32_t *ntiles_p,
-                                    const int32_t *round_p, const int64_t *flag, const int64_t *fscan,
-                                    const int64_t *sub_first)
-{
-  const int round = *round_p;
-  TILE_LOOP(tiles, ntiles_p)
-  {
-    const int4 tl = tiles[t_];
-    const int s = tl.x;
-    const int end = min(tl.z, S.off[s] + S.ntgt[s]);
-    const int64_t first = sub_first[s];
-    for (int q = tl.y + threadIdx.x; q < end; q += blockDim.x)
-    {
-      const int p = Q.cur[q];
-      if (flag[q] & 0xffffffffll)
-      {
-        const int r = (int)((fscan[q] - first) & 0xffffffffll);
-        Q.cur2[S.off[s] + r] = p;
-      }
-      else
-        Q.state[p] = round;
-    }
-  }
+  for (int t = *ntiles_p + blockIdx.x * blockDim.x + threadIdx.x; t < maxt; t += gridDim.x * blockDim.x)
+    tsum[t] = 0;
 }
 
 __global__ void k_copy_cur(SubArrays S, PartArrays Q, const int4 *tiles, const int32_t *ntiles_p,
