@@ -1174,8 +1174,12 @@ void run_final(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool want_pot
   // kinematics / types over the bound particles (input order)
   k_final_ranges<<<gs, 256, 0, st>>>(V.S, B.rbegin.p, B.rlen.p);
   const int32_t *ntiles = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, n);
-  k_final_tile<<<red_grid(c, n, V.nsub), RED_THREADS, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, ntiles,
-                                                                c->red_partial.p);
+  if (V.Q.ptype)
+    k_final_tile<true><<<red_grid(c, n, V.nsub), RED_THREADS, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, ntiles,
+                                                                      c->red_partial.p);
+  else
+    k_final_tile<false><<<red_grid(c, n, V.nsub), RED_THREADS, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, ntiles,
+                                                                       c->red_partial.p);
   k_red_finalize<<<red_finalize_blocks(V.nsub), 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
   const int gw = blocks_for((int64_t)V.nsub * HBTX_WAVE, 256, 1 << 24);
   k_first_tracer<<<gw, 256, 0, st>>>(V.S, V.Q, P, V.fvals, V.first_tracer);

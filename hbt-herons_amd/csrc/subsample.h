@@ -39,24 +39,45 @@ __global__ __launch_bounds__(256) void k_init_mass_tiles(SubArrays S, PartArrays
     for (int k = 0; k < RED_NV; k++)
       v[k] = 0.;
     int mymin = 0x7fffffff;
-    for (int p = tl.y + threadIdx.x; p < tl.z; p += blockDim.x)
+    // four of this thread's slots at a time (the same order), their loads in flight together
+    for (int p0 = tl.y + threadIdx.x; p0 < tl.z; p0 += 4 * blockDim.x)
     {
-      const float4 x = Q.posm[p];
-      v[0] += x.w;
-      if (!donot_subsample(P, ptype_of(Q, p)))
-        mymin = min(mymin, p - S.off[s]);
-      if (Q.proposal && Q.proposal[p])
+      bool in[4], pro[4];
+      float4 xb[4];
+      int tb[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++)
       {
-        // proposed set: count, mass, and the mass-weighted frame of the warm round
-        const float4 u = Q.velu[p];
-        v[1] += 1.0;
-        v[2] += x.w;
-        v[3] += x.x * x.w;
-        v[4] += x.y * x.w;
-        v[5] += x.z * x.w;
-        v[6] += u.x * x.w;
-        v[7] += u.y * x.w;
-        v[8] += u.z * x.w;
+        const int p = p0 + r * blockDim.x;
+        in[r] = p < tl.z;
+        const int pc = in[r] ? p : p0;
+        xb[r] = Q.posm[pc];
+        tb[r] = ptype_of(Q, pc);
+        pro[r] = in[r] && Q.proposal && Q.proposal[pc];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+      {
+        if (!in[r])
+          continue;
+        const int p = p0 + r * blockDim.x;
+        const float4 x = xb[r];
+        v[0] += x.w;
+        if (!donot_subsample(P, tb[r]))
+          mymin = min(mymin, p - S.off[s]);
+        if (pro[r])
+        {
+          // proposed set: count, mass, and the mass-weighted frame of the warm round
+          const float4 u = Q.velu[p];
+          v[1] += 1.0;
+          v[2] += x.w;
+          v[3] += x.x * x.w;
+          v[4] += x.y * x.w;
+          v[5] += x.z * x.w;
+          v[6] += u.x * x.w;
+          v[7] += u.y * x.w;
+          v[8] += u.z * x.w;
+        }
       }
     }
     __syncthreads();

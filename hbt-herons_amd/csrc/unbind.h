@@ -1031,6 +1031,11 @@ __global__ void k_final_segments(SubArrays S, int32_t *seg_begin, int32_t *seg_l
 // of converged subhaloes (input order): v[0]=sum E m, v[1]=sum u m, v[2]=sum K m (2x),
 // v[3..5]=L m, v[6]=M, v[7..12]=per-type mass, v[13..18]=per-type count.
 
+#define FIN_BATCH 4
+// TYPED false (no particle types: every particle is type 1): the per-type sums are
+// the type-1 mass, equal to v[6] term for term, and the count, so only those
+// eight sums are carried (fewer registers) and the other type sums are zero.
+template <bool TYPED>
 __global__ __launch_bounds__(RED_THREADS) void k_final_tile(SubArrays S, PartArrays Q, DevParams P, const int4 *tiles,
                                                            const int32_t *ntiles_p, double *partial)
 {
@@ -1045,40 +1050,81 @@ __global__ __launch_bounds__(RED_THREADS) void k_final_tile(SubArrays S, PartArr
     for (int k = 0; k < RED_NV; k++)
       v[k] = 0.;
     const float *rf = S.ref + 6 * s;
-    for (int p = tl.y + threadIdx.x; p < tl.z; p += RED_THREADS)
+    // FIN_BATCH of this thread's slots at a time (the same order): their states,
+    // then the bound ones' data, all loads of a batch in flight together
+    for (int p0 = tl.y + threadIdx.x; p0 < tl.z; p0 += FIN_BATCH * RED_THREADS)
     {
-      if (Q.state[p] != 0)
-        continue;
-      const float4 x = Q.posm[p];
-      const float4 u = Q.velu[p];
-      const float m = x.w;
-      const int ty = ptype_of(Q, p);
-      v[0] += Q.energy[p] * m;
-      if (P.thermal)
-        v[1] += u.w * m;
-      double dx[3] = {(double)(x.x - rf[0]), (double)(x.y - rf[1]), (double)(x.z - rf[2])};
-      double dv[3];
-      const double xv[3] = {u.x, u.y, u.z};
-      for (int j = 0; j < 3; j++)
-      {
-        if (P.periodic)
-          dx[j] = nearest_d(dx[j], P.box, P.box_half);
-        dx[j] *= P.a;
-        dv[j] = (double)((float)xv[j] - rf[3 + j]) + P.hz * dx[j];
-        v[2] += dv[j] * dv[j] * m;
-      }
-      v[3] += (dx[1] * dv[2] - dx[2] * dv[1]) * m;
-      v[4] += (dx[2] * dv[0] - dx[0] * dv[2]) * m;
-      v[5] += (dx[0] * dv[1] - dx[1] * dv[0]) * m;
-      v[6] += m;
+      bool in[FIN_BATCH];
+      float4 xb[FIN_BATCH], ub[FIN_BATCH];
+      float eb[FIN_BATCH];
+      int tb[FIN_BATCH];
 #pragma unroll
-      for (int k = 0; k < 6; k++)
+      for (int r = 0; r < FIN_BATCH; r++)
       {
-        v[7 + k] += (ty == k) ? (double)m : 0.;
-        v[13 + k] += (ty == k) ? 1.0 : 0.;
+        const int p = p0 + r * RED_THREADS;
+        in[r] = p < tl.z && Q.state[p] == 0;
+      }
+#pragma unroll
+      for (int r = 0; r < FIN_BATCH; r++)
+      {
+        const int p = in[r] ? p0 + r * RED_THREADS : p0;
+        xb[r] = Q.posm[p];
+        ub[r] = Q.velu[p];
+        eb[r] = Q.energy[p];
+        tb[r] = TYPED ? ptype_of(Q, p) : 1;
+      }
+#pragma unroll
+      for (int r = 0; r < FIN_BATCH; r++)
+      {
+        if (!in[r])
+          continue;
+        const float4 x = xb[r];
+        const float4 u = ub[r];
+        const float m = x.w;
+        const int ty = tb[r];
+        v[0] += eb[r] * m;
+        if (P.thermal)
+          v[1] += u.w * m;
+        double dx[3] = {(double)(x.x - rf[0]), (double)(x.y - rf[1]), (double)(x.z - rf[2])};
+        double dv[3];
+        const double xv[3] = {u.x, u.y, u.z};
+        for (int j = 0; j < 3; j++)
+        {
+          if (P.periodic)
+            dx[j] = nearest_d(dx[j], P.box, P.box_half);
+          dx[j] *= P.a;
+          dv[j] = (double)((float)xv[j] - rf[3 + j]) + P.hz * dx[j];
+          v[2] += dv[j] * dv[j] * m;
+        }
+        v[3] += (dx[1] * dv[2] - dx[2] * dv[1]) * m;
+        v[4] += (dx[2] * dv[0] - dx[0] * dv[2]) * m;
+        v[5] += (dx[0] * dv[1] - dx[1] * dv[0]) * m;
+        v[6] += m;
+        if (TYPED)
+        {
+#pragma unroll
+          for (int k = 0; k < 6; k++)
+          {
+            v[7 + k] += (ty == k) ? (double)m : 0.;
+            v[13 + k] += (ty == k) ? 1.0 : 0.;
+          }
+        }
+        else
+          v[7] += 1.0;
       }
     }
-    block_reduce_vec<19>(v, lds, wave_in_tile(tl.y, tl.z)); // v[0..18]
+    if (TYPED)
+      block_reduce_vec<19>(v, lds, wave_in_tile(tl.y, tl.z)); // v[0..18]
+    else
+    {
+      block_reduce_vec<8>(v, lds, wave_in_tile(tl.y, tl.z)); // v[0..7]
+      const double cnt = v[7];
+#pragma unroll
+      for (int k = 7; k < 19; k++)
+        v[k] = 0.;
+      v[8] = v[6];
+      v[14] = cnt;
+    }
     if (threadIdx.x == 0)
       for (int k = 0; k < RED_NV; k++)
         partial[(int64_t)t * RED_NV + k] = v[k];
