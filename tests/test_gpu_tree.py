@@ -20,11 +20,21 @@ PRM = dict(theta=0.7, softening=0.001)
 def _cases():
     p0 = synth.single_nfw(0)
     rng = np.random.default_rng(3)
-    yield "nfw12k", p0.pos, p0.mass
-    yield "one", p0.pos[:1], p0.mass[:1]
-    yield "two", p0.pos[:2], p0.mass[:2]
+    yield "nfw12k", p0.pos, p0.mass, PRM
+    yield "one", p0.pos[:1], p0.mass[:1], PRM
+    yield "two", p0.pos[:2], p0.mass[:2], PRM
     u = rng.uniform(0, 1, (3000, 3)).astype(np.float32)
-    yield "uniform3k", u, np.full(3000, 0.01, np.float32)
+    yield "uniform3k", u, np.full(3000, 0.01, np.float32), PRM
+    # 6.5e5 entries: the moments cross 638 LDS windows and two levels of window
+    # groups (a finer softening keeps every particle out of the random-placement scale)
+    p1 = synth.single_nfw(4, n_halo=400000, n_inter=40000)
+    yield "nfw440k", p1.pos, p1.mass, dict(theta=0.7, softening=1e-5)
+    # near pairs: chains of single-child cells down to depth ~19, some crossing
+    # the moment windows' boundaries
+    c = rng.uniform(0, 1, (4000, 3)).astype(np.float32)
+    d = (c[:400] + np.float32(3e-6)).astype(np.float32)
+    yield "chains", np.concatenate([c, d]), rng.uniform(0.5, 1.5, 4400).astype(np.float32), \
+        dict(theta=0.7, softening=1e-7)
 
 
 def _shares_resolution_cell(pos, eps):
@@ -45,13 +55,13 @@ def _shares_resolution_cell(pos, eps):
     return len(np.unique(idx, axis=0)) < len(idx)
 
 
-@pytest.mark.parametrize("name,pos,mass", list(_cases()), ids=[c[0] for c in _cases()])
-def test_tree_structure_matches_oracle(ctx, name, pos, mass):
+@pytest.mark.parametrize("name,pos,mass,prm", list(_cases()), ids=[c[0] for c in _cases()])
+def test_tree_structure_matches_oracle(ctx, name, pos, mass, prm):
     # Below TreeNodeResolutionHalf (0.05 eps) the reference places particles at
     # random (src/oct_tree.tpp:426-438); the structure is pinned above that scale.
-    assert not _shares_resolution_cell(pos, PRM["softening"])
-    ref = O.tree_dfs(pos, mass, params=PRM)
-    t = ctx.tree(pos, mass, hbtx.make_params(**PRM))
+    assert not _shares_resolution_cell(pos, prm["softening"])
+    ref = O.tree_dfs(pos, mass, params=prm)
+    t = ctx.tree(pos, mass, hbtx.make_params(**prm))
     got = t.export()
     assert len(got["kind"]) == len(ref["kind"]), f"entry count {len(got['kind'])} vs {len(ref['kind'])}"
     assert np.array_equal(got["kind"], ref["kind"])
