@@ -1219,10 +1219,13 @@ void init_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool warm)
   k_final_ranges_all(c, B, V);
   const int32_t *nt = red_tiles(c, B.rbegin.p, B.rlen.p, V.nsub, V.n);
   k_fill_i32<<<gs, 256, 0, st>>>(B.aux32.p, V.nsub, 0x7fffffff);
+  k_fill_i32<<<gs, 256, 0, st>>>(V.S.first_idx, V.nsub, 0x7fffffff);
+  if (warm && P.periodic)
+    k_first_proposed<<<gb, 256, 0, st>>>(V.S, V.Q, V.n);
   k_init_mass_tiles<<<red_grid(c, V.n, V.nsub), 256, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, nt, c->red_partial.p,
                                                               B.aux32.p);
   k_red_finalize<<<red_finalize_blocks(V.nsub), 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
-  k_init_mass_finish<<<gs, 256, 0, st>>>(V.S, P, V.red, B.aux32.p, warm ? 1 : 0);
+  k_init_mass_finish<<<gs, 256, 0, st>>>(V.S, V.Q, P, V.red, B.aux32.p, warm ? 1 : 0);
   HIPCHK(hipGetLastError());
 }
 

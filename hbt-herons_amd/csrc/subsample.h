@@ -32,6 +32,10 @@ __global__ __launch_bounds__(256) void k_init_mass_tiles(SubArrays S, PartArrays
   {
     const int4 tl = tiles[t];
     const int s = tl.x;
+    // periodic boxes: the proposed set's positions as offsets from its first
+    // particle, wrapped to the nearest image (CentreOfMassReferenceFrame, :151-198)
+    const bool wrap = P.periodic && Q.proposal && S.first_idx[s] != 0x7fffffff;
+    const float4 r0 = wrap ? Q.posm[S.first_idx[s]] : make_float4(0.f, 0.f, 0.f, 0.f);
     if (threadIdx.x == 0)
       lmin = 0x7fffffff;
     double v[RED_NV];
@@ -71,9 +75,18 @@ __global__ __launch_bounds__(256) void k_init_mass_tiles(SubArrays S, PartArrays
           const float4 u = Q.velu[p];
           v[1] += 1.0;
           v[2] += x.w;
-          v[3] += x.x * x.w;
-          v[4] += x.y * x.w;
-          v[5] += x.z * x.w;
+          if (wrap)
+          {
+            v[3] += nearest_d((double)x.x - (double)r0.x, P.box, P.box_half) * x.w;
+            v[4] += nearest_d((double)x.y - (double)r0.y, P.box, P.box_half) * x.w;
+            v[5] += nearest_d((double)x.z - (double)r0.z, P.box, P.box_half) * x.w;
+          }
+          else
+          {
+            v[3] += x.x * x.w;
+            v[4] += x.y * x.w;
+            v[5] += x.z * x.w;
+          }
           v[6] += u.x * x.w;
           v[7] += u.y * x.w;
           v[8] += u.z * x.w;
@@ -94,8 +107,19 @@ __global__ __launch_bounds__(256) void k_init_mass_tiles(SubArrays S, PartArrays
   }
 }
 
-__global__ void k_init_mass_finish(SubArrays S, DevParams P, const double *red, const int32_t *first_sampled,
-                                   int warm_start)
+// Smallest slot of a proposed particle per subhalo (the slots hold the input order
+// when the unbinding starts), the origin of the proposed set's periodic frame.
+__global__ void k_first_proposed(SubArrays S, PartArrays Q, int64_t n)
+{
+  GRID_STRIDE(p, n)
+  {
+    if (Q.proposal[p])
+      atomicMin(S.first_idx + S.sub_of[p], (int32_t)p);
+  }
+}
+
+__global__ void k_init_mass_finish(SubArrays S, PartArrays Q, DevParams P, const double *red,
+                                   const int32_t *first_sampled, int warm_start)
 {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S.nsub)
@@ -111,12 +135,14 @@ __global__ void k_init_mass_finish(SubArrays S, DevParams P, const double *red, 
     // whether they survive depends on the iteration's path
     S.warm[s] = WARM_PENDING;
     S.npro[s] = (int)v[1];
-    if (!P.periodic)
-      for (int k = 0; k < 3; k++)
-      {
-        S.ref[6 * s + k] = (float)(v[3 + k] / v[2]);
-        S.ref[6 * s + 3 + k] = (float)(v[6 + k] / v[2]);
-      }
+    // (periodic: the wrapped mean offset from the first proposed particle, added back)
+    const float4 r0 = P.periodic ? Q.posm[S.first_idx[s]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const double o[3] = {r0.x, r0.y, r0.z};
+    for (int k = 0; k < 3; k++)
+    {
+      S.ref[6 * s + k] = (float)(v[3 + k] / v[2] + (P.periodic ? o[k] : 0.0));
+      S.ref[6 * s + 3 + k] = (float)(v[6 + k] / v[2]);
+    }
   }
   // Nunsample (:522-524): only when subsampling can apply to this subhalo
   S.nuns[s] = (P.mss > 0 && n > P.mss) ? min(first_sampled[s], n) : 0;

@@ -285,3 +285,54 @@ def test_warm_start_per_subhalo_gate(ctx, evolved_large, trained_large, k, prec)
     assert np.all(per <= 1e-3), (np.where(per > 1e-3)[0], per[per > 1e-3])
     assert np.all(per[~warmed] == 0.0)
     assert np.array_equal(warm["result"]["disrupted"], plain["result"]["disrupted"])
+
+
+def hydro_periodic_population():
+    """The hydro mixture (types, thermal energy, tracers) wrapped so that every
+    subhalo straddles the edge of a 50-unit periodic box."""
+    pop = synth.population(seed=41, nsub=50, nmax=15000, hydro=True)
+    pos = pop.pos.copy()
+    box = 50.0
+    for s in range(pop.nsub):
+        a, b = pop.offsets[s], pop.offsets[s + 1]
+        c = pos[a:b].mean(0)
+        pos[a:b] = np.mod(pos[a:b] - c + 0.01, box)
+    return synth.Population(pos, pop.vel, pop.mass, pop.offsets, pop.pid, pop.ptype, pop.u,
+                            ref_pos=np.full((pop.nsub, 3), 0.01, np.float32), ref_vel=pop.ref_vel)
+
+
+@pytest.mark.parametrize("kind", ["exact", "noisy"])
+def test_warm_start_hydro_periodic_cosmological(ctx, kind):
+    """The warm start on the hydro, periodic, cosmological path (types and thermal
+    energy, minimum-image offsets, a = 0.5 with H != 0): the GPU follows the
+    oracle's warm-start restatement (bound sets, energies, type counts), and the
+    plain iteration judges both (summed bound mass within 1e-3)."""
+    pop = hydro_periodic_population()
+    prm = hbtx.workload_params(dm_only=0, thermal=1, periodic=1, box_size=50.0, scale_factor=0.5, hz=1000.0)
+    plain = ctx.unbind(pop, prm)
+    prop = learned.labels(pop, plain).astype(np.uint8)
+    if kind == "noisy":
+        rng = np.random.default_rng(8)
+        flip = rng.random(pop.n) < 0.05
+        prop = np.where(flip, 1 - prop, prop).astype(np.uint8)
+    ctx.load(pop)
+    ctx.set_proposal(prop)
+    ctx.run(prm, hbtx.WARMSTART | hbtx.SAVE_POTENTIAL)
+    warm = ctx.fetch()
+    opr = O.default_params(**{k: v for k, v in prm.items() if k != "direct_cutoff"})
+    ref = O.unbind_batch(pop, opr, want_potential=True, proposal=prop)
+    frac, _ = bound_mass_agreement(warm, ref, pop)
+    err = energy_errors(warm, ref, pop, ref["potential"])
+    print(f"{kind}: GPU vs oracle warm start: mismatch {frac:.2e}, energy q99.9 {np.quantile(err, 0.999):.2e}")
+    assert frac <= 1e-4, frac
+    assert np.quantile(err, 0.999) < 1e-5
+    g, r = warm["result"], ref["result"]
+    assert np.array_equal(g["disrupted"], r["disrupted"])
+    same = (g["disrupted"] == 0) & (g["nbound"] == r["nbound"])
+    assert same.mean() >= 0.9
+    assert np.array_equal(g["nbound_type"][same], r["nbound_type"][same])
+    assert np.array_equal(g["nbound_type"].sum(1)[g["disrupted"] == 0], g["nbound"][g["disrupted"] == 0])
+    assert order_is_reference_layout(warm, pop)
+    frac_p, _ = bound_mass_agreement(warm, plain, pop)
+    assert frac_p <= 1e-3, frac_p
+    assert np.array_equal(g["disrupted"], plain["result"]["disrupted"])
