@@ -161,6 +161,35 @@ def test_hydro_cosmological_matches_oracle(ctx):
     assert np.array_equal(g["mostbound_id"][alive], r["mostbound_id"][alive])
 
 
+def test_subsampled_refined_cosmological_periodic_matches_oracle(ctx):
+    # MaxSampleSizeOfPotentialEstimate with RefineMostBoundParticle on the periodic,
+    # a < 1, H != 0 path, hydro types (upscale per type, do-not-subsample type 5)
+    from test_gpu_unbind import prepare_for_subsampling
+    pop = prepare_for_subsampling(synth.population(seed=34, nsub=40, nmax=20000, hydro=True))
+    pos = pop.pos.copy()
+    box = 50.0
+    for s in range(pop.nsub):
+        a, b = pop.offsets[s], pop.offsets[s + 1]
+        c = pos[a:b].mean(0)
+        pos[a:b] = np.mod(pos[a:b] - c + 0.01, box)
+    p2 = synth.Population(pos, pop.vel, pop.mass, pop.offsets, pop.pid, pop.ptype, pop.u,
+                          ref_pos=np.full((pop.nsub, 3), 0.01, np.float32), ref_vel=pop.ref_vel)
+    prm = hbtx.workload_params(max_sample_size=1000, refine_most_bound=1, dm_only=0, thermal=1, periodic=1,
+                               box_size=box, scale_factor=0.5, hz=1000.0, theta=0.45)
+    gpu = ctx.unbind(p2, prm, hbtx.BATCHED | hbtx.SAVE_POTENTIAL)
+    ref = O.unbind_batch(p2, oracle_params(prm), want_potential=True)
+    frac, _ = bound_mass_agreement(gpu, ref, p2)
+    assert frac <= 1e-3, frac
+    err = energy_errors(gpu, ref, p2, ref["potential"])
+    assert np.quantile(err, 0.999) < 1e-4, np.quantile(err, 0.999)
+    assert np.array_equal(gpu["result"]["disrupted"], ref["result"]["disrupted"])
+    assert order_is_reference_layout(gpu, p2, 1000)
+    g, r = gpu["result"], ref["result"]
+    same = (g["disrupted"] == 0) & (g["nbound"] == r["nbound"])
+    assert same.mean() >= 0.9
+    np.testing.assert_allclose(g["com_pos"][same], r["com_pos"][same], rtol=0, atol=2e-6)
+
+
 def test_periodic_cosmological_matches_oracle(ctx):
     pop = synth.population(seed=33, nsub=40, nmax=6000)
     pos = pop.pos.copy()
