@@ -2,7 +2,7 @@
 // reference's member names (Particle_t, Subhalo_t subsets) and writes the
 // results for tests/test_gpu_compat.py to compare with the C ABI's.
 //
-// usage: compat_test <input.bin> <output.bin>
+// usage: compat_test <input.bin> <output.bin> [hydro]
 //   input : int64 n, nsub; int64 offsets[nsub+1]; float pos[n][3], vel[n][3],
 //           mass[n], ref_pos[nsub][3], ref_vel[nsub][3]
 //   output: per subhalo int64 nbound, int64 mostbound_id, float mbound,
@@ -12,10 +12,15 @@
 //           first k = min(16, n0) particles; then per subhalo float
 //           {Rmax, Vmax, RHalf, REnclose, Ixx, Izz weighted}, int32
 //           SnapshotOfLastMaxVmax.
+//   hydro: the input also holds int32 type[n], float u[n]; particles carry Type
+//           and InternalEnergy, subhaloes NboundType / MboundType; unbinding with
+//           thermal energy; output per subhalo int64 nbound, int64 NboundType[6],
+//           float MboundType[6], float mbound, int64 TracerIndex; then int64 ids[n].
 #include "hbtx_compat.hpp"
 
 #include <array>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 typedef std::array<float, 3> HBTxyz;
@@ -67,6 +72,31 @@ struct SnapView
   }
 };
 
+struct HydroParticle_t
+{
+  int64_t Id;
+  HBTxyz ComovingPosition;
+  HBTxyz PhysicalVelocity;
+  float Mass;
+  float InternalEnergy;
+  int Type;
+};
+
+struct HydroSubhalo_t
+{
+  int64_t Nbound = 0;
+  float Mbound = 0;
+  int64_t NboundType[6] = {0, 0, 0, 0, 0, 0};
+  float MboundType[6] = {0, 0, 0, 0, 0, 0};
+  int64_t TracerIndex = 0;
+  float SpecificSelfPotentialEnergy = 0, SpecificSelfKineticEnergy = 0, SpecificAngularMomentum[3] = {0, 0, 0};
+  HBTxyz ComovingAveragePosition, PhysicalAverageVelocity, ComovingMostBoundPosition, PhysicalMostBoundVelocity;
+  int64_t MostBoundParticleId = -1;
+  HBTxyz CoreComovingPosition, CorePhysicalVelocity;
+  float CoreComovingSigmaR = 0, CorePhysicalSigmaV = 0;
+  std::vector<HydroParticle_t> Particles;
+};
+
 template <class T>
 static bool rd(FILE *f, T *p, size_t n)
 {
@@ -78,9 +108,75 @@ static void wr(FILE *f, const T *p, size_t n)
   fwrite(p, sizeof(T), n, f);
 }
 
+static int hydro(const std::vector<int64_t> &off, const std::vector<float> &pos, const std::vector<float> &vel,
+                 const std::vector<float> &mass, const std::vector<float> &rp, const std::vector<float> &rv,
+                 const std::vector<int32_t> &type, const std::vector<float> &u, const char *outp)
+{
+  const int64_t nsub = (int64_t)off.size() - 1;
+  std::vector<HydroSubhalo_t> subs(nsub);
+  for (int64_t s = 0; s < nsub; s++)
+  {
+    HydroSubhalo_t &S = subs[s];
+    for (int k = 0; k < 3; k++)
+    {
+      S.ComovingAveragePosition[k] = S.ComovingMostBoundPosition[k] = rp[3 * s + k];
+      S.PhysicalAverageVelocity[k] = S.PhysicalMostBoundVelocity[k] = rv[3 * s + k];
+    }
+    for (int64_t i = off[s]; i < off[s + 1]; i++)
+    {
+      HydroParticle_t p;
+      p.Id = i - off[s];
+      for (int k = 0; k < 3; k++)
+      {
+        p.ComovingPosition[k] = pos[3 * i + k];
+        p.PhysicalVelocity[k] = vel[3 * i + k];
+      }
+      p.Mass = mass[i];
+      p.InternalEnergy = u[i];
+      p.Type = type[i];
+      S.Particles.push_back(p);
+    }
+  }
+  try
+  {
+    hbtx::Context ctx(0);
+    hbtx_params prm = hbtx::DefaultParams();
+    prm.theta = 0.7f;
+    prm.max_sample_size = 0;
+    prm.periodic = 0;
+    prm.direct_cutoff = 0;
+    prm.dm_only = 0;
+    prm.thermal = 1;
+    std::vector<HydroSubhalo_t *> ptrs;
+    for (auto &s : subs)
+      ptrs.push_back(&s);
+    hbtx::RefineParticles(ctx, prm, ptrs, 7);
+  }
+  catch (const hbtx::Error &e)
+  {
+    fprintf(stderr, "hbtx error %d: %s\n", e.code, e.what());
+    return 1;
+  }
+  FILE *o = fopen(outp, "wb");
+  for (auto &S : subs)
+  {
+    wr(o, &S.Nbound, 1);
+    wr(o, S.NboundType, 6);
+    wr(o, S.MboundType, 6);
+    wr(o, &S.Mbound, 1);
+    wr(o, &S.TracerIndex, 1);
+  }
+  for (auto &S : subs)
+    for (auto &p : S.Particles)
+      wr(o, &p.Id, 1);
+  fclose(o);
+  return 0;
+}
+
 int main(int argc, char **argv)
 {
-  if (argc != 3)
+  const bool with_types = argc == 4 && std::string(argv[3]) == "hydro";
+  if (argc != 3 && !with_types)
     return 2;
   FILE *f = fopen(argv[1], "rb");
   if (!f)
@@ -94,6 +190,15 @@ int main(int argc, char **argv)
   if (!rd(f, off.data(), nsub + 1) || !rd(f, pos.data(), 3 * n) || !rd(f, vel.data(), 3 * n) ||
       !rd(f, mass.data(), n) || !rd(f, rp.data(), 3 * nsub) || !rd(f, rv.data(), 3 * nsub))
     return 2;
+  if (with_types)
+  {
+    std::vector<int32_t> type(n);
+    std::vector<float> u(n);
+    if (!rd(f, type.data(), n) || !rd(f, u.data(), n))
+      return 2;
+    fclose(f);
+    return hydro(off, pos, vel, mass, rp, rv, type, u, argv[2]);
+  }
   fclose(f);
   auto make = [&](std::vector<Subhalo_t> &subs) {
     subs.assign(nsub, Subhalo_t());

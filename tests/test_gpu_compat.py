@@ -78,3 +78,36 @@ def test_cpp_host_layer_matches_c_abi(ctx, tmp_path):
     np.testing.assert_array_equal(pr["f"][:, 5], g["inertia_w"][:, 5])
     live = res["nbound"] > 1
     assert np.all(pr["snap"][live] == 7) and np.all(pr["snap"][~live] == -1)
+
+
+def test_cpp_host_layer_hydro_types_and_thermal(ctx, tmp_path):
+    """The host layer with particle types and internal energy (the hydro build:
+    Type, InternalEnergy; NboundType / MboundType written back): the same bound
+    sets, permutations, per-type counts and masses and tracer index as the C ABI
+    with the same types and u."""
+    assert os.path.exists(BIN), "build the C++ test driver first (make -C hbt-herons_amd)"
+    pop = synth.population(seed=32, nsub=40, nmax=5000, hydro=True)
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        np.array([pop.n, pop.nsub], np.int64).tofile(f)
+        pop.offsets.astype(np.int64).tofile(f)
+        for a in (pop.pos, pop.vel, pop.mass, pop.ref_pos, pop.ref_vel):
+            np.ascontiguousarray(a, np.float32).tofile(f)
+        np.ascontiguousarray(pop.ptype, np.int32).tofile(f)
+        np.ascontiguousarray(pop.u, np.float32).tofile(f)
+    r = subprocess.run([BIN, str(inp), str(outp), "hydro"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rec = np.dtype([("nbound", np.int64), ("nbt", np.int64, 6), ("mbt", np.float32, 6), ("mbound", np.float32),
+                    ("tracer", np.int64)])
+    raw = open(outp, "rb").read()
+    subs = np.frombuffer(raw, rec, count=pop.nsub)
+    ids = np.frombuffer(raw, np.int64, count=pop.n, offset=rec.itemsize * pop.nsub)
+    ref = ctx.unbind(pop, hbtx.workload_params(dm_only=0, thermal=1), hbtx.BATCHED)
+    res = ref["result"]
+    assert np.array_equal(subs["nbound"], res["nbound"])
+    assert np.array_equal(ids, ref["order"])
+    assert np.array_equal(subs["nbt"], res["nbound_type"])
+    assert np.array_equal(subs["mbt"], res["mbound_type"])
+    assert np.array_equal(subs["mbound"], res["mbound"])
+    assert np.array_equal(subs["tracer"], res["tracer_index"])
+    assert res["nbound_type"][:, 0].sum() > 0  # gas is bound somewhere
