@@ -372,6 +372,8 @@ void seg_sort(hbtx_ctx *c, uint64_t *keys, int32_t *vals, int32_t nseg, const in
   reserve_sort(c, n_cap, nseg);
   k_sort_wave<<<std::max(1, std::min((nseg + 3) / 4, 16384)), SORT_THREADS, 0, c->stream>>>(keys, vals, seg_begin,
                                                                                             seg_len, nseg);
+  k_sort_wave4<<<std::max(1, std::min((nseg + 3) / 4, 16384)), SORT_THREADS, 0, c->stream>>>(keys, vals, seg_begin,
+                                                                                            seg_len, nseg);
   k_sort_small<<<std::max(1, std::min(nseg, 16384)), SORT_THREADS, 0, c->stream>>>(keys, vals, seg_begin, seg_len,
                                                                                     nseg);
   if (small_only)
@@ -1222,8 +1224,12 @@ void init_view(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool warm)
   k_fill_i32<<<gs, 256, 0, st>>>(V.S.first_idx, V.nsub, 0x7fffffff);
   if (warm && P.periodic)
     k_first_proposed<<<gb, 256, 0, st>>>(V.S, V.Q, V.n);
-  k_init_mass_tiles<<<red_grid(c, V.n, V.nsub), 256, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, nt, c->red_partial.p,
-                                                              B.aux32.p);
+  if (P.periodic)
+    k_init_mass_tiles<true><<<red_grid(c, V.n, V.nsub), 256, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, nt, c->red_partial.p,
+                                                                B.aux32.p);
+  else
+    k_init_mass_tiles<false><<<red_grid(c, V.n, V.nsub), 256, 0, st>>>(V.S, V.Q, P, c->red_tiles.p, nt, c->red_partial.p,
+                                                                B.aux32.p);
   k_red_finalize<<<red_finalize_blocks(V.nsub), 256, 0, st>>>(c->red_toff.p, c->red_ntile.p, V.nsub, c->red_partial.p, V.red);
   k_init_mass_finish<<<gs, 256, 0, st>>>(V.S, V.Q, P, V.red, B.aux32.p, warm ? 1 : 0);
   HIPCHK(hipGetLastError());

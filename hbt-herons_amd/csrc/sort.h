@@ -17,6 +17,7 @@ namespace hbtx
 #endif
 #define SORT_SMALL_MAX 2048
 #define SORT_WAVE_MAX HBTX_WAVE // segments of at most one wave: k_sort_wave
+#define SORT_WAVE4_MAX (4 * HBTX_WAVE) // segments of at most four per lane: k_sort_wave4
 #ifndef SORT_TILE
 #define SORT_TILE 2048
 #endif
@@ -32,8 +33,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_small(uint64_t *__restric
   for (int s = blockIdx.x; s < nseg; s += gridDim.x)
   {
     const int len = seg_len[s];
-    if (len <= SORT_WAVE_MAX || len > SORT_SMALL_MAX)
-      continue; // (one-wave segments: k_sort_wave)
+    if (len <= SORT_WAVE4_MAX || len > SORT_SMALL_MAX)
+      continue; // (segments of up to four per lane: k_sort_wave, k_sort_wave4)
     const int b = seg_begin[s];
     int P = 1;
     while (P < len)
@@ -114,6 +115,93 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_wave(uint64_t *__restrict
     {
       keys[b + lane] = key;
       vals[b + lane] = val;
+    }
+  }
+}
+
+// Segments of SORT_WAVE_MAX + 1 .. SORT_WAVE4_MAX elements: one wave each, element
+// lane + 64 r in register r, padded to 128 or 256 like k_sort_small, and the same
+// bitonic network (so the same permutation, ties included): pairs less than 64
+// apart meet through lane shuffles, pairs 64 or more apart in a lane's registers.
+// No LDS and no barrier.
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_wave4(uint64_t *__restrict__ keys, int32_t *__restrict__ vals,
+                                                            const int32_t *__restrict__ seg_begin,
+                                                            const int32_t *__restrict__ seg_len, int32_t nseg)
+{
+  const int lane = lane_id();
+  const int wpb = SORT_THREADS / HBTX_WAVE;
+  for (int s = uniform(blockIdx.x * wpb + (int)(threadIdx.x / HBTX_WAVE)); s < nseg; s += gridDim.x * wpb)
+  {
+    const int len = seg_len[s];
+    if (len <= SORT_WAVE_MAX || len > SORT_WAVE4_MAX)
+      continue;
+    const int b = seg_begin[s];
+    const int P = (len > 2 * HBTX_WAVE) ? 4 * HBTX_WAVE : 2 * HBTX_WAVE;
+    const int R = P / HBTX_WAVE; // registers in use: 2 or 4
+    uint64_t key[4];
+    int32_t val[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+    {
+      const int e = lane + HBTX_WAVE * r;
+      key[r] = (r < R && e < len) ? keys[b + e] : ~0ull;
+      val[r] = (r < R && e < len) ? vals[b + e] : 0;
+    }
+    for (int k = 2; k <= P; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1)
+      {
+        if (j < HBTX_WAVE)
+        {
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+          {
+            if (r >= R)
+              continue;
+            const int e = lane + HBTX_WAVE * r;
+            const uint64_t ok = __shfl_xor(key[r], j, HBTX_WAVE);
+            const int32_t ov = __shfl_xor(val[r], j, HBTX_WAVE);
+            const bool low = (lane & j) == 0; // this lane holds the pair's first element
+            const bool up = (e & k) == 0;     // the pair's direction (the same for both lanes)
+            const uint64_t a = low ? key[r] : ok, c = low ? ok : key[r];
+            if ((a > c) == up)
+            {
+              key[r] = ok;
+              val[r] = ov;
+            }
+          }
+        }
+        else
+        {
+          const int jr = j / HBTX_WAVE; // the pair's registers differ by jr (1 or 2)
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+          {
+            const int r2 = r ^ jr;
+            if (r >= R || r2 < r)
+              continue; // (r2 < R: pairs stay inside the padded length)
+            const int e = lane + HBTX_WAVE * r;
+            const bool up = (e & k) == 0;
+            if ((key[r] > key[r2]) == up)
+            {
+              const uint64_t tk = key[r];
+              key[r] = key[r2];
+              key[r2] = tk;
+              const int32_t tv = val[r];
+              val[r] = val[r2];
+              val[r2] = tv;
+            }
+          }
+        }
+      }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+    {
+      const int e = lane + HBTX_WAVE * r;
+      if (r < R && e < len)
+      {
+        keys[b + e] = key[r];
+        vals[b + e] = val[r];
+      }
     }
   }
 }

@@ -21,6 +21,8 @@ namespace hbtx
 
 // Initial bound mass (AccumulateMass over all particles, :537-541) and the
 // first particle that may be subsampled (end of the leading do-not-subsample run).
+// PERIODIC: the box is periodic (the proposed set's frame then wraps about its first particle)
+template <bool PERIODIC>
 __global__ __launch_bounds__(256) void k_init_mass_tiles(SubArrays S, PartArrays Q, DevParams P, const int4 *tiles,
                                                          const int32_t *ntiles_p, double *partial,
                                                          int32_t *first_sampled)
@@ -34,7 +36,7 @@ __global__ __launch_bounds__(256) void k_init_mass_tiles(SubArrays S, PartArrays
     const int s = tl.x;
     // periodic boxes: the proposed set's positions as offsets from its first
     // particle, wrapped to the nearest image (CentreOfMassReferenceFrame, :151-198)
-    const bool wrap = P.periodic && Q.proposal && S.first_idx[s] != 0x7fffffff;
+    const bool wrap = PERIODIC && Q.proposal && S.first_idx[s] != 0x7fffffff;
     const float4 r0 = wrap ? Q.posm[S.first_idx[s]] : make_float4(0.f, 0.f, 0.f, 0.f);
     if (threadIdx.x == 0)
       lmin = 0x7fffffff;

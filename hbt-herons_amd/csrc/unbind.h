@@ -253,7 +253,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_mark_sources(SubArrays S, Part
         f = 1;
       flag[p] = f;
       cnt += f;
-      Q.selfm[p] = sm;
+      if (md != MODE_CORR) // (a correction's walk reads no self mass)
+        Q.selfm[p] = sm;
     }
     int32_t tot;
     block_excl_scan(cnt, lds, &tot);

@@ -133,7 +133,7 @@ def test_small_sorts_follow_the_bitonic_network(ctx):
     """Segments of up to 2048 keys: keys and the order of tied keys' values equal
     the bitonic network's, whichever barriers separate its stages on the GPU."""
     rng = np.random.default_rng(11)
-    sizes = [33, 64, 65, 100, 128, 129, 255, 300, 512, 513, 700, 1024, 1500, 2047, 2048]
+    sizes = [33, 64, 65, 100, 127, 128, 129, 192, 255, 256, 257, 300, 512, 513, 700, 1024, 1500, 2047, 2048]
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     n = int(off[-1])
     keys = rng.integers(0, 8, n).astype(np.uint64)  # many ties
