@@ -380,6 +380,7 @@ def main():
     t0 = time.perf_counter()
     prof = ctx.profiles(prm)
     prof_ms = (time.perf_counter() - t0) * 1e3
+    dev_bytes = hbtx.device_bytes()
     st["walk_ms"] = walk_ms
     ms_step = float(np.mean(ms_learned))
     value = pop.n / (ms_step / 1e3)
@@ -432,6 +433,9 @@ def main():
                     "mean_iterations": float(res["iterations"].mean()),
                     "mean_iterations_plain": float(ref["result"]["iterations"].mean())},
         "walk_stats": {k: v for k, v in st.items() if not k.startswith("unused")},
+        "device_memory": {"bytes": dev_bytes, "bytes_per_particle": dev_bytes / pop.n,
+                          "note": "device memory the library's buffers held after the timed runs "
+                                  "(batch arrays, tree entry reservation, sort and reduction scratch, network)"},
     }
 
     if sweep is not None:

@@ -92,6 +92,7 @@ void *scratch_address(int device)
 // hold buffer addresses, so a graph is valid only for the generation it was
 // captured in.
 std::atomic<uint64_t> g_alloc_gen{0};
+std::atomic<int64_t> g_device_bytes{0}; // device memory the library's buffers hold, all contexts
 
 template <class T>
 struct DArr
@@ -113,6 +114,7 @@ struct DArr
       throw Error(HBTX_E_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
     }
     n = count;
+    g_device_bytes += (int64_t)bytes;
   }
   void release()
   {
@@ -120,6 +122,7 @@ struct DArr
     {
       (void)hipFree(p);
       g_alloc_gen++;
+      g_device_bytes -= (int64_t)(std::max<size_t>(n, 1) * sizeof(T));
     }
     p = nullptr;
     n = 0;
@@ -1976,6 +1979,14 @@ int hbtx_batch_stats(hbtx_ctx *c, double *stats, int nstats)
     if (nstats > 15)
       stats[15] = c->graph_captures;
   });
+}
+
+int hbtx_device_bytes(int64_t *bytes)
+{
+  if (!bytes)
+    return HBTX_E_INVALID;
+  *bytes = g_device_bytes.load();
+  return 0;
 }
 
 const char *hbtx_stat_name(int i)

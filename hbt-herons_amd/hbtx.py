@@ -135,6 +135,7 @@ def lib():
         L.hbtx_batch_run.argtypes = [P, C.POINTER(Params), C.c_uint32, C.POINTER(C.c_double)]
         L.hbtx_batch_fetch.argtypes = [P, P, P, P, P]
         L.hbtx_batch_stats.argtypes = [P, P, C.c_int]
+        L.hbtx_device_bytes.argtypes = [C.POINTER(C.c_int64)]
         L.hbtx_stat_name.argtypes = [C.c_int]
         L.hbtx_stat_name.restype = C.c_char_p
         L.hbtx_tree_build.argtypes = [P, C.POINTER(Params), i64, P, P, C.POINTER(C.c_void_p), C.POINTER(i64)]
@@ -170,6 +171,15 @@ def lib():
         L.hbtx_set_tuning.argtypes = [P, i64, C.c_int32]
         _lib = L
     return _lib
+
+
+def device_bytes():
+    """Device memory the library's buffers hold now (all contexts of the process)."""
+    b = C.c_int64(0)
+    rc = lib().hbtx_device_bytes(C.byref(b))
+    if rc != 0:
+        raise HbtxError(rc, "hbtx_device_bytes failed")
+    return b.value
 
 
 def check_layout(n, offsets):

@@ -199,6 +199,10 @@ int hbtx_reassign_targets(hbtx_ctx *ctx, const hbtx_params *prm, int64_t n, cons
  * 12 walk ms, 13 rounds, 14 rounds replayed from graphs, 15 graph captures. */
 int hbtx_batch_stats(hbtx_ctx *ctx, double *stats, int nstats);
 const char *hbtx_stat_name(int i);
+/* Device memory the library's buffers hold now, over all contexts of the process
+ * (batch arrays, tree entries and records, sort and reduction scratch, trees,
+ * networks); no counterpart in the reference (a footprint diagnostic). */
+int hbtx_device_bytes(int64_t *bytes);
 
 /* GravityTree_t replacement.  Build over n sources (pos float[n][3], mass float[n]);
  * *ncells receives the number of tree cells (OctTree_t::Build's return value). */
