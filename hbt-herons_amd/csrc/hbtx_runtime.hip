@@ -933,6 +933,7 @@ WalkArgs walk_args(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   W.eref = B.eref.p + V.p0;
   W.phi_out = nullptr;
   W.stats = c->stats.p;
+  W.overflow = B.misc.p + 2;
   W.P = P;
   return W;
 }
@@ -959,21 +960,22 @@ void round_walk_launch(hbtx_ctx *c, Batch &B, View &V, const DevParams &P)
   // (the persistent walk needs no more waves than items; a round has no more
   // items than the round before it, as every subhalo's target count only shrinks)
   const int64_t maxitems = std::min<int64_t>(V.n / HBTX_WAVE + V.nsub + 1, c->walk_items_hint);
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxitems, (int64_t)c->walk_waves * c->num_cu));
+  const int waves = (int)std::max<int64_t>(1, std::min<int64_t>(maxitems, (int64_t)c->walk_waves * c->num_cu));
+  const int grid = (waves + HBTX_WALK_WG_WAVES - 1) / HBTX_WALK_WG_WAVES;
   HIPCHK(hipEventRecord(c->wev0, st));
   if (P.periodic)
   {
     if (c->count_work)
-      k_walk_energy<true, true><<<grid, 64, 0, st>>>(W);
+      k_walk_energy<true, true><<<grid, WALK_WG_THREADS, 0, st>>>(W);
     else
-      k_walk_energy<true, false><<<grid, 64, 0, st>>>(W);
+      k_walk_energy<true, false><<<grid, WALK_WG_THREADS, 0, st>>>(W);
   }
   else
   {
     if (c->count_work)
-      k_walk_energy<false, true><<<grid, 64, 0, st>>>(W);
+      k_walk_energy<false, true><<<grid, WALK_WG_THREADS, 0, st>>>(W);
     else
-      k_walk_energy<false, false><<<grid, 64, 0, st>>>(W);
+      k_walk_energy<false, false><<<grid, WALK_WG_THREADS, 0, st>>>(W);
   }
   HIPCHK(hipEventRecord(c->wev1, st));
   HIPCHK(hipGetLastError());
@@ -1343,9 +1345,11 @@ bool ensure_round_graphs(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, int
 }
 
 // The overflow word of a round: bit 1 the entry buffers were too small, bit 2 a
-// tree's records exceed the walk's int32 byte offsets.
+// tree's records exceed the walk's int32 byte offsets, bit 4 a walk stack was full
+// (not reachable for trees of at most 22 cell levels).
 void check_overflow(int32_t ov)
 {
+  REQUIRE(!(ov & WALK_STACK_OVERFLOW), HBTX_E_CAPACITY, "tree walk stack capacity exceeded");
   REQUIRE(!(ov & 2), HBTX_E_CAPACITY,
           "a subhalo's octree exceeds the walk's 2 GiB record span (int32 byte offsets)");
   REQUIRE(ov == 0, HBTX_E_CAPACITY, "tree entry capacity exceeded");
@@ -1670,7 +1674,7 @@ void tree_points(hbtx_ctx *c, hbtx_tree *T, int64_t nt, const float *tpos, const
   vals.alloc(nt);
   segb.alloc(1);
   segl.alloc(1);
-  counter.alloc(1);
+  counter.alloc(2); // [0] work counter, [1] overflow word
   std::vector<float4> h(nt);
   for (int64_t i = 0; i < nt; i++)
     h[i] = make_float4(tpos[3 * i], tpos[3 * i + 1], tpos[3 * i + 2], tmass ? tmass[i] : 0.f);
@@ -1682,7 +1686,7 @@ void tree_points(hbtx_ctx *c, hbtx_tree *T, int64_t nt, const float *tpos, const
   HIPCHK(hipMemcpyAsync(segl.p, &hl, 4, hipMemcpyHostToDevice, st));
   if (T->treemode)
     seg_sort(c, keys.p, vals.p, 1, segb.p, segl.p, nt, 63);
-  HIPCHK(hipMemsetAsync(counter.p, 0, 4, st));
+  HIPCHK(hipMemsetAsync(counter.p, 0, 8, st));
   PointArgs A;
   A.tgt = dt.p;
   A.tgt_order = vals.p;
@@ -1695,28 +1699,33 @@ void tree_points(hbtx_ctx *c, hbtx_tree *T, int64_t nt, const float *tpos, const
   A.phi_d = phid.p;
   A.counter = counter.p;
   A.stats = c->stats.p;
+  A.overflow = counter.p + 1;
   A.P = T->P;
   HIPCHK(hipMemsetAsync(c->stats.p, 0, NSTATS * sizeof(unsigned long long), st));
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nt / HBTX_WAVE + 1, 32 * c->num_cu));
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nt / HBTX_WAVE + 1, 32 * c->num_cu) /
+                                                 HBTX_WALK_WG_WAVES);
   HIPCHK(hipEventRecord(c->wev0, st));
   if (T->P.periodic)
   {
     if (c->count_work)
-      k_points_potential<true, true><<<grid, 64, 0, st>>>(A);
+      k_points_potential<true, true><<<grid, WALK_WG_THREADS, 0, st>>>(A);
     else
-      k_points_potential<true, false><<<grid, 64, 0, st>>>(A);
+      k_points_potential<true, false><<<grid, WALK_WG_THREADS, 0, st>>>(A);
   }
   else
   {
     if (c->count_work)
-      k_points_potential<false, true><<<grid, 64, 0, st>>>(A);
+      k_points_potential<false, true><<<grid, WALK_WG_THREADS, 0, st>>>(A);
     else
-      k_points_potential<false, false><<<grid, 64, 0, st>>>(A);
+      k_points_potential<false, false><<<grid, WALK_WG_THREADS, 0, st>>>(A);
   }
   HIPCHK(hipEventRecord(c->wev1, st));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, phid.p, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+  int32_t ov = 0;
+  HIPCHK(hipMemcpyAsync(&ov, counter.p + 1, 4, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  REQUIRE(ov == 0, HBTX_E_CAPACITY, "tree walk stack capacity exceeded");
   float wms = 0.f;
   HIPCHK(hipEventElapsedTime(&wms, c->wev0, c->wev1));
   c->walk_ms = wms;
@@ -2191,12 +2200,14 @@ int hbtx_direct_potential_noalloc(hbtx_ctx *c, const hbtx_params *prm, int64_t n
       A.phi_d = d_out_pot;
       A.counter = counter;
       A.stats = nullptr;
+      A.overflow = nullptr; // direct sums only
       A.P = P;
-      const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(n_tgt / HBTX_WAVE + 1, 32 * c->num_cu));
+      const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(n_tgt / HBTX_WAVE + 1, 32 * c->num_cu) /
+                                                     HBTX_WALK_WG_WAVES);
       if (P.periodic)
-        k_points_potential<true, false><<<grid, 64, 0, st>>>(A);
+        k_points_potential<true, false><<<grid, WALK_WG_THREADS, 0, st>>>(A);
       else
-        k_points_potential<false, false><<<grid, 64, 0, st>>>(A);
+        k_points_potential<false, false><<<grid, WALK_WG_THREADS, 0, st>>>(A);
     }
     HIPCHK(hipGetLastError());
   });

@@ -18,13 +18,16 @@ struct PointArgs
   double *phi_d;           // optional double output
   int32_t *counter;
   unsigned long long *stats;
+  int32_t *overflow;       // set to WALK_STACK_OVERFLOW if the walk stack was full, or null
   DevParams P;
 };
 
 template <bool PERIODIC, bool STATS>
-__global__ __launch_bounds__(64) void k_points_potential(PointArgs A)
+__global__ __launch_bounds__(WALK_WG_THREADS, 8) void k_points_potential(PointArgs A)
 {
   const int lane = lane_id();
+  __shared__ WalkLds wls[HBTX_WALK_WG_WAVES];
+  WalkLds &wl = wls[threadIdx.x / HBTX_WAVE];
   unsigned long long st_steps = 0, st_visits = 0, st_accepts = 0, st_pairs = 0, st_near = 0;
   const int nitems = (A.ntgt + HBTX_WAVE - 1) / HBTX_WAVE;
   for (;;)
@@ -47,7 +50,13 @@ __global__ __launch_bounds__(64) void k_points_potential(PointArgs A)
     if (A.ee > A.eb)
     {
       if (A.treemode)
+      {
+#ifdef HBTX_UNION_WALK
         tree_walk<PERIODIC, STATS>(A.wt, A.eb, A.ee, valid, x.x, x.y, x.z, A.P, acc, wc);
+#else
+        tree_walk_stack<PERIODIC, STATS>(A.wt, A.eb, A.ee, valid, x.x, x.y, x.z, A.P, acc, wc, wl, A.overflow);
+#endif
+      }
       else
       {
         direct_sum<PERIODIC>(A.wt, A.eb, A.ee, x.x, x.y, x.z, A.P, acc, wc);

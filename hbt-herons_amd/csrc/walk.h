@@ -59,6 +59,7 @@ struct WalkArgs
   float *eref;              // refinement energies (MODE_REFINE targets)
   float *phi_out;           // optional: specific potential per particle (float), may be null
   unsigned long long *stats;
+  int32_t *overflow;        // the round's overflow word (WALK_STACK_OVERFLOW)
   DevParams P;
 };
 
@@ -154,6 +155,216 @@ HBTX_DEV void tree_walk(const uint2 *wt, int ub, int ue, bool valid, float px, f
   }
 }
 
+// Batched stack walk: the same lane-exact opening decisions as tree_walk, with
+// the records a wave visits gathered 64 at a time instead of one dependent scalar
+// load per step.
+//
+// A stack entry (o, e, mask) stands for the sibling chain of records from byte
+// offset o up to e, and mask holds the lanes that reached it (that opened every
+// ancestor).  Processing record o: each reached lane applies its own opening
+// test; a lane accepts the record (adds its term) unless it opens it; the chain's
+// rest (o's skip, up to e) is pushed with the same mask, and an opened cell pushes
+// its children's chain (o + 24, skip) with the mask of the lanes that opened it.
+// A lane therefore accepts exactly the records its own walk accepts, and the wave
+// visits exactly the records some lane reaches - the steps of the union walk -
+// but a batch pops up to 64 entries, loads their records together (one vector
+// gather), stages them in LDS and then evaluates them back to back with no memory
+// access on the chain between one record's test and the next.
+//
+// Capacity: a batch of B entries pushes at most 2 B, so B is cut to keep the stack
+// WSTACK_HEADROOM below its capacity; once the room is gone single entries are
+// popped, a depth-first descent that adds at most one entry per level (the
+// chain's rest) plus the entry it stands on.  A tree has at most 22 cell levels
+// (21-bit keys per axis under the root), so the stack never holds more than
+// WSTACK_CAP - WSTACK_HEADROOM + 24 entries; a push beyond the capacity (never
+// taken) raises the overflow word instead of writing.
+constexpr int WSTACK_CAP = 128;
+constexpr int WSTACK_HEADROOM = 32;
+
+// Waves per walk workgroup.  Every wave walks on its own (its own items and its
+// own LDS stack); the workgroup only packs waves so that the CU's limit on
+// resident workgroups does not cap the resident waves of an LDS-using kernel.
+#ifndef HBTX_WALK_WG_WAVES
+#define HBTX_WALK_WG_WAVES 2
+#endif
+constexpr int WALK_WG_THREADS = HBTX_WAVE * HBTX_WALK_WG_WAVES;
+
+// m's bit for this lane ? a : b, with m a wave-uniform lane mask: one
+// v_cndmask on the mask's SGPR pair, where a select on a per-lane bool would first
+// turn the mask back into that bool with vector instructions.
+HBTX_DEV float lane_select(uint64_t m, float a, float b)
+{
+  float r;
+  asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(m));
+  return r;
+}
+
+// Orders one wave's LDS writes before its later LDS reads by other lanes: the
+// LDS serves a wave's instructions in order, so only the compiler's reordering
+// has to be held back (no workgroup barrier: the waves of a walk workgroup are
+// independent).
+HBTX_DEV void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct WalkLds
+{
+  int4 stk[WSTACK_CAP];  // (record offset, chain end, reached mask low, high word)
+  float4 rec[HBTX_WAVE]; // a batch's records: x, y, z, m (then the lanes that opened each)
+  float4 aux[HBTX_WAVE]; // crit2 (0 for a particle), mask low, crit2, mask high
+};
+
+#define WALK_STACK_OVERFLOW 4 // overflow-word bit: the walk stack was full (never expected)
+
+template <bool PERIODIC, bool STATS>
+HBTX_DEV void tree_walk_stack(const uint2 *wt, int ub, int ue, bool valid, float px, float py, float pz,
+                              const DevParams &P, Kahan &acc, WalkCounts &wc, WalkLds &L, int32_t *overflow)
+{
+  const uint2 *tree = wt + ub;
+  const int32_t span = 8 * (ue - ub);
+  const int lane = lane_id();
+  const uint64_t vmask = wave_mask(valid);
+  if (vmask == 0ull)
+    return;
+  // lanes 0-31 read (crit2, mask low), lanes 32-63 (crit2, mask high) of a staged record
+  const float2 *aw = reinterpret_cast<const float2 *>(L.aux) + (lane >> 5);
+  // the staged records and open masks through VGPR base addresses (a zero the
+  // compiler cannot see through), so that record j of an unrolled group is an
+  // immediate offset instead of an address moved from an SGPR per record
+  int vz;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(vz));
+  const float4 *rec = L.rec + vz;
+  uint2 *omw = reinterpret_cast<uint2 *>(L.rec) + vz;
+  const uint32_t bit = 1u << (lane & 31);
+  if (lane == 0)
+    L.stk[0] = make_int4(0, span, (int)(uint32_t)vmask, (int)(uint32_t)(vmask >> 32));
+  wave_lds_sync();
+  int top = 1;
+  float blk = 0.f;
+  int nblk = 0;
+  while (top > 0)
+  {
+    const int room = WSTACK_CAP - WSTACK_HEADROOM - top;
+    const int nb = uniform(min(min(HBTX_WAVE, top), max(1, room)));
+    top -= nb;
+    const bool mine = lane < nb;
+    int4 it = make_int4(0, 0, 0, 0);
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    float c2 = 0.f;
+    int32_t nx = 0;
+    bool cell = false;
+    if (mine)
+    {
+      it = L.stk[top + lane];
+      const uint2 *rp = tree + (it.x >> 3);
+      const uint2 u0 = rp[0], u1 = rp[1];
+      a = make_float4(__uint_as_float(u0.x), __uint_as_float(u0.y), __uint_as_float(u1.x),
+                      __uint_as_float(u1.y));
+      cell = (int32_t)u1.y < 0;
+      nx = it.x + 16;
+      if (cell)
+      {
+        const uint2 u2 = rp[2];
+        c2 = __uint_as_float(u2.x);
+        nx = (int32_t)u2.y;
+      }
+    }
+    L.rec[lane] = make_float4(a.x, a.y, a.z, fabsf(a.w));
+    L.aux[lane] = make_float4(c2, __int_as_float(it.z), c2, __int_as_float(it.w));
+    wave_lds_sync();
+    // Record j of the batch against every lane's target; returns the lanes that
+    // opened it.  The lane predicates are single compares whose masks the scalar
+    // unit combines: reached R, opened Lt (r2 < crit2), near N (r2 < h2).  An
+    // accepted lane (R, not Lt) adds -m/r, or the spline where N (a lane that
+    // does not accept never uses its 1/r, which is infinite where the record is
+    // the target itself).
+    auto visit = [&](int j) -> uint64_t {
+      const float4 r = rec[j];
+      const float2 w = aw[2 * j];
+      const float r2 = sep2<PERIODIC>(r.x, r.y, r.z, px, py, pz, P);
+      const uint64_t rm = __builtin_amdgcn_uicmp(__float_as_uint(w.y) & bit, 0u, 33 /* ne */);
+      const uint64_t lm = __builtin_amdgcn_fcmpf(r2, w.x, 4 /* ordered < */);
+      const uint64_t nm = __builtin_amdgcn_fcmpf(r2, P.h2, 4 /* ordered < */);
+      const float rinv = __builtin_amdgcn_rsqf(r2);
+      blk = lane_select(rm & ~(lm | nm), fmaf(-r.w, rinv, blk), blk);
+      const uint64_t na = rm & nm & ~lm;
+      if (na != 0ull)
+      {
+        if (STATS)
+          wc.nnear++;
+        const float tn = spline_near(r2, rinv, r.w, P.hinv);
+        blk += lane_select(na, tn, 0.f);
+      }
+      if (STATS)
+      {
+        wc.steps++;
+        wc.visits += __popcll(rm);
+        wc.accepts += __popcll(rm & ~lm);
+      }
+      return rm & lm;
+    };
+    // The open masks go to the slots of records already read (record j's first
+    // 8 bytes); plain FP32 sums over blocks of at least 16 records, Kahan-
+    // compensated across blocks.
+    int j = 0;
+    for (; j + 4 <= nb; j += 4)
+    {
+      const uint64_t o0 = visit(j), o1 = visit(j + 1), o2 = visit(j + 2), o3 = visit(j + 3);
+      if (lane == 0)
+      {
+        omw[2 * j] = make_uint2((uint32_t)o0, (uint32_t)(o0 >> 32));
+        omw[2 * j + 2] = make_uint2((uint32_t)o1, (uint32_t)(o1 >> 32));
+        omw[2 * j + 4] = make_uint2((uint32_t)o2, (uint32_t)(o2 >> 32));
+        omw[2 * j + 6] = make_uint2((uint32_t)o3, (uint32_t)(o3 >> 32));
+      }
+      nblk += 4;
+      if (nblk >= 16)
+      {
+        acc.add(blk);
+        blk = 0.f;
+        nblk = 0;
+      }
+    }
+    for (; j < nb; j++)
+    {
+      const uint64_t o = visit(j);
+      if (lane == 0)
+        omw[2 * j] = make_uint2((uint32_t)o, (uint32_t)(o >> 32));
+      if (++nblk >= 16)
+      {
+        acc.add(blk);
+        blk = 0.f;
+        nblk = 0;
+      }
+    }
+    // push the chains' rests and the opened cells' children
+    wave_lds_sync();
+    const uint2 om = reinterpret_cast<const uint2 *>(L.rec)[2 * lane];
+    const bool sib = mine && nx < it.y;
+    const bool kid = mine && cell && (om.x | om.y) != 0u;
+    const uint64_t msib = wave_mask(sib), mkid = wave_mask(kid);
+    const uint64_t lt = lanemask_lt();
+    int pos = top + __popcll(msib & lt) + __popcll(mkid & lt);
+    const int pushed = __popcll(msib) + __popcll(mkid);
+    if (top + pushed > WSTACK_CAP)
+    {
+      if (lane == 0 && overflow)
+        atomicOr(overflow, WALK_STACK_OVERFLOW);
+      break;
+    }
+    if (sib)
+      L.stk[pos++] = make_int4(nx, it.y, it.z, it.w);
+    if (kid)
+      L.stk[pos] = make_int4(it.x + 24, nx, (int)om.x, (int)om.y);
+    top = uniform(top + pushed);
+    wave_lds_sync();
+  }
+  acc.add(blk);
+}
+
 template <bool PERIODIC>
 HBTX_DEV float direct_term(const uint2 *wt, int i, float px, float py, float pz, const DevParams &P, bool &near)
 {
@@ -214,10 +425,12 @@ HBTX_DEV void rel_vel(const DevParams &P, float x0, float x1, float x2, float v0
 }
 
 template <bool PERIODIC, bool STATS>
-__global__ __launch_bounds__(64) void k_walk_energy(WalkArgs A)
+__global__ __launch_bounds__(WALK_WG_THREADS, 8) void k_walk_energy(WalkArgs A)
 {
   const DevParams &P = A.P;
   const int lane = lane_id();
+  __shared__ WalkLds wls[HBTX_WALK_WG_WAVES];
+  WalkLds &wl = wls[threadIdx.x / HBTX_WAVE];
   unsigned long long st_steps = 0, st_visits = 0, st_accepts = 0, st_pairs = 0, st_targets = 0, st_items = 0,
                      st_near = 0;
   // XCD-aware work distribution: workgroups are dispatched round-robin over the
@@ -266,7 +479,13 @@ __global__ __launch_bounds__(64) void k_walk_energy(WalkArgs A)
     if (eb < ee)
     {
       if (A.treemode[s])
+      {
+#ifdef HBTX_UNION_WALK
         tree_walk<PERIODIC, STATS>(A.wt, eb, ee, valid, x.x, x.y, x.z, P, acc, wc);
+#else
+        tree_walk_stack<PERIODIC, STATS>(A.wt, eb, ee, valid, x.x, x.y, x.z, P, acc, wc, wl, A.overflow);
+#endif
+      }
       else
       {
         direct_sum<PERIODIC>(A.wt, eb, ee, x.x, x.y, x.z, P, acc, wc);
