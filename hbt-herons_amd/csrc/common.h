@@ -104,34 +104,7 @@ HBTX_DEV int2 const_load(const int2 *p, int i)
 // starts at unit 2 j + 3 cscan[j].  A particle's skip is the next record and it
 // is never opened.  Packing the particles into 16 bytes shrinks a tree's walk
 // footprint by almost half against the 32-byte entries.
-struct WRec
-{
-  float x, y, z, m, crit2;
-  int32_t rskip;
-  bool cell;
-};
-// Two scalar loads at a wave-uniform byte offset (8-byte aligned): the record's
-// first 16 bytes, and the 8 bytes after them (a cell's crit2 and rskip).
-HBTX_DEV WRec const_load_wrec(const uint2 *tree, uint32_t off)
-{
-  typedef int v4i __attribute__((ext_vector_type(4)));
-  typedef int v2i __attribute__((ext_vector_type(2)));
-  typedef const __attribute__((address_space(4))) char cc;
-  typedef const __attribute__((address_space(4))) v4i c4;
-  typedef const __attribute__((address_space(4))) v2i c2;
-  const cc *base = (const cc *)(uintptr_t)tree + off;
-  const v4i a = *(const c4 *)base;
-  const v2i b = *(const c2 *)(base + 16);
-  WRec e;
-  e.x = __int_as_float(a[0]);
-  e.y = __int_as_float(a[1]);
-  e.z = __int_as_float(a[2]);
-  e.cell = a[3] < 0; // sign bit of m
-  e.m = __int_as_float(a[3] & 0x7fffffff);
-  e.crit2 = e.cell ? __int_as_float(b[0]) : 0.f;
-  e.rskip = e.cell ? b[1] : (int32_t)off + 16;
-  return e;
-}
+
 // a particle record (x, y, z, m) at an 8-byte aligned unit, through the scalar path
 HBTX_DEV float4 const_load_f4u(const uint2 *p)
 {

@@ -51,11 +51,7 @@ __global__ __launch_bounds__(WALK_WG_THREADS, 8) void k_points_potential(PointAr
     {
       if (A.treemode)
       {
-#ifdef HBTX_UNION_WALK
-        tree_walk<PERIODIC, STATS>(A.wt, A.eb, A.ee, valid, x.x, x.y, x.z, A.P, acc, wc);
-#else
         tree_walk_stack<PERIODIC, STATS>(A.wt, A.eb, A.ee, valid, x.x, x.y, x.z, A.P, acc, wc, wl, A.overflow);
-#endif
       }
       else
       {

@@ -1,16 +1,12 @@
 // Potential + binding-energy kernel (one wave = 64 targets of one subhalo).
 //
-// Tree mode: wavefront-cooperative walk of the depth-first walk records (16-byte
-// particles, 24-byte cells).  The wave visits records in preorder at a
-// wave-uniform byte offset (scalar loads).  Each
-// lane applies the reference's own opening test to its own target
-// (open if len^2 > r^2 theta^2 with r to the cell's centre of mass,
-// src/gravity_tree.cpp:139, stored as r^2 < crit2 = len^2/theta^2) and keeps a
-// private "skip" bound: after accepting an entry it ignores that entry's
-// subtree.  The wave descends (index+1) if any active lane opened the entry,
-// otherwise it jumps to the entry's skip.  Every lane therefore sees exactly the
-// entries its own stackless walk would see, in the same order, while lanes on
-// spatially adjacent targets share the traversal.
+// Tree mode: each lane applies the reference's own opening test to its own
+// target (open if len^2 > r^2 theta^2 with r to the cell's centre of mass,
+// src/gravity_tree.cpp:139, stored as r^2 < crit2 = len^2/theta^2) over the
+// depth-first walk records (16-byte particles, 24-byte cells), so that every lane
+// accepts exactly the records its own walk accepts (src/gravity_tree.cpp:82-166),
+// while the 64 lanes on spatially adjacent targets share one traversal: the
+// batched stack walk below (tree_walk_stack).
 //
 // Direct mode: the entry list holds particles only and every pair is summed.
 //
@@ -95,69 +91,8 @@ struct WalkCounts
   uint32_t steps, visits, accepts, nnear;
 };
 
-// Walk the records of units [ub, ue) for this lane's target, accumulating the
-// pair terms (-m/r outside h, the spline inside): plain FP32 sums over blocks of
-// 16 steps, Kahan-compensated across blocks.  The wave position is a byte offset
-// from the tree's first record (an SGPR, the scalar loads' offset); cells carry
-// their skip in the same units.
-template <bool PERIODIC, bool STATS>
-HBTX_DEV void tree_walk(const uint2 *wt, int ub, int ue, bool valid, float px, float py, float pz,
-                        const DevParams &P, Kahan &acc, WalkCounts &wc)
-{
-  const uint2 *tree = wt + ub;
-  const int32_t span = 8 * (ue - ub);
-  int32_t off = 0;
-  int32_t myskip = valid ? 0 : 0x7fffffff;
-  // (offsets stay below 2^31: k_sub_entries leaves a tree over WALK_MAX_SPAN_UNITS units empty
-  // and reports it)
-  for (;;)
-  {
-    float blk = 0.f;
-#pragma unroll
-    for (int k = 0; k < 16; k++)
-    {
-      const WRec e = const_load_wrec(tree, (uint32_t)off);
-      const bool active = off >= myskip;
-      const float r2 = sep2<PERIODIC>(e.x, e.y, e.z, px, py, pz, P);
-      const bool open = r2 < e.crit2;
-      const bool accept = active && !open;
-      // lane masks straight from the compares (no round trip through a VGPR)
-      const bool anyopen = (__builtin_amdgcn_sicmp(off, myskip, 39 /* signed >= */) &
-                            __builtin_amdgcn_fcmpf(r2, e.crit2, 4 /* ordered < */)) != 0ull;
-      const float rinv = __builtin_amdgcn_rsqf(r2);
-      float term = -e.m * rinv;
-      if (wave_mask(accept && r2 < P.h2))
-      {
-        if (STATS)
-          wc.nnear++;
-        const float tn = spline_near(r2, rinv, e.m, P.hinv);
-        term = (r2 < P.h2) ? tn : term;
-      }
-      blk += accept ? term : 0.f;
-      myskip = accept ? e.rskip : myskip;
-      if (STATS)
-      {
-        wc.steps++;
-        wc.visits += __popcll(wave_mask(active));
-        wc.accepts += __popcll(wave_mask(accept));
-      }
-      // records of a subtree follow it, so rskip > off; max() keeps the walk finite regardless
-      // (only a cell is ever opened: the next record then follows its 24 bytes)
-      const int32_t next = off + (e.cell ? 24 : 16);
-      off = uniform(anyopen ? next : max(e.rskip, next));
-      if (off >= span)
-      {
-        acc.add(blk);
-        return;
-      }
-    }
-    acc.add(blk);
-  }
-}
-
-// Batched stack walk: the same lane-exact opening decisions as tree_walk, with
-// the records a wave visits gathered 64 at a time instead of one dependent scalar
-// load per step.
+// Batched stack walk: lane-exact opening decisions, with the records a wave
+// visits gathered 64 at a time instead of one dependent load per record.
 //
 // A stack entry (o, e, mask) stands for the sibling chain of records from byte
 // offset o up to e, and mask holds the lanes that reached it (that opened every
@@ -166,8 +101,8 @@ HBTX_DEV void tree_walk(const uint2 *wt, int ub, int ue, bool valid, float px, f
 // rest (o's skip, up to e) is pushed with the same mask, and an opened cell pushes
 // its children's chain (o + 24, skip) with the mask of the lanes that opened it.
 // A lane therefore accepts exactly the records its own walk accepts, and the wave
-// visits exactly the records some lane reaches - the steps of the union walk -
-// but a batch pops up to 64 entries, loads their records together (one vector
+// visits exactly the records some lane reaches (the union of its lanes' walks);
+// a batch pops up to 64 entries, loads their records together (one vector
 // gather), stages them in LDS and then evaluates them back to back with no memory
 // access on the chain between one record's test and the next.
 //
@@ -199,6 +134,21 @@ HBTX_DEV float lane_select(uint64_t m, float a, float b)
   return r;
 }
 
+// acc = -a * b + acc on the lanes of the wave-uniform mask m, the others keeping
+// acc: one FMA under a narrowed exec mask (restored in the same block), where a
+// select would take a second vector instruction.
+HBTX_DEV void fma_neg_on(uint64_t m, float &acc, float a, float b)
+{
+  uint64_t save;
+  asm volatile("s_mov_b64 %1, exec\n\t"
+               "s_and_b64 exec, exec, %4\n\t"
+               "v_fma_f32 %0, -%2, %3, %0\n\t"
+               "s_mov_b64 exec, %1"
+               : "+v"(acc), "=&s"(save)
+               : "v"(a), "v"(b), "s"(m)
+               : "scc");
+}
+
 // Orders one wave's LDS writes before its later LDS reads by other lanes: the
 // LDS serves a wave's instructions in order, so only the compiler's reordering
 // has to be held back (no workgroup barrier: the waves of a walk workgroup are
@@ -213,8 +163,9 @@ HBTX_DEV void wave_lds_sync()
 struct WalkLds
 {
   int4 stk[WSTACK_CAP];  // (record offset, chain end, reached mask low, high word)
-  float4 rec[HBTX_WAVE]; // a batch's records: x, y, z, m (then the lanes that opened each)
-  float4 aux[HBTX_WAVE]; // crit2 (0 for a particle), mask low, crit2, mask high
+  float4 rec[HBTX_WAVE + 1]; // a batch's records: x, y, z, m (then the lanes that opened each)
+  float4 aux[HBTX_WAVE + 1]; // crit2 (0 for a particle), mask low, crit2, mask high
+  // (one spare slot each: the record after a batch's last is read ahead and not used)
 };
 
 #define WALK_STACK_OVERFLOW 4 // overflow-word bit: the walk stack was full (never expected)
@@ -259,37 +210,49 @@ HBTX_DEV void tree_walk_stack(const uint2 *wt, int ub, int ue, bool valid, float
     if (mine)
     {
       it = L.stk[top + lane];
+      // 24 bytes whatever the record: a particle's last 8 are the next record's
+      // (or the spare units past the records' end) and are not used, so that
+      // both loads go out together
       const uint2 *rp = tree + (it.x >> 3);
-      const uint2 u0 = rp[0], u1 = rp[1];
+      const uint2 u0 = rp[0], u1 = rp[1], u2 = rp[2];
       a = make_float4(__uint_as_float(u0.x), __uint_as_float(u0.y), __uint_as_float(u1.x),
                       __uint_as_float(u1.y));
       cell = (int32_t)u1.y < 0;
-      nx = it.x + 16;
-      if (cell)
-      {
-        const uint2 u2 = rp[2];
-        c2 = __uint_as_float(u2.x);
-        nx = (int32_t)u2.y;
-      }
+      c2 = cell ? __uint_as_float(u2.x) : 0.f;
+      nx = cell ? (int32_t)u2.y : it.x + 16;
     }
-    L.rec[lane] = make_float4(a.x, a.y, a.z, fabsf(a.w));
-    L.aux[lane] = make_float4(c2, __int_as_float(it.z), c2, __int_as_float(it.w));
+    // staged cells first, then particles (the order within a batch is free): a
+    // particle is never opened, so its visit skips the opening test and mask
+    const uint64_t lt = lanemask_lt();
+    const uint64_t cmask = wave_mask(mine && cell), pmask = wave_mask(mine && !cell);
+    const int ncell = __popcll(cmask);
+    const int slot = cell ? __popcll(cmask & lt) : ncell + __popcll(pmask & lt);
+    if (mine)
+    {
+      L.rec[slot] = make_float4(a.x, a.y, a.z, fabsf(a.w));
+      L.aux[slot] = make_float4(c2, __int_as_float(it.z), c2, __int_as_float(it.w));
+    }
     wave_lds_sync();
-    // Record j of the batch against every lane's target; returns the lanes that
-    // opened it.  The lane predicates are single compares whose masks the scalar
-    // unit combines: reached R, opened Lt (r2 < crit2), near N (r2 < h2).  An
-    // accepted lane (R, not Lt) adds -m/r, or the spline where N (a lane that
-    // does not accept never uses its 1/r, which is infinite where the record is
-    // the target itself).
-    auto visit = [&](int j) -> uint64_t {
-      const float4 r = rec[j];
-      const float2 w = aw[2 * j];
+    // Record j of the batch against every lane's target.  The lane predicates are
+    // single compares whose masks the scalar unit combines: reached R, opened Lt
+    // (r2 < crit2), near N (r2 < h2).  An accepted lane (R, not Lt) adds -m/r,
+    // or the spline where N; the sum runs under the accepting lanes' exec mask,
+    // so that a lane that does not accept never uses its 1/r (infinite where the
+    // record is the target itself).  A cell's visit returns the lanes that opened it.
+    float4 rn = rec[0];
+    float2 wn = aw[0];
+    auto visit = [&](int j, bool is_cell) -> uint64_t {
+      // record j was read ahead; read record j + 1 while it is evaluated
+      const float4 r = rn;
+      const float2 w = wn;
+      rn = rec[j + 1];
+      wn = aw[2 * (j + 1)];
       const float r2 = sep2<PERIODIC>(r.x, r.y, r.z, px, py, pz, P);
       const uint64_t rm = __builtin_amdgcn_uicmp(__float_as_uint(w.y) & bit, 0u, 33 /* ne */);
-      const uint64_t lm = __builtin_amdgcn_fcmpf(r2, w.x, 4 /* ordered < */);
+      const uint64_t lm = is_cell ? __builtin_amdgcn_fcmpf(r2, w.x, 4 /* ordered < */) : 0ull;
       const uint64_t nm = __builtin_amdgcn_fcmpf(r2, P.h2, 4 /* ordered < */);
       const float rinv = __builtin_amdgcn_rsqf(r2);
-      blk = lane_select(rm & ~(lm | nm), fmaf(-r.w, rinv, blk), blk);
+      fma_neg_on(rm & ~(lm | nm), blk, r.w, rinv);
       const uint64_t na = rm & nm & ~lm;
       if (na != 0ull)
       {
@@ -306,13 +269,24 @@ HBTX_DEV void tree_walk_stack(const uint2 *wt, int ub, int ue, bool valid, float
       }
       return rm & lm;
     };
-    // The open masks go to the slots of records already read (record j's first
-    // 8 bytes); plain FP32 sums over blocks of at least 16 records, Kahan-
-    // compensated across blocks.
+    // plain FP32 sums over blocks of at least 16 records, Kahan-compensated
+    // across blocks
+    auto block_end = [&](int n) {
+      nblk += n;
+      if (nblk >= 16)
+      {
+        acc.add(blk);
+        blk = 0.f;
+        nblk = 0;
+      }
+    };
+    // cells: the open masks go to the slots of records already read (record j's
+    // first 8 bytes)
     int j = 0;
-    for (; j + 4 <= nb; j += 4)
+    for (; j + 4 <= ncell; j += 4)
     {
-      const uint64_t o0 = visit(j), o1 = visit(j + 1), o2 = visit(j + 2), o3 = visit(j + 3);
+      const uint64_t o0 = visit(j, true), o1 = visit(j + 1, true), o2 = visit(j + 2, true),
+                     o3 = visit(j + 3, true);
       if (lane == 0)
       {
         omw[2 * j] = make_uint2((uint32_t)o0, (uint32_t)(o0 >> 32));
@@ -320,33 +294,39 @@ HBTX_DEV void tree_walk_stack(const uint2 *wt, int ub, int ue, bool valid, float
         omw[2 * j + 4] = make_uint2((uint32_t)o2, (uint32_t)(o2 >> 32));
         omw[2 * j + 6] = make_uint2((uint32_t)o3, (uint32_t)(o3 >> 32));
       }
-      nblk += 4;
-      if (nblk >= 16)
-      {
-        acc.add(blk);
-        blk = 0.f;
-        nblk = 0;
-      }
+      block_end(4);
+    }
+    for (; j < ncell; j++)
+    {
+      const uint64_t o = visit(j, true);
+      if (lane == 0)
+        omw[2 * j] = make_uint2((uint32_t)o, (uint32_t)(o >> 32));
+      block_end(1);
+    }
+    // particles
+    for (; j + 4 <= nb; j += 4)
+    {
+      visit(j, false);
+      visit(j + 1, false);
+      visit(j + 2, false);
+      visit(j + 3, false);
+      block_end(4);
     }
     for (; j < nb; j++)
     {
-      const uint64_t o = visit(j);
-      if (lane == 0)
-        omw[2 * j] = make_uint2((uint32_t)o, (uint32_t)(o >> 32));
-      if (++nblk >= 16)
-      {
-        acc.add(blk);
-        blk = 0.f;
-        nblk = 0;
-      }
+      visit(j, false);
+      block_end(1);
     }
     // push the chains' rests and the opened cells' children
     wave_lds_sync();
-    const uint2 om = reinterpret_cast<const uint2 *>(L.rec)[2 * lane];
+    // (the popped entries are still in place: nothing is pushed before this point)
+    if (mine)
+      it = L.stk[top + lane];
+    const bool is_cell = slot < ncell;
+    const uint2 om = reinterpret_cast<const uint2 *>(L.rec)[2 * slot];
     const bool sib = mine && nx < it.y;
-    const bool kid = mine && cell && (om.x | om.y) != 0u;
+    const bool kid = mine && is_cell && (om.x | om.y) != 0u;
     const uint64_t msib = wave_mask(sib), mkid = wave_mask(kid);
-    const uint64_t lt = lanemask_lt();
     int pos = top + __popcll(msib & lt) + __popcll(mkid & lt);
     const int pushed = __popcll(msib) + __popcll(mkid);
     if (top + pushed > WSTACK_CAP)
@@ -366,9 +346,8 @@ HBTX_DEV void tree_walk_stack(const uint2 *wt, int ub, int ue, bool valid, float
 }
 
 template <bool PERIODIC>
-HBTX_DEV float direct_term(const uint2 *wt, int i, float px, float py, float pz, const DevParams &P, bool &near)
+HBTX_DEV float direct_term(const float4 e, float px, float py, float pz, const DevParams &P, bool &near)
 {
-  const float4 e = const_load_f4u(wt + 2 * i);
   const float r2 = sep2<PERIODIC>(e.x, e.y, e.z, px, py, pz, P);
   const float rinv = __builtin_amdgcn_rsqf(r2);
   float term = -e.w * rinv;
@@ -382,7 +361,20 @@ HBTX_DEV float direct_term(const uint2 *wt, int i, float px, float py, float pz,
   return term;
 }
 
-// Direct mode: the records are particles only (two units each).
+// Four particle records (64 bytes at an 8-byte aligned unit) in one scalar load.
+HBTX_DEV void const_load_4rec(const uint2 *p, float4 r[4])
+{
+  typedef int v16i __attribute__((ext_vector_type(16)));
+  typedef const __attribute__((address_space(4))) v16i c16;
+  const v16i a = *(const c16 *)(uintptr_t)p;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    r[k] = make_float4(__int_as_float(a[4 * k]), __int_as_float(a[4 * k + 1]), __int_as_float(a[4 * k + 2]),
+                       __int_as_float(a[4 * k + 3]));
+}
+
+// Direct mode: the records are particles only (two units each), summed in
+// groups of four (one scalar load each), Kahan-compensated across groups.
 template <bool PERIODIC>
 HBTX_DEV void direct_sum(const uint2 *wt, int ub, int ue, float px, float py, float pz, const DevParams &P,
                          Kahan &acc, WalkCounts &wc)
@@ -392,18 +384,20 @@ HBTX_DEV void direct_sum(const uint2 *wt, int ub, int ue, float px, float py, fl
   int i = 0;
   for (; i + 4 <= ee; i += 4)
   {
+    float4 r[4];
+    const_load_4rec(ent + 2 * i, r);
     bool nr = false;
-    float part = direct_term<PERIODIC>(ent, i, px, py, pz, P, nr);
-    part += direct_term<PERIODIC>(ent, i + 1, px, py, pz, P, nr);
-    part += direct_term<PERIODIC>(ent, i + 2, px, py, pz, P, nr);
-    part += direct_term<PERIODIC>(ent, i + 3, px, py, pz, P, nr);
+    float part = direct_term<PERIODIC>(r[0], px, py, pz, P, nr);
+    part += direct_term<PERIODIC>(r[1], px, py, pz, P, nr);
+    part += direct_term<PERIODIC>(r[2], px, py, pz, P, nr);
+    part += direct_term<PERIODIC>(r[3], px, py, pz, P, nr);
     wc.nnear += nr;
     acc.add(part);
   }
   for (; i < ee; i++)
   {
     bool nr = false;
-    acc.add(direct_term<PERIODIC>(ent, i, px, py, pz, P, nr));
+    acc.add(direct_term<PERIODIC>(const_load_f4u(ent + 2 * i), px, py, pz, P, nr));
   }
 }
 
@@ -480,11 +474,7 @@ __global__ __launch_bounds__(WALK_WG_THREADS, 8) void k_walk_energy(WalkArgs A)
     {
       if (A.treemode[s])
       {
-#ifdef HBTX_UNION_WALK
-        tree_walk<PERIODIC, STATS>(A.wt, eb, ee, valid, x.x, x.y, x.z, P, acc, wc);
-#else
         tree_walk_stack<PERIODIC, STATS>(A.wt, eb, ee, valid, x.x, x.y, x.z, P, acc, wc, wl, A.overflow);
-#endif
       }
       else
       {
