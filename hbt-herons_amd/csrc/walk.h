@@ -113,7 +113,7 @@ struct WalkCounts
 // (21-bit keys per axis under the root), so the stack never holds more than
 // WSTACK_CAP - WSTACK_HEADROOM + 24 entries; a push beyond the capacity (never
 // taken) raises the overflow word instead of writing.
-constexpr int WSTACK_CAP = 128;
+constexpr int WSTACK_CAP = 160;
 constexpr int WSTACK_HEADROOM = 32;
 
 // Waves per walk workgroup.  Every wave walks on its own (its own items and its
