@@ -258,7 +258,10 @@ HBTX_DEV void tree_walk_stack(const uint2 *wt, int ub, int ue, bool valid, float
       {
         if (STATS)
           wc.nnear++;
-        const float tn = spline_near(r2, rinv, r.w, P.hinv);
+        // at zero distance (the target's own record, or a coincident source) the
+        // spline is its value at u = 0, m hinv (-2.8), as spline_near forms it
+        const uint64_t zm = __builtin_amdgcn_fcmpf(r2, 0.f, 1 /* ordered == */);
+        const float tn = ((na & ~zm) == 0ull) ? (r.w * P.hinv) * -2.8f : spline_near(r2, rinv, r.w, P.hinv);
         blk += lane_select(na, tn, 0.f);
       }
       if (STATS)

@@ -114,6 +114,26 @@ def test_theta_zero_walk_equals_fp64_direct_sum(ctx):
     assert rel.max() < 2e-6, rel.max()
 
 
+def test_walk_stack_deepest_trees_theta_zero(ctx):
+    """theta = 0 opens every cell for every target, so each wave's walk stack
+    holds the most entries a tree allows: here trees at the key depth limit (a
+    clump of 300 coincident particles, chains of single-child cells down to
+    depth ~19) beside a uniform field.  The walk must finish (no capacity error)
+    and equal the FP64 direct sum."""
+    rng = np.random.default_rng(11)
+    u = rng.uniform(0, 1, (2500, 3)).astype(np.float32)
+    chains = (u[:300] + np.float32(3e-6)).astype(np.float32)
+    clump = np.repeat(np.array([[0.25, 0.5, 0.75]], np.float32), 300, axis=0)
+    pos = np.concatenate([u, chains, clump])
+    mass = rng.uniform(0.5, 1.5, len(pos)).astype(np.float32)
+    prm = dict(theta=0.0, softening=0.001)
+    ref = O.direct_potential(pos, mass, pos, mass, params=prm)
+    t = ctx.tree(pos, mass, hbtx.make_params(**prm))
+    got = t.potential(pos, mass)
+    rel = np.abs(got - ref) / np.abs(ref)
+    assert rel.max() < 2e-6, rel.max()
+
+
 def test_direct_potential_matches_fp64(ctx):
     p = synth.single_nfw(1, n_halo=4000, n_inter=800)
     ref = O.direct_potential(p.pos, p.mass, p.pos, p.mass, params=PRM)
