@@ -9,6 +9,7 @@
 #pragma once
 #include "common.h"
 #include "scan.h"
+#include "hilbert.h"
 
 namespace hbtx
 {
@@ -715,8 +716,10 @@ __global__ void k_warm_keys(SubArrays S, PartArrays Q, int64_t n, uint64_t *keys
       const float4 x = Q.posm[Q.cur[q]];
       const double *c = S.cube + 4 * s;
       // the top 10 levels are enough to group targets for the walk (cells below
-      // 1/1024 of the cube hold particles closer than the softening)
-      keys[q] = octree_key(x.x, x.y, x.z, c[0], c[1], c[2], c[3]) >> (3 * (HBTX_KEY_LEVELS - 10));
+      // 1/1024 of the cube hold particles closer than the softening), along the
+      // Hilbert curve, whose runs of 64 cells are more compact than the Morton
+      // curve's
+      keys[q] = hilbert3_10(octree_key(x.x, x.y, x.z, c[0], c[1], c[2], c[3]) >> (3 * (HBTX_KEY_LEVELS - 10)));
     }
   }
 }
