@@ -1254,7 +1254,7 @@ void round_head(hbtx_ctx *c, Batch &B, View &V, const DevParams &P, bool graph, 
   round_sources_and_tree(c, B, V, P, graph, level);
   if (warm_round)
   {
-    // warm round: every target in key order of the proposal's tree cube
+    // warm round: every target, in Hilbert order of its cell in the proposal's tree cube
     k_warm_keys<<<blocks_for(V.n), 256, 0, st>>>(V.S, V.Q, V.n, B.keys.p, B.seg_begin.p, B.seg_len.p);
     k_warm_segments<<<gs, 256, 0, st>>>(V.S, B.seg_begin.p, B.seg_len.p);
     seg_sort(c, B.keys.p, V.Q.cur, V.nsub, B.seg_begin.p, B.seg_len.p, V.n, 30);
