@@ -117,8 +117,8 @@ constexpr int WSTACK_CAP = 160;
 constexpr int WSTACK_HEADROOM = 32;
 
 // Waves per walk workgroup.  Every wave walks on its own (its own items and its
-// own LDS stack); the workgroup only packs waves so that the CU's limit on
-// resident workgroups does not cap the resident waves of an LDS-using kernel.
+// own LDS stack, no workgroup barrier); one, two and four waves a workgroup
+// time the same, as a CU holds 32 one-wave workgroups with this LDS size.
 #ifndef HBTX_WALK_WG_WAVES
 #define HBTX_WALK_WG_WAVES 2
 #endif
